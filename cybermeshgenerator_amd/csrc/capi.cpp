@@ -1,0 +1,389 @@
+// capi.cpp — the C ABI of libcmg_hip (include/cmg_hip.h).
+//
+// Handle semantics, status codes and errbuf behaviour follow the reference's stable C
+// ABI (bindings/c/src/cmg_c.cpp:23-52, 79-106, 242-291): handles are created with new
+// and released by *_destroy, *out is cleared first, errbuf receives a NUL-terminated
+// message, and no C++ exception crosses the boundary. All computation runs on the GPU
+// through the launchers in cmg_hip_internal.hpp; without a HIP device every compute
+// entry point fails with CMG_ERR_INTERNAL.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cmg_hip.h"
+#include "cmg_hip_internal.hpp"
+#include "device_common.hpp"
+
+struct cmg_plc {
+    std::vector<double> xyz;            // 3 per point
+    std::vector<std::vector<int>> polys;  // one polygon per facet (cmg_c.cpp:88-100)
+    std::vector<int> markers;
+};
+
+struct cmg_voxels {
+    cmg::hip::GridSpec grid;
+    int mode = 0;                        // 0 occupancy, 1 signed distance
+    std::vector<unsigned char> occupancy;  // exposed in occupancy mode only
+    std::vector<float> distance;           // exposed in SDF mode only
+    std::vector<unsigned char> inside;     // SDF mode: the sign bits, kept for backward
+};
+
+namespace {
+
+thread_local const char* g_last_backend = "none";
+
+void set_err(char* buf, size_t len, const std::string& msg) {
+    if (buf && len) {
+        std::strncpy(buf, msg.c_str(), len - 1);
+        buf[len - 1] = '\0';
+    }
+}
+
+template <class F>
+cmg_status guarded(char* errbuf, size_t len, F&& f) {
+    try {
+        f();
+        return CMG_OK;
+    } catch (const cmg::hip::InvalidInput& e) {
+        set_err(errbuf, len, e.what());
+        return CMG_ERR_INVALID_INPUT;
+    } catch (const std::exception& e) {
+        set_err(errbuf, len, e.what());
+        return CMG_ERR_INTERNAL;
+    } catch (...) {
+        set_err(errbuf, len, "unknown error");
+        return CMG_ERR_INTERNAL;
+    }
+}
+
+void require_device() {
+    if (!cmg::hip::device_available())
+        throw cmg::hip::HipError("no usable HIP device: libcmg_hip has no CPU path");
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// Fan-triangulate every polygon: (v0, v[i-1], v[i]) (src/voxelize/voxelize.cpp:29-38).
+std::vector<int> fan_triangles(const cmg_plc& p) {
+    std::vector<int> t;
+    for (const auto& v : p.polys)
+        for (size_t i = 2; i < v.size(); ++i) {
+            t.push_back(v[0]);
+            t.push_back(v[i - 1]);
+            t.push_back(v[i]);
+        }
+    return t;
+}
+
+// Host-pointer batched signs: stage inputs in the library scratch, run, copy back.
+void host_signs(int npts_per, int mode, const double* xyz, size_t npts, const int* idx,
+                size_t n, int8_t* out) {
+    require_device();
+    if (mode != cmg::hip::kModeExact && mode != cmg::hip::kModeCompat)
+        throw cmg::hip::InvalidInput("mode must be CMG_PRED_EXACT (0) or CMG_PRED_COMPAT (1)");
+    if (n && (!xyz || !idx || !out)) throw cmg::hip::InvalidInput("null pointer");
+    for (size_t i = 0; i < n * (size_t)npts_per; ++i)
+        if (idx[i] < 0 || (size_t)idx[i] >= npts)
+            throw cmg::hip::InvalidInput("point index out of range");
+    g_last_backend = "hip";
+    if (n == 0) return;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 1);
+    size_t wsb = cmg::hip::pred_workspace_bytes(plan, n);
+    size_t b_xyz = al(npts * 3 * sizeof(double)), b_idx = al(n * npts_per * sizeof(int)),
+           b_out = al(n);
+    cmg::hip::ScratchLease lease;
+    char* base = static_cast<char*>(lease.s->get(b_xyz + b_idx + b_out + wsb));
+    double* d_xyz = reinterpret_cast<double*>(base);
+    int* d_idx = reinterpret_cast<int*>(base + b_xyz);
+    int8_t* d_out = reinterpret_cast<int8_t*>(base + b_xyz + b_idx);
+    void* d_ws = base + b_xyz + b_idx + b_out;
+    hipStream_t st = nullptr;
+    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_idx, idx, n * npts_per * sizeof(int), hipMemcpyHostToDevice, st));
+    cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(d_ws, plan, n);
+    cmg::hip::launch_signs_indexed(npts_per, mode, d_xyz, d_idx, n, d_out, plan, ws, st);
+    CMG_HIP(hipMemcpyAsync(out, d_out, n, hipMemcpyDeviceToHost, st));
+    cmg::hip::PredCounters c{};
+    CMG_HIP(hipMemcpyAsync(&c, ws.ctr, sizeof c, hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    if (c.failed)
+        throw cmg::hip::HipError("exact predicate: expansion capacity exceeded on " +
+                                 std::to_string(c.failed) + " tuples");
+}
+
+cmg_status soa_signs(int npts, const double* soa, size_t n, size_t ld, int mode,
+                     signed char* out, void* workspace, size_t workspace_bytes,
+                     void* stream, char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] {
+        require_device();
+        if (mode != cmg::hip::kModeExact && mode != cmg::hip::kModeCompat)
+            throw cmg::hip::InvalidInput("mode must be CMG_PRED_EXACT or CMG_PRED_COMPAT");
+        if (n && (!soa || !out)) throw cmg::hip::InvalidInput("null pointer");
+        if (n && ld < n) throw cmg::hip::InvalidInput("ld must be >= n");
+        if (n > 0xFFFFFFFFull) throw cmg::hip::InvalidInput("n must be < 2^32");
+        g_last_backend = "hip";
+        cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 2);
+        size_t need = cmg::hip::pred_workspace_bytes(plan, n);
+        hipStream_t st = as_stream(stream);
+        if (workspace) {
+            if (workspace_bytes < need)
+                throw cmg::hip::InvalidInput("workspace smaller than cmg_hip_pred_workspace_size");
+            cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(workspace, plan, n);
+            cmg::hip::launch_signs_soa(npts, mode, soa, n, ld, reinterpret_cast<int8_t*>(out),
+                                       plan, ws, st);
+        } else {
+            cmg::hip::ScratchLease lease;
+            cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(lease.s->get(need), plan, n);
+            cmg::hip::launch_signs_soa(npts, mode, soa, n, ld, reinterpret_cast<int8_t*>(out),
+                                       plan, ws, st);
+            CMG_HIP(hipStreamSynchronize(st));
+        }
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- 1. the reference's C ABI subset ---------------------------------------------------
+
+cmg_plc* cmg_plc_create(void) { return new cmg_plc{}; }
+void cmg_plc_destroy(cmg_plc* p) { delete p; }
+
+cmg_status cmg_plc_set_points(cmg_plc* plc, const double* xyz, size_t count) {
+    if (!plc || (!xyz && count)) return CMG_ERR_INVALID_INPUT;
+    plc->xyz.assign(xyz, xyz + 3 * count);
+    return CMG_OK;
+}
+
+cmg_status cmg_plc_add_facet(cmg_plc* plc, const int* idx, size_t count, int marker) {
+    if (!plc || (!idx && count)) return CMG_ERR_INVALID_INPUT;
+    if (count < 3) return CMG_ERR_INVALID_INPUT;
+    plc->polys.emplace_back(idx, idx + count);
+    plc->markers.push_back(marker);
+    return CMG_OK;
+}
+
+cmg_status cmg_plc_voxelize(const cmg_plc* in, int resolution, int mode, int pad,
+                            cmg_voxels** out, char* errbuf, size_t len) {
+    if (out) *out = nullptr;
+    if (!in || !out) return CMG_ERR_INVALID_INPUT;
+    if (resolution < 1) {  // cmg_c.cpp:246-249
+        set_err(errbuf, len, "resolution must be >= 1");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        // voxelize.cpp:363-379
+        const size_t npts = in->xyz.size() / 3;
+        if (npts == 0) throw cmg::hip::InvalidInput("voxelize: PLC has no points");
+        std::vector<int> tris = fan_triangles(*in);
+        if (tris.empty())
+            throw cmg::hip::InvalidInput("voxelize: PLC has no boundary triangles");
+        for (int v : tris)
+            if (v < 0 || (size_t)v >= npts)
+                throw cmg::hip::InvalidInput("voxelize: facet index out of range");
+        cmg::hip::GridSpec g;
+        if (!cmg::hip::make_grid(in->xyz.data(), npts, resolution, pad, g))
+            throw cmg::hip::InvalidInput("voxelize: degenerate (zero-extent) PLC");
+        if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
+            throw cmg::hip::InvalidInput("voxelize: grid has no cells (negative pad)");
+        require_device();
+        g_last_backend = "hip";
+        auto* v = new cmg_voxels{};
+        v->grid = g;
+        v->mode = mode == 1 ? 1 : 0;
+        const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+        std::vector<unsigned char> occ(ncell);
+        if (v->mode == 1) v->distance.resize(ncell);
+        try {
+            cmg::hip::voxelize_host(in->xyz.data(), npts, tris.data(), tris.size() / 3, g,
+                                    v->mode == 1, occ.data(),
+                                    v->mode == 1 ? v->distance.data() : nullptr);
+        } catch (...) {
+            delete v;
+            throw;
+        }
+        if (v->mode == 1) v->inside = std::move(occ);
+        else v->occupancy = std::move(occ);
+        *out = v;
+    });
+}
+
+void cmg_voxels_destroy(cmg_voxels* v) { delete v; }
+
+void cmg_voxels_dims(const cmg_voxels* v, int* nx, int* ny, int* nz) {
+    if (nx) *nx = v ? v->grid.nx : 0;
+    if (ny) *ny = v ? v->grid.ny : 0;
+    if (nz) *nz = v ? v->grid.nz : 0;
+}
+
+void cmg_voxels_origin(const cmg_voxels* v, double* x, double* y, double* z) {
+    // origin = corner + 0.5*spacing, the centre of cell (0,0,0) (voxelize.cpp:389-391)
+    if (x) *x = v ? v->grid.corner[0] + 0.5 * v->grid.spacing : 0.0;
+    if (y) *y = v ? v->grid.corner[1] + 0.5 * v->grid.spacing : 0.0;
+    if (z) *z = v ? v->grid.corner[2] + 0.5 * v->grid.spacing : 0.0;
+}
+
+double cmg_voxels_spacing(const cmg_voxels* v) { return v ? v->grid.spacing : 0.0; }
+
+const unsigned char* cmg_voxels_occupancy(const cmg_voxels* v) {
+    return (v && !v->occupancy.empty()) ? v->occupancy.data() : nullptr;
+}
+
+const float* cmg_voxels_distance(const cmg_voxels* v) {
+    return (v && !v->distance.empty()) ? v->distance.data() : nullptr;
+}
+
+const char* cmg_version(void) { return "0.5.0-hip"; }
+
+// ---- 2. additive host-pointer entry points ---------------------------------------------
+
+cmg_status cmg_orient3d_signs(const double* xyz, size_t npts, const int* quads, size_t n,
+                              int mode, signed char* out, char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] {
+        host_signs(4, mode, xyz, npts, quads, n, reinterpret_cast<int8_t*>(out));
+    });
+}
+
+cmg_status cmg_insphere_signs(const double* xyz, size_t npts, const int* quints, size_t n,
+                              int mode, signed char* out, char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] {
+        host_signs(5, mode, xyz, npts, quints, n, reinterpret_cast<int8_t*>(out));
+    });
+}
+
+cmg_status cmg_orient3d_values(const double* xyz, size_t npts, const int* quads, size_t n,
+                               double* out, char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] {
+        require_device();
+        if (n && (!xyz || !quads || !out)) throw cmg::hip::InvalidInput("null pointer");
+        for (size_t i = 0; i < 4 * n; ++i)
+            if (quads[i] < 0 || (size_t)quads[i] >= npts)
+                throw cmg::hip::InvalidInput("point index out of range");
+        g_last_backend = "hip";
+        if (n == 0) return;
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 1);
+        size_t wsb = cmg::hip::pred_workspace_bytes(plan, n);
+        size_t b_xyz = al(npts * 3 * sizeof(double)), b_idx = al(n * 4 * sizeof(int)),
+               b_out = al(n * sizeof(double));
+        cmg::hip::ScratchLease lease;
+        char* base = static_cast<char*>(lease.s->get(b_xyz + b_idx + b_out + wsb));
+        double* d_xyz = reinterpret_cast<double*>(base);
+        int* d_idx = reinterpret_cast<int*>(base + b_xyz);
+        double* d_out = reinterpret_cast<double*>(base + b_xyz + b_idx);
+        cmg::hip::PredWorkspace ws =
+            cmg::hip::pred_workspace(base + b_xyz + b_idx + b_out, plan, n);
+        hipStream_t st = nullptr;
+        CMG_HIP(hipMemcpyAsync(d_xyz, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        CMG_HIP(hipMemcpyAsync(d_idx, quads, n * 4 * sizeof(int), hipMemcpyHostToDevice, st));
+        cmg::hip::launch_o3_values_indexed(d_xyz, d_idx, n, d_out, plan, ws, st);
+        CMG_HIP(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+    });
+}
+
+cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,
+                                   const float* grad_sdf, float* grad_xyz, char* errbuf,
+                                   size_t len) {
+    if (!plc || !grid || !grad_sdf || !grad_xyz) {
+        set_err(errbuf, len, "null argument");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        if (grid->mode != 1)
+            throw cmg::hip::InvalidInput("sdf backward needs a grid voxelized with mode 1");
+        const size_t npts = plc->xyz.size() / 3;
+        std::vector<int> tris = fan_triangles(*plc);
+        for (int v : tris)
+            if (v < 0 || (size_t)v >= npts)
+                throw cmg::hip::InvalidInput("facet index out of range");
+        require_device();
+        g_last_backend = "hip";
+        cmg::hip::sdf_backward_host(plc->xyz.data(), npts, tris.data(), tris.size() / 3,
+                                    grid->grid, grid->inside.data(), grad_sdf, grad_xyz);
+    });
+}
+
+const char* cmg_backend_last(void) { return g_last_backend; }
+
+// ---- 3. device layer -------------------------------------------------------------------
+
+int cmg_hip_available(void) { return cmg::hip::device_available() ? 1 : 0; }
+
+int cmg_hip_orient3d_signs(const double* xyz, int npts, const int* tets, int n,
+                           int* out_sign) {
+    if (n < 0 || npts < 0) return 1;
+    std::vector<signed char> s((size_t)n);
+    cmg_status st = cmg_orient3d_signs(xyz, (size_t)npts, tets, (size_t)n,
+                                       CMG_PRED_COMPAT, s.data(), nullptr, 0);
+    if (st != CMG_OK) return (int)st;
+    for (int i = 0; i < n; ++i) out_sign[i] = s[(size_t)i];
+    return 0;
+}
+
+size_t cmg_hip_pred_workspace_size(size_t n, int npts) {
+    (void)npts;
+    return cmg::hip::pred_workspace_bytes(cmg::hip::pred_plan(n, 2), n);
+}
+
+cmg_status cmg_hip_orient3d_signs_soa(const double* soa, size_t n, size_t ld, int mode,
+                                      signed char* out, void* workspace,
+                                      size_t workspace_bytes, void* stream, char* errbuf,
+                                      size_t len) {
+    return soa_signs(4, soa, n, ld, mode, out, workspace, workspace_bytes, stream, errbuf,
+                     len);
+}
+
+cmg_status cmg_hip_insphere_signs_soa(const double* soa, size_t n, size_t ld, int mode,
+                                      signed char* out, void* workspace,
+                                      size_t workspace_bytes, void* stream, char* errbuf,
+                                      size_t len) {
+    return soa_signs(5, soa, n, ld, mode, out, workspace, workspace_bytes, stream, errbuf,
+                     len);
+}
+
+cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
+                              cmg_pred_stats* out, char* errbuf, size_t len) {
+    (void)npts;
+    if (!out) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        require_device();
+        cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 2);
+        hipStream_t st = as_stream(stream);
+        cmg::hip::PredCounters c{};
+        auto read = [&](void* base) {
+            cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(base, plan, n);
+            cmg::hip::launch_pred_stats(ws, st);
+            CMG_HIP(hipMemcpyAsync(&c, ws.ctr, sizeof c, hipMemcpyDeviceToHost, st));
+            CMG_HIP(hipStreamSynchronize(st));
+        };
+        if (workspace) {
+            read(workspace);
+        } else {
+            cmg::hip::ScratchLease lease;
+            read(lease.s->get(cmg::hip::pred_workspace_bytes(plan, n)));
+        }
+        out->lanes = n;
+        out->fp32 = c.fp32;
+        out->fp64 = c.fp64;
+        out->exact = c.exact;
+        out->wide = c.wide_n;
+        out->failed = c.failed;
+    });
+}
+
+void cmg_hip_kernel_timer(const char* name, int enable) {
+    if (name) cmg::hip::timer_enable(name, enable != 0);
+}
+
+cmg_status cmg_hip_kernel_time(const char* name, double* total_ms, uint64_t* count,
+                               char* errbuf, size_t len) {
+    if (!name) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] { cmg::hip::timer_read(name, total_ms, count); });
+}
+
+}  // extern "C"

@@ -1,0 +1,307 @@
+// expansion.hpp — exact FP64 sign evaluation on gfx950 for the lanes the filters leave
+// uncertain.
+//
+// Expansion arithmetic after Shewchuk, "Adaptive Precision Floating-Point Arithmetic and
+// Fast Robust Geometric Predicates" (CMU-CS-96-140): a value is an array of FP64
+// components, nonoverlapping, increasing in magnitude, zeros eliminated; its sign is the
+// sign of the last component. Two-Product uses the hardware FMA (p = a*b,
+// e = fma(a, b, -p) is exact), which replaces Dekker's split. Sums merge the two
+// inputs by magnitude and run one Two-Sum chain (linear time, never reading past either
+// input), and Compress keeps every intermediate short.
+//
+// Both determinants are evaluated exactly in the same difference form as the naive FP64
+// formula of src/predicates/predicates.cpp:2201-2222 (orient3d) and :4063-4106
+// (insphere): each difference is first made exact as a 1- or 2-component expansion.
+// Intermediates live in fixed-capacity private arrays; a lane whose expansion would
+// exceed a capacity reports kOverflow and is re-run by a wide-capacity kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cmg::hip {
+
+constexpr int kOverflow = 2;  // sign slot value meaning "capacity exceeded"
+
+__device__ __forceinline__ void two_sum(double a, double b, double& x, double& y) {
+    double s = a + b;
+    double bv = s - a;
+    double av = s - bv;
+    x = s;
+    y = (a - av) + (b - bv);
+}
+
+__device__ __forceinline__ void fast_two_sum(double a, double b, double& x, double& y) {
+    double s = a + b;
+    x = s;
+    y = b - (s - a);
+}
+
+__device__ __forceinline__ void two_prod(double a, double b, double& x, double& y) {
+    double p = a * b;
+    x = p;
+    y = __builtin_fma(a, b, -p);
+}
+
+// h = e + f. h must hold en + fn; h may not alias e or f.
+__device__ inline int exp_sum(const double* e, int en, const double* f, int fn, double* h) {
+    int i = 0, j = 0, k = 0;
+    double q;
+    // smallest-magnitude component first
+    if (j >= fn || (i < en && __builtin_fabs(e[i]) <= __builtin_fabs(f[j]))) q = e[i++];
+    else q = f[j++];
+    while (i < en || j < fn) {
+        double x;
+        if (j >= fn || (i < en && __builtin_fabs(e[i]) <= __builtin_fabs(f[j]))) x = e[i++];
+        else x = f[j++];
+        double s, t;
+        two_sum(q, x, s, t);
+        if (t != 0.0) h[k++] = t;
+        q = s;
+    }
+    if (q != 0.0 || k == 0) h[k++] = q;
+    return k;
+}
+
+// h = b * e. h must hold 2*en; h may not alias e.
+__device__ inline int exp_scale(const double* e, int en, double b, double* h) {
+    int k = 0;
+    double q, t;
+    two_prod(e[0], b, q, t);
+    if (t != 0.0) h[k++] = t;
+    for (int i = 1; i < en; ++i) {
+        double p1, p0, s;
+        two_prod(e[i], b, p1, p0);
+        two_sum(q, p0, s, t);
+        if (t != 0.0) h[k++] = t;
+        fast_two_sum(p1, s, q, t);
+        if (t != 0.0) h[k++] = t;
+    }
+    if (q != 0.0 || k == 0) h[k++] = q;
+    return k;
+}
+
+// In-place Compress: same value, nonadjacent components, usually very few.
+__device__ inline int exp_compress(double* e, int n) {
+    if (n <= 1) return n;
+    int bottom = n - 1;
+    double q = e[bottom];
+    for (int i = n - 2; i >= 0; --i) {
+        double qn, r;
+        fast_two_sum(q, e[i], qn, r);
+        if (r != 0.0) {
+            e[bottom--] = qn;
+            q = r;
+        } else {
+            q = qn;
+        }
+    }
+    int top = 0;
+    for (int i = bottom + 1; i < n; ++i) {
+        double qn, r;
+        fast_two_sum(e[i], q, qn, r);
+        if (r != 0.0) e[top++] = r;
+        q = qn;
+    }
+    e[top++] = q;
+    return top;
+}
+
+__device__ __forceinline__ void exp_negate(double* e, int n) {
+    for (int i = 0; i < n; ++i) e[i] = -e[i];
+}
+
+__device__ __forceinline__ int exp_sign(const double* e, int n) {
+    double t = e[n - 1];
+    return t > 0.0 ? 1 : (t < 0.0 ? -1 : 0);
+}
+
+// exact a - b as a 1- or 2-component expansion
+__device__ __forceinline__ int exp_diff(double a, double b, double* h) {
+    double x, y;
+    two_sum(a, -b, x, y);
+    if (y != 0.0) {
+        h[0] = y;
+        h[1] = x;
+        return 2;
+    }
+    h[0] = x;
+    return 1;
+}
+
+// h = e * f, e and f short (<= 2 components each): at most 8 components.
+__device__ inline int exp_mul_small(const double* e, int en, const double* f, int fn,
+                                    double* h) {
+    double a[4], b[4];
+    int an = exp_scale(e, en, f[0], a);
+    if (fn == 1) {
+        for (int i = 0; i < an; ++i) h[i] = a[i];
+        return an;
+    }
+    int bn = exp_scale(e, en, f[1], b);
+    return exp_sum(a, an, b, bn, h);
+}
+
+// h = e * f where f is short (fn components) and e has en <= ECAP components; the
+// result is compressed after every partial sum. Returns -1 if HCAP would be exceeded.
+template <int HCAP>
+__device__ inline int exp_mul(const double* e, int en, const double* f, int fn, double* h,
+                              double* tmp /* HCAP */, double* part /* 2*en */) {
+    if (2 * en > HCAP) return -1;
+    int hn = exp_scale(e, en, f[0], h);
+    hn = exp_compress(h, hn);
+    for (int j = 1; j < fn; ++j) {
+        int pn = exp_scale(e, en, f[j], part);
+        if (hn + pn > HCAP) return -1;
+        int tn = exp_sum(h, hn, part, pn, tmp);
+        tn = exp_compress(tmp, tn);
+        for (int i = 0; i < tn; ++i) h[i] = tmp[i];
+        hn = tn;
+    }
+    return hn;
+}
+
+// 2x2 minor p*q - r*s of 1-2 component entries (<= 16 components, compressed).
+__device__ inline int exp_minor(const double* p, int pn, const double* q, int qn,
+                                const double* r, int rn, const double* s, int sn,
+                                double* out /* 16 */) {
+    double a[8], b[8];
+    int an = exp_mul_small(p, pn, q, qn, a);
+    int bn = exp_mul_small(r, rn, s, sn, b);
+    exp_negate(b, bn);
+    int n = exp_sum(a, an, b, bn, out);
+    return exp_compress(out, n);
+}
+
+// acc += sgn * z * m  (z: 1-2 components, m: <= 16). Returns false on overflow.
+template <int CAP>
+__device__ inline bool exp_accumulate_zm(double* acc, int& an, const double* z, int zn,
+                                         const double* m, int mn, int sgn) {
+    double t[64];
+    double tmp[64];
+    double part[32];
+    int tn = exp_mul<64>(m, mn, z, zn, t, tmp, part);
+    if (tn < 0) return false;
+    if (sgn < 0) exp_negate(t, tn);
+    if (an + tn > CAP) return false;
+    double sum[CAP];
+    int sn = exp_sum(acc, an, t, tn, sum);
+    sn = exp_compress(sum, sn);
+    for (int i = 0; i < sn; ++i) acc[i] = sum[i];
+    an = sn;
+    return true;
+}
+
+// Exact orient3d(pa, pb, pc, pd) as an expansion; `top` receives its largest
+// component (same sign as the determinant, within a relative 2^-52 of its value).
+// Returns false if CAP is too small.
+template <int CAP>
+__device__ inline bool orient3d_exact(const double* pa, const double* pb, const double* pc,
+                                      const double* pd, double& top) {
+    double adx[2], ady[2], adz[2], bdx[2], bdy[2], bdz[2], cdx[2], cdy[2], cdz[2];
+    int nadx = exp_diff(pa[0], pd[0], adx), nady = exp_diff(pa[1], pd[1], ady),
+        nadz = exp_diff(pa[2], pd[2], adz);
+    int nbdx = exp_diff(pb[0], pd[0], bdx), nbdy = exp_diff(pb[1], pd[1], bdy),
+        nbdz = exp_diff(pb[2], pd[2], bdz);
+    int ncdx = exp_diff(pc[0], pd[0], cdx), ncdy = exp_diff(pc[1], pd[1], cdy),
+        ncdz = exp_diff(pc[2], pd[2], cdz);
+    double acc[CAP];
+    int an = 0;
+    double m[16];
+    int mn;
+    // adz * (bdx*cdy - cdx*bdy)
+    mn = exp_minor(bdx, nbdx, cdy, ncdy, cdx, ncdx, bdy, nbdy, m);
+    if (!exp_accumulate_zm<CAP>(acc, an, adz, nadz, m, mn, 1)) return false;
+    // + bdz * (cdx*ady - adx*cdy)
+    mn = exp_minor(cdx, ncdx, ady, nady, adx, nadx, cdy, ncdy, m);
+    if (!exp_accumulate_zm<CAP>(acc, an, bdz, nbdz, m, mn, 1)) return false;
+    // + cdz * (adx*bdy - bdx*ady)
+    mn = exp_minor(adx, nadx, bdy, nbdy, bdx, nbdx, ady, nady, m);
+    if (!exp_accumulate_zm<CAP>(acc, an, cdz, ncdz, m, mn, 1)) return false;
+    top = acc[an - 1];
+    return true;
+}
+
+// Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small.
+template <int CAP>
+__device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
+                                          const double* pc, const double* pd) {
+    double top;
+    if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
+    return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
+}
+
+// Exact sign of insphere(pa, pb, pc, pd, pe); kOverflow if a capacity is too small.
+// CCAP bounds a 3x3 cofactor, LCAP a lifted product.
+template <int CCAP, int LCAP>
+__device__ inline int insphere_exact_sign(const double* pa, const double* pb,
+                                          const double* pc, const double* pd,
+                                          const double* pe) {
+    double ex[4][2], ey[4][2], ez[4][2];
+    int nx[4], ny[4], nz[4];
+    const double* P[4] = {pa, pb, pc, pd};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        nx[v] = exp_diff(P[v][0], pe[0], ex[v]);
+        ny[v] = exp_diff(P[v][1], pe[1], ey[v]);
+        nz[v] = exp_diff(P[v][2], pe[2], ez[v]);
+    }
+    // The four cofactors of the naive formula (predicates.cpp:4072-4102), as
+    //   abc = aez*bc - bez*ac + cez*ab      bcd = bez*cd - cez*bd + dez*bc
+    //   cda = cez*da + dez*ac + aez*cd      dab = dez*ab + aez*bd + bez*da
+    // with xy minors uv = uex*vey - vex*uey, and
+    //   det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd).
+    // Table: for each term, (lift vertex, sign, then three (z vertex, minor u, minor v,
+    // sign) entries).
+    constexpr int A = 0, B = 1, C = 2, D = 3;
+    struct Entry { int z, u, v, s; };
+    struct Term { int lift, sgn; Entry e[3]; };
+    const Term terms[4] = {
+        {D, 1, {{A, B, C, 1}, {B, A, C, -1}, {C, A, B, 1}}},   // dlift * abc
+        {C, -1, {{D, A, B, 1}, {A, B, D, 1}, {B, D, A, 1}}},   // - clift * dab
+        {B, 1, {{C, D, A, 1}, {D, A, C, 1}, {A, C, D, 1}}},    // + blift * cda
+        {A, -1, {{B, C, D, 1}, {C, B, D, -1}, {D, B, C, 1}}},  // - alift * bcd
+    };
+    double acc[LCAP];
+    int an = 0;
+    for (int ti = 0; ti < 4; ++ti) {
+        const Term& T = terms[ti];
+        double cof[CCAP];
+        int cn = 0;
+        for (int k = 0; k < 3; ++k) {
+            const Entry& E = T.e[k];
+            double m[16];
+            int mn = exp_minor(ex[E.u], nx[E.u], ey[E.v], ny[E.v], ex[E.v], nx[E.v],
+                               ey[E.u], ny[E.u], m);
+            if (!exp_accumulate_zm<CCAP>(cof, cn, ez[E.z], nz[E.z], m, mn, E.s))
+                return kOverflow;
+        }
+        // lift = x^2 + y^2 + z^2 of the term's vertex
+        double lift[24];
+        int ln;
+        {
+            double a[8], b[8], t[16];
+            int an2 = exp_mul_small(ex[T.lift], nx[T.lift], ex[T.lift], nx[T.lift], a);
+            int bn2 = exp_mul_small(ey[T.lift], ny[T.lift], ey[T.lift], ny[T.lift], b);
+            int tn = exp_sum(a, an2, b, bn2, t);
+            int cn2 = exp_mul_small(ez[T.lift], nz[T.lift], ez[T.lift], nz[T.lift], a);
+            ln = exp_sum(t, tn, a, cn2, lift);
+            ln = exp_compress(lift, ln);
+        }
+        // acc += sgn * lift * cof, one lift component at a time
+        for (int j = 0; j < ln; ++j) {
+            if (2 * cn > LCAP) return kOverflow;
+            double part[2 * CCAP];
+            int pn = exp_scale(cof, cn, lift[j], part);
+            if (T.sgn < 0) exp_negate(part, pn);
+            if (an + pn > LCAP) return kOverflow;
+            double sum[LCAP];
+            int sn = exp_sum(acc, an, part, pn, sum);
+            sn = exp_compress(sum, sn);
+            for (int i = 0; i < sn; ++i) acc[i] = sum[i];
+            an = sn;
+        }
+    }
+    return exp_sign(acc, an);
+}
+
+}  // namespace cmg::hip

@@ -1,0 +1,623 @@
+// predicates.hip — batched orient3d / insphere signs on gfx950.
+//
+// Pass 1 (filter kernel, HBM-bound): each thread streams two tuples per iteration with
+// 16-byte non-temporal loads from the structure-of-arrays planes and decides a sign
+// through a staged filter:
+//   exact mode  : FP32 filter with a forward error bound (input rounding + Shewchuk's
+//                 o3derrboundA/isperrboundA evaluation bound at u = 2^-24), then the
+//                 FP64 naive determinant with the FP64 semi-static bound
+//                 (predicates.cpp:2235-2245, :4121-4161);
+//   compat mode : the naive FP64 determinant in the reference's operation order, whose
+//                 sign is the answer whenever it is nonzero (what cmg::robust returns
+//                 with its zero static filter, predicates.cpp:2230-2231, :4113).
+// Lanes still undecided are compacted with a wavefront ballot and a popcount prefix into
+// the wave's own queue segment (no atomics); each wave records its segment length.
+// Pass 2 (exact kernel): one 64-lane block per segment (lengths are read on the device,
+// so the host never waits), exact expansion arithmetic per lane; a lane whose
+// expansion outgrows the small private arrays is re-queued for a wide-capacity pass.
+//
+// This TU is compiled with -ffp-contract=off: the naive determinants must round exactly
+// as the reference's (no FMA contraction), and the error bounds assume it.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cmg_hip_internal.hpp"
+#include "device_common.hpp"
+#include "expansion.hpp"
+
+namespace cmg::hip {
+
+namespace {
+
+constexpr int kUndecided = 3;
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// ---- naive determinants in the reference's operation order ------------------------
+
+__device__ __forceinline__ double o3_naive(const double* v, double* perm) {
+    // predicates.cpp:2201-2222 with pa = v[0..2], pb = v[3..5], pc = v[6..8], pd = v[9..11]
+    double adx = v[0] - v[9], ady = v[1] - v[10], adz = v[2] - v[11];
+    double bdx = v[3] - v[9], bdy = v[4] - v[10], bdz = v[5] - v[11];
+    double cdx = v[6] - v[9], cdy = v[7] - v[10], cdz = v[8] - v[11];
+    double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    double cdxady = cdx * ady, adxcdy = adx * cdy;
+    double adxbdy = adx * bdy, bdxady = bdx * ady;
+    double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    if (perm)
+        *perm = (__builtin_fabs(bdxcdy) + __builtin_fabs(cdxbdy)) * __builtin_fabs(adz) +
+                (__builtin_fabs(cdxady) + __builtin_fabs(adxcdy)) * __builtin_fabs(bdz) +
+                (__builtin_fabs(adxbdy) + __builtin_fabs(bdxady)) * __builtin_fabs(cdz);
+    return det;
+}
+
+__device__ __forceinline__ double is_naive(const double* v, double* perm) {
+    // predicates.cpp:4063-4106 with pa..pe = v[0..14]
+    double aex = v[0] - v[12], bex = v[3] - v[12], cex = v[6] - v[12], dex = v[9] - v[12];
+    double aey = v[1] - v[13], bey = v[4] - v[13], cey = v[7] - v[13], dey = v[10] - v[13];
+    double aez = v[2] - v[14], bez = v[5] - v[14], cez = v[8] - v[14], dez = v[11] - v[14];
+    double aexbey = aex * bey, bexaey = bex * aey;
+    double ab = aexbey - bexaey;
+    double bexcey = bex * cey, cexbey = cex * bey;
+    double bc = bexcey - cexbey;
+    double cexdey = cex * dey, dexcey = dex * cey;
+    double cd = cexdey - dexcey;
+    double dexaey = dex * aey, aexdey = aex * dey;
+    double da = dexaey - aexdey;
+    double aexcey = aex * cey, cexaey = cex * aey;
+    double ac = aexcey - cexaey;
+    double bexdey = bex * dey, dexbey = dex * bey;
+    double bd = bexdey - dexbey;
+    double abc = aez * bc - bez * ac + cez * ab;
+    double bcd = bez * cd - cez * bd + dez * bc;
+    double cda = cez * da + dez * ac + aez * cd;
+    double dab = dez * ab + aez * bd + bez * da;
+    double alift = aex * aex + aey * aey + aez * aez;
+    double blift = bex * bex + bey * bey + bez * bez;
+    double clift = cex * cex + cey * cey + cez * cez;
+    double dlift = dex * dex + dey * dey + dez * dez;
+    double det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+    if (perm) {
+        // predicates.cpp:4122-4156
+        double aezp = __builtin_fabs(aez), bezp = __builtin_fabs(bez),
+               cezp = __builtin_fabs(cez), dezp = __builtin_fabs(dez);
+        double aexbeyp = __builtin_fabs(aexbey), bexaeyp = __builtin_fabs(bexaey);
+        double bexceyp = __builtin_fabs(bexcey), cexbeyp = __builtin_fabs(cexbey);
+        double cexdeyp = __builtin_fabs(cexdey), dexceyp = __builtin_fabs(dexcey);
+        double dexaeyp = __builtin_fabs(dexaey), aexdeyp = __builtin_fabs(aexdey);
+        double aexceyp = __builtin_fabs(aexcey), cexaeyp = __builtin_fabs(cexaey);
+        double bexdeyp = __builtin_fabs(bexdey), dexbeyp = __builtin_fabs(dexbey);
+        *perm = ((cexdeyp + dexceyp) * bezp + (dexbeyp + bexdeyp) * cezp +
+                 (bexceyp + cexbeyp) * dezp) * alift +
+                ((dexaeyp + aexdeyp) * cezp + (aexceyp + cexaeyp) * dezp +
+                 (cexdeyp + dexceyp) * aezp) * blift +
+                ((aexbeyp + bexaeyp) * dezp + (bexdeyp + dexbeyp) * aezp +
+                 (dexaeyp + aexdeyp) * bezp) * clift +
+                ((bexceyp + cexbeyp) * aezp + (cexaeyp + aexceyp) * bezp +
+                 (aexbeyp + bexaeyp) * cezp) * dlift;
+    }
+    return det;
+}
+
+// ---- FP32 stage ------------------------------------------------------------------
+//
+// Inputs are rounded to float (|f - x| <= u|x|, u = 2^-24) and the naive formula is
+// evaluated in FP32. With M_a = max |rounded coordinate| on axis a (orient3d) or over all
+// coordinates (insphere), the true determinant D satisfies
+//   orient3d : |D32 - D| <= (7u + 56u^2) P32 + 145 u Mx My Mz
+//   insphere : |D32 - D| <= (16u + 224u^2) P32 + 11521 u M^5
+// where P32 is the FP32 permanent (Shewchuk's evaluation bound at FP32 precision) and the
+// second term bounds how far rounding the inputs moves the exact determinant (6 resp.
+// 72 monomials of degree 3 resp. 5 in differences bounded by 2M, each difference moved
+// by at most 2uM). The coefficients below are rounded up so that computing the bound in
+// FP32 cannot undercut it. The range checks keep every product normal and finite, so
+// the relative error model holds; outside them the lane goes to the FP64 stage.
+
+__device__ __forceinline__ int o3_fp32(const double* v) {
+    float f[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) f[q] = (float)v[q];
+    float mx = fmaxf(fmaxf(__builtin_fabsf(f[0]), __builtin_fabsf(f[3])),
+                     fmaxf(__builtin_fabsf(f[6]), __builtin_fabsf(f[9])));
+    float my = fmaxf(fmaxf(__builtin_fabsf(f[1]), __builtin_fabsf(f[4])),
+                     fmaxf(__builtin_fabsf(f[7]), __builtin_fabsf(f[10])));
+    float mz = fmaxf(fmaxf(__builtin_fabsf(f[2]), __builtin_fabsf(f[5])),
+                     fmaxf(__builtin_fabsf(f[8]), __builtin_fabsf(f[11])));
+    const float lo = 0x1p-30f, hi = 0x1p30f;
+    if (!(mx >= lo && mx <= hi && my >= lo && my <= hi && mz >= lo && mz <= hi))
+        return kUndecided;
+    float adx = f[0] - f[9], ady = f[1] - f[10], adz = f[2] - f[11];
+    float bdx = f[3] - f[9], bdy = f[4] - f[10], bdz = f[5] - f[11];
+    float cdx = f[6] - f[9], cdy = f[7] - f[10], cdz = f[8] - f[11];
+    float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    float cdxady = cdx * ady, adxcdy = adx * cdy;
+    float adxbdy = adx * bdy, bdxady = bdx * ady;
+    float det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    float perm = (__builtin_fabsf(bdxcdy) + __builtin_fabsf(cdxbdy)) * __builtin_fabsf(adz) +
+                 (__builtin_fabsf(cdxady) + __builtin_fabsf(adxcdy)) * __builtin_fabsf(bdz) +
+                 (__builtin_fabsf(adxbdy) + __builtin_fabsf(bdxady)) * __builtin_fabsf(cdz);
+    float bound = 0x1p-21f * perm + (146.0f * 0x1p-24f) * (mx * my * mz);
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
+    return kUndecided;
+}
+
+__device__ __forceinline__ int is_fp32(const double* v) {
+    float f[15];
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 15; ++q) {
+        f[q] = (float)v[q];
+        m = fmaxf(m, __builtin_fabsf(f[q]));
+    }
+    if (!(m >= 0x1p-20f && m <= 0x1p20f)) return kUndecided;
+    float aex = f[0] - f[12], bex = f[3] - f[12], cex = f[6] - f[12], dex = f[9] - f[12];
+    float aey = f[1] - f[13], bey = f[4] - f[13], cey = f[7] - f[13], dey = f[10] - f[13];
+    float aez = f[2] - f[14], bez = f[5] - f[14], cez = f[8] - f[14], dez = f[11] - f[14];
+    float aexbey = aex * bey, bexaey = bex * aey, ab = aexbey - bexaey;
+    float bexcey = bex * cey, cexbey = cex * bey, bc = bexcey - cexbey;
+    float cexdey = cex * dey, dexcey = dex * cey, cd = cexdey - dexcey;
+    float dexaey = dex * aey, aexdey = aex * dey, da = dexaey - aexdey;
+    float aexcey = aex * cey, cexaey = cex * aey, ac = aexcey - cexaey;
+    float bexdey = bex * dey, dexbey = dex * bey, bd = bexdey - dexbey;
+    float abc = aez * bc - bez * ac + cez * ab;
+    float bcd = bez * cd - cez * bd + dez * bc;
+    float cda = cez * da + dez * ac + aez * cd;
+    float dab = dez * ab + aez * bd + bez * da;
+    float alift = aex * aex + aey * aey + aez * aez;
+    float blift = bex * bex + bey * bey + bez * bez;
+    float clift = cex * cex + cey * cey + cez * cez;
+    float dlift = dex * dex + dey * dey + dez * dez;
+    float det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+    float aezp = __builtin_fabsf(aez), bezp = __builtin_fabsf(bez), cezp = __builtin_fabsf(cez),
+          dezp = __builtin_fabsf(dez);
+    float p_ab = __builtin_fabsf(aexbey) + __builtin_fabsf(bexaey);
+    float p_bc = __builtin_fabsf(bexcey) + __builtin_fabsf(cexbey);
+    float p_cd = __builtin_fabsf(cexdey) + __builtin_fabsf(dexcey);
+    float p_da = __builtin_fabsf(dexaey) + __builtin_fabsf(aexdey);
+    float p_ac = __builtin_fabsf(aexcey) + __builtin_fabsf(cexaey);
+    float p_bd = __builtin_fabsf(bexdey) + __builtin_fabsf(dexbey);
+    float perm = (p_cd * bezp + p_bd * cezp + p_bc * dezp) * alift +
+                 (p_da * cezp + p_ac * dezp + p_cd * aezp) * blift +
+                 (p_ab * dezp + p_bd * aezp + p_da * bezp) * clift +
+                 (p_bc * aezp + p_ac * bezp + p_ab * cezp) * dlift;
+    float m2 = m * m;
+    float bound = (17.0f * 0x1p-24f) * perm + (11600.0f * 0x1p-24f) * (m2 * m2 * m);
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
+    return kUndecided;
+}
+
+// ---- staged decision per tuple -----------------------------------------------------
+
+__device__ __forceinline__ int sgn(double x) { return x > 0.0 ? 1 : (x < 0.0 ? -1 : 0); }
+
+constexpr double kEps = 0x1p-53;
+constexpr double kO3ErrA = (7.0 + 56.0 * kEps) * kEps;     // predicates.cpp:619
+constexpr double kIspErrA = (16.0 + 224.0 * kEps) * kEps;  // predicates.cpp:625
+
+// Returns the sign, or kUndecided; `stage` records which stage decided (1 fp32, 2 fp64).
+template <int NPTS, int MODE>
+__device__ __forceinline__ int decide(const double* v, int& stage) {
+    if (MODE == kModeExact) {
+        int s = NPTS == 4 ? o3_fp32(v) : is_fp32(v);
+        if (s != kUndecided) {
+            stage = 1;
+            return s;
+        }
+        double perm;
+        double det = NPTS == 4 ? o3_naive(v, &perm) : is_naive(v, &perm);
+        double eb = (NPTS == 4 ? kO3ErrA : kIspErrA) * perm;
+        stage = 2;
+        if (det > eb || -det > eb) return sgn(det);
+        return kUndecided;
+    } else {
+        double det = NPTS == 4 ? o3_naive(v, nullptr) : is_naive(v, nullptr);
+        stage = 2;
+        if (det > 0.0 || det < -0.0) return sgn(det);
+        return kUndecided;
+    }
+}
+
+// ---- tuple sources -------------------------------------------------------------------
+
+template <int NPTS>
+struct SoaSource {
+    const double* soa;
+    size_t ld;
+    __device__ __forceinline__ void load(size_t i, double* v) const {
+#pragma unroll
+        for (int q = 0; q < NPTS * 3; ++q) v[q] = soa[(size_t)q * ld + i];
+    }
+};
+
+template <int NPTS>
+struct IndexedSource {  // NPTS-point tuples as index records into AoS points
+    const double* xyz;
+    const int* idx;
+    __device__ __forceinline__ void load(size_t i, double* v) const {
+#pragma unroll
+        for (int s = 0; s < NPTS; ++s) {
+            const size_t id = (size_t)idx[(size_t)NPTS * i + s];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) v[3 * s + a] = xyz[3 * id + a];
+        }
+    }
+};
+
+// ---- wave helpers ----------------------------------------------------------------------
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Per-wave compaction: every wave owns queue segment [wave*seg, (wave+1)*seg) and a
+// wave-uniform fill count `off`; flagged lanes write their value at off + (number of
+// flagged lanes below them). No atomics: the exact pass reads each segment's count.
+__device__ __forceinline__ void wave_push(bool flag, uint32_t value, uint32_t* seg_base,
+                                          unsigned& off) {
+    unsigned long long mask = __ballot(flag);
+    if (flag) {
+        unsigned long long below = mask & ((1ull << lane_id()) - 1ull);
+        seg_base[off + __popcll(below)] = value;
+    }
+    off += (unsigned)__popcll(mask);
+}
+
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// ---- kernels -------------------------------------------------------------------------
+//
+// Grid-stride kernels run the same number of iterations in every lane (`iters`), so the
+// ballots and the per-wave fill count stay wave-uniform.
+
+// Structure-of-arrays filter: two tuples per thread per iteration, 16-byte
+// non-temporal loads (requires 16-byte aligned planes and an even leading dimension).
+template <int NPTS, int MODE, bool VEC>
+__global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restrict__ soa,
+                                                         size_t n, size_t ld,
+                                                         int8_t* __restrict__ out,
+                                                         PredWorkspace ws, unsigned iters) {
+    constexpr int NPL = NPTS * 3;
+    unsigned n32 = 0, n64 = 0;
+    const size_t npairs = (n + 1) / 2;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = (unsigned)(tid / 64);
+    uint32_t* seg = ws.queue + (size_t)wave * ws.seg;
+    unsigned off = 0;
+    for (unsigned it = 0; it < iters; ++it) {
+        const size_t p = tid + (size_t)it * stride;
+        const bool valid = p < npairs;
+        const size_t i0 = 2 * p;
+        const bool two = valid && i0 + 1 < n;
+        bool u0 = false, u1 = false;
+        if (valid) {
+            double v0[NPL], v1[NPL];
+            if (VEC && two) {
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) {
+                    d2v x = __builtin_nontemporal_load(
+                        reinterpret_cast<const d2v*>(soa + (size_t)q * ld + i0));
+                    v0[q] = x.x;
+                    v1[q] = x.y;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) {
+                    v0[q] = soa[(size_t)q * ld + i0];
+                    v1[q] = two ? soa[(size_t)q * ld + i0 + 1] : v0[q];
+                }
+            }
+            int st0 = 0, st1 = 0;
+            int s0 = decide<NPTS, MODE>(v0, st0);
+            int s1 = decide<NPTS, MODE>(v1, st1);
+            u0 = s0 == kUndecided;
+            u1 = two && s1 == kUndecided;
+            int8_t o0 = u0 ? 0 : (int8_t)s0, o1 = (u1 || !two) ? 0 : (int8_t)s1;
+            if (two && ((reinterpret_cast<uintptr_t>(out + i0) & 1) == 0)) {
+                uint16_t packed = (uint16_t)(uint8_t)o0 | ((uint16_t)(uint8_t)o1 << 8);
+                __builtin_nontemporal_store(packed, reinterpret_cast<uint16_t*>(out + i0));
+            } else {
+                out[i0] = o0;
+                if (two) out[i0 + 1] = o1;
+            }
+            n32 += (!u0 && st0 == 1) + (two && !u1 && st1 == 1);
+            n64 += (!u0 && st0 == 2) + (two && !u1 && st1 == 2);
+        }
+        wave_push(u0, (uint32_t)i0, seg, off);
+        wave_push(u1, (uint32_t)(i0 + 1), seg, off);
+    }
+    n32 = wave_sum(n32);
+    n64 = wave_sum(n64);
+    if (lane_id() == 0) {
+        ws.wave_q[wave] = off;
+        ws.wave_32[wave] = n32;
+        ws.wave_64[wave] = n64;
+    }
+}
+
+// Generic filter over any source (indexed form), one tuple per thread.
+template <int NPTS, int MODE, class Src>
+__global__ __launch_bounds__(256) void filter_kernel(Src src, size_t n,
+                                                     int8_t* __restrict__ out,
+                                                     PredWorkspace ws, unsigned iters) {
+    unsigned n32 = 0, n64 = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = (unsigned)(tid / 64);
+    uint32_t* seg = ws.queue + (size_t)wave * ws.seg;
+    unsigned off = 0;
+    for (unsigned it = 0; it < iters; ++it) {
+        const size_t i = tid + (size_t)it * stride;
+        bool u = false;
+        if (i < n) {
+            double v[NPTS * 3];
+            src.load(i, v);
+            int st = 0;
+            int s = decide<NPTS, MODE>(v, st);
+            u = s == kUndecided;
+            out[i] = u ? 0 : (int8_t)s;
+            n32 += !u && st == 1;
+            n64 += !u && st == 2;
+        }
+        wave_push(u, (uint32_t)i, seg, off);
+    }
+    n32 = wave_sum(n32);
+    n64 = wave_sum(n64);
+    if (lane_id() == 0) {
+        ws.wave_q[wave] = off;
+        ws.wave_32[wave] = n32;
+        ws.wave_64[wave] = n64;
+    }
+}
+
+// Exact pass. Pass 1 (WIDE = 0): block b walks queue segment b (and every gridDim.x-th
+// after it) with small private arrays; lanes that outgrow them are appended to the wide
+// queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide arrays;
+// a lane that still overflows is counted in ctr->failed and left at 0.
+template <int NPTS, bool WIDE, class Src>
+__global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__ out,
+                                                   PredWorkspace ws) {
+    if (!WIDE) {
+        for (unsigned sgi = blockIdx.x; sgi < ws.nwaves; sgi += gridDim.x) {
+            const unsigned count = ws.wave_q[sgi];
+            const uint32_t* q = ws.queue + (size_t)sgi * ws.seg;
+            for (unsigned k = threadIdx.x; k < count; k += blockDim.x) {
+                uint32_t i = q[k];
+                double v[NPTS * 3];
+                src.load(i, v);
+                int s = NPTS == 4 ? orient3d_exact_sign<40>(v, v + 3, v + 6, v + 9)
+                                  : insphere_exact_sign<40, 96>(v, v + 3, v + 6, v + 9, v + 12);
+                if (s == kOverflow) {
+                    unsigned slot = atomicAdd(&ws.ctr->wide_n, 1u);
+                    ws.wide[slot] = i;
+                } else {
+                    out[i] = (int8_t)s;
+                }
+            }
+        }
+    } else {
+        const unsigned count = ws.ctr->wide_n;
+        const unsigned stride = gridDim.x * blockDim.x;
+        for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+            uint32_t i = ws.wide[k];
+            double v[NPTS * 3];
+            src.load(i, v);
+            int s = NPTS == 4 ? orient3d_exact_sign<320>(v, v + 3, v + 6, v + 9)
+                              : insphere_exact_sign<200, 1024>(v, v + 3, v + 6, v + 9, v + 12);
+            if (s == kOverflow) {
+                atomicAdd(&ws.ctr->failed, 1u);
+                s = 0;
+            }
+            out[i] = (int8_t)s;
+        }
+    }
+}
+
+// orient3d values (cmg::robust::orient3d_batch): the naive FP64 determinant when it is
+// nonzero, else the exact determinant's leading expansion component (same sign).
+template <class Src>
+__global__ __launch_bounds__(256) void o3_values_kernel(Src src, size_t n,
+                                                        double* __restrict__ out,
+                                                        PredWorkspace ws, unsigned iters) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned wave = (unsigned)(tid / 64);
+    uint32_t* seg = ws.queue + (size_t)wave * ws.seg;
+    unsigned off = 0;
+    for (unsigned it = 0; it < iters; ++it) {
+        const size_t i = tid + (size_t)it * stride;
+        bool u = false;
+        if (i < n) {
+            double v[12];
+            src.load(i, v);
+            double det = o3_naive(v, nullptr);
+            u = !(det > 0.0 || det < -0.0);
+            out[i] = det;
+        }
+        wave_push(u, (uint32_t)i, seg, off);
+    }
+    if (lane_id() == 0) ws.wave_q[wave] = off;
+}
+
+template <class Src>
+__global__ __launch_bounds__(64) void o3_values_exact_kernel(Src src,
+                                                             double* __restrict__ out,
+                                                             PredWorkspace ws) {
+    for (unsigned sgi = blockIdx.x; sgi < ws.nwaves; sgi += gridDim.x) {
+        const unsigned count = ws.wave_q[sgi];
+        const uint32_t* q = ws.queue + (size_t)sgi * ws.seg;
+        for (unsigned k = threadIdx.x; k < count; k += blockDim.x) {
+            uint32_t i = q[k];
+            double v[12];
+            src.load(i, v);
+            double top = 0.0;
+            if (!orient3d_exact<320>(v, v + 3, v + 6, v + 9, top)) {
+                atomicAdd(&ws.ctr->failed, 1u);
+                top = 0.0;
+            }
+            out[i] = top;
+        }
+    }
+}
+
+// Sums the per-wave statistics into ctr (only when the caller asks for them).
+__global__ __launch_bounds__(256) void reduce_stats_kernel(PredWorkspace ws) {
+    unsigned long long q = 0, a = 0, b = 0;
+    for (unsigned w = threadIdx.x; w < ws.nwaves; w += blockDim.x) {
+        q += ws.wave_q[w];
+        a += ws.wave_32[w];
+        b += ws.wave_64[w];
+    }
+    __shared__ unsigned long long sq[256], sa[256], sb[256];
+    sq[threadIdx.x] = q;
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sq[threadIdx.x] += sq[threadIdx.x + o];
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ws.ctr->exact = sq[0];
+        ws.ctr->fp32 = sa[0];
+        ws.ctr->fp64 = sb[0];
+    }
+}
+
+__global__ void zero_counters_kernel(PredCounters* c) {
+    if (threadIdx.x == 0) *c = PredCounters{};
+}
+
+template <int NPTS, class Src>
+void launch_exact(Src src, int8_t* out, const PredWorkspace& ws, hipStream_t stream) {
+    unsigned g = ws.nwaves < (unsigned)cu_count() * 8 ? ws.nwaves : (unsigned)cu_count() * 8;
+    exact_kernel<NPTS, false, Src><<<g ? g : 1, 64, 0, stream>>>(src, out, ws);
+    CMG_HIP_LAUNCH();
+    unsigned gw = cu_count() / 4 > 0 ? cu_count() / 4 : 1;
+    exact_kernel<NPTS, true, Src><<<gw, 64, 0, stream>>>(src, out, ws);
+    CMG_HIP_LAUNCH();
+}
+
+}  // namespace
+
+PredPlan pred_plan(size_t items, unsigned per_thread) {
+    PredPlan p;
+    p.block = 256;
+    size_t units = (items + per_thread - 1) / per_thread;
+    p.grid = grid_for(units ? units : 1, p.block, 16);
+    size_t stride = (size_t)p.grid * p.block;
+    p.iters = (unsigned)((units + stride - 1) / stride);
+    p.nwaves = p.grid * (p.block / 64);
+    p.seg = 64 * per_thread * p.iters;
+    return p;
+}
+
+size_t pred_workspace_bytes(const PredPlan& p, size_t n) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return 256 + 3 * al((size_t)p.nwaves * 4) + al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
+}
+
+PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    PredWorkspace ws;
+    char* c = static_cast<char*>(base);
+    ws.ctr = reinterpret_cast<PredCounters*>(c);
+    c += 256;
+    ws.wave_q = reinterpret_cast<uint32_t*>(c);
+    c += al((size_t)p.nwaves * 4);
+    ws.wave_32 = reinterpret_cast<uint32_t*>(c);
+    c += al((size_t)p.nwaves * 4);
+    ws.wave_64 = reinterpret_cast<uint32_t*>(c);
+    c += al((size_t)p.nwaves * 4);
+    ws.queue = reinterpret_cast<uint32_t*>(c);
+    c += al((size_t)p.nwaves * p.seg * 4);
+    ws.wide = reinterpret_cast<uint32_t*>(c);
+    ws.seg = p.seg;
+    ws.nwaves = p.nwaves;
+    return ws;
+}
+
+void launch_pred_stats(const PredWorkspace& ws, hipStream_t stream) {
+    reduce_stats_kernel<<<1, 256, 0, stream>>>(ws);
+    CMG_HIP_LAUNCH();
+}
+
+void launch_signs_soa(int npts, int mode, const double* soa, size_t n, size_t ld,
+                      int8_t* out, const PredPlan& p, const PredWorkspace& ws,
+                      hipStream_t stream) {
+    zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
+    CMG_HIP_LAUNCH();
+    if (n == 0) return;
+    const bool vec = (reinterpret_cast<uintptr_t>(soa) % 16 == 0) && (ld % 2 == 0);
+#define CMG_FILTER(NP, MD)                                                                \
+    do {                                                                                  \
+        {                                                                                 \
+            TimedLaunch timed(NP == 4 ? "orient3d_filter" : "insphere_filter", stream);   \
+            if (vec)                                                                      \
+                filter_soa_kernel<NP, MD, true>                                           \
+                    <<<p.grid, p.block, 0, stream>>>(soa, n, ld, out, ws, p.iters);       \
+            else                                                                          \
+                filter_soa_kernel<NP, MD, false>                                          \
+                    <<<p.grid, p.block, 0, stream>>>(soa, n, ld, out, ws, p.iters);       \
+            CMG_HIP_LAUNCH();                                                             \
+        }                                                                                 \
+        launch_exact<NP>(SoaSource<NP>{soa, ld}, out, ws, stream);                        \
+    } while (0)
+    if (npts == 4) {
+        if (mode == kModeExact) CMG_FILTER(4, kModeExact);
+        else CMG_FILTER(4, kModeCompat);
+    } else {
+        if (mode == kModeExact) CMG_FILTER(5, kModeExact);
+        else CMG_FILTER(5, kModeCompat);
+    }
+#undef CMG_FILTER
+}
+
+void launch_signs_indexed(int npts, int mode, const double* xyz, const int* idx, size_t n,
+                          int8_t* out, const PredPlan& p, const PredWorkspace& ws,
+                          hipStream_t stream) {
+    zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
+    CMG_HIP_LAUNCH();
+    if (n == 0) return;
+#define CMG_FILTER_IDX(NP, MD)                                                            \
+    do {                                                                                  \
+        IndexedSource<NP> src{xyz, idx};                                                  \
+        filter_kernel<NP, MD, IndexedSource<NP>>                                          \
+            <<<p.grid, p.block, 0, stream>>>(src, n, out, ws, p.iters);                   \
+        CMG_HIP_LAUNCH();                                                                 \
+        launch_exact<NP>(src, out, ws, stream);                                           \
+    } while (0)
+    if (npts == 4) {
+        if (mode == kModeExact) CMG_FILTER_IDX(4, kModeExact);
+        else CMG_FILTER_IDX(4, kModeCompat);
+    } else {
+        if (mode == kModeExact) CMG_FILTER_IDX(5, kModeExact);
+        else CMG_FILTER_IDX(5, kModeCompat);
+    }
+#undef CMG_FILTER_IDX
+}
+
+void launch_o3_values_indexed(const double* xyz, const int* quads, size_t n, double* out,
+                              const PredPlan& p, const PredWorkspace& ws,
+                              hipStream_t stream) {
+    zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
+    CMG_HIP_LAUNCH();
+    if (n == 0) return;
+    IndexedSource<4> src{xyz, quads};
+    o3_values_kernel<IndexedSource<4>>
+        <<<p.grid, p.block, 0, stream>>>(src, n, out, ws, p.iters);
+    CMG_HIP_LAUNCH();
+    unsigned g = ws.nwaves < (unsigned)cu_count() * 8 ? ws.nwaves : (unsigned)cu_count() * 8;
+    o3_values_exact_kernel<IndexedSource<4>><<<g ? g : 1, 64, 0, stream>>>(src, out, ws);
+    CMG_HIP_LAUNCH();
+}
+
+}  // namespace cmg::hip

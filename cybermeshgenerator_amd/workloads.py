@@ -1,0 +1,362 @@
+"""Synthetic inputs for the predicate and voxelization path.
+
+Meshes are closed, consistently wound triangle soups (float64 vertices, int32
+triangles). Grids "N^3 over [0,1]^3" are expressed through the unchanged
+voxelize API the way the reference allows it: two unreferenced anchor points
+(0,0,0) and (1,1,1) are appended to the vertex list and the grid is requested
+with ``resolution=N, pad=0``; make_grid takes the bounding box over all points
+(src/voxelize/voxelize.cpp:52-74) while triangles come only from facets, which
+gives origin 1/(2N) and spacing 1/N.
+
+Workloads:
+  icosphere_c1      642 V / 1,280 T icosphere (radius 0.45), seeded rotation, 64^3
+  tuples_orient3d   batched orient3d tuples, 90 % uniform, 10 % near-degenerate
+  tuples_insphere   batched insphere tuples, same mix
+  torus_c3          65,536-triangle jittered torus, 256^3
+  harmonic_c4       icosphere L5 with spherical-harmonic displacement, 128^3
+  batch_c5          64 uneven meshes with per-mesh resolutions
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+ANCHORS = np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]])
+
+
+@dataclass
+class MeshCase:
+    """A closed mesh plus the grid request that voxelizes it."""
+    name: str
+    points: np.ndarray       # (V, 3) float64, anchors included when anchored
+    tris: np.ndarray         # (T, 3) int32
+    resolution: int
+    pad: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+# ---- meshes -------------------------------------------------------------------------
+
+def _icosahedron():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+                  [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+                  [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                  [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]],
+                 dtype=np.int64)
+    return v / np.linalg.norm(v, axis=1, keepdims=True), f
+
+
+def icosphere(level: int):
+    """Unit icosphere by recursive midpoint subdivision with projection:
+    10*4^L + 2 vertices, 20*4^L triangles, outward winding."""
+    v, f = _icosahedron()
+    for _ in range(level):
+        edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(edges, axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        m01 = base + inv[:nf]
+        m12 = base + inv[nf:2 * nf]
+        m20 = base + inv[2 * nf:]
+        f = np.concatenate([
+            np.stack([f[:, 0], m01, m20], 1),
+            np.stack([f[:, 1], m12, m01], 1),
+            np.stack([f[:, 2], m20, m12], 1),
+            np.stack([m01, m12, m20], 1)])
+    return v, f.astype(np.int32)
+
+
+def geodesic_sphere(freq: int):
+    """Unit geodesic sphere: every icosahedron face split into freq^2 triangles
+    (20*freq^2 triangles), vertices projected to the sphere. Covers triangle counts
+    between the icosphere levels."""
+    v0, f0 = _icosahedron()
+    n = max(1, int(freq))
+    pts, keys, faces = [], {}, []
+
+    def vid(fa, i, j):
+        # point i*A + j*B + (n-i-j)*C on face fa, keyed by its integer weights
+        a, b, c = f0[fa]
+        w = {}
+        for vert, wt in ((a, i), (b, j), (c, n - i - j)):
+            if wt:
+                w[vert] = w.get(vert, 0) + wt
+        k = tuple(sorted(w.items()))
+        if k not in keys:
+            p = sum(v0[vv] * wt for vv, wt in k) / n
+            keys[k] = len(pts)
+            pts.append(p / np.linalg.norm(p))
+        return keys[k]
+
+    for fa in range(20):
+        for i in range(n):
+            for j in range(n - i):
+                p0, p1, p2 = vid(fa, i, j), vid(fa, i + 1, j), vid(fa, i, j + 1)
+                faces.append((p0, p1, p2))
+                if i + j + 1 < n:
+                    faces.append((p1, vid(fa, i + 1, j + 1), p2))
+    v = np.array(pts)
+    f = np.array(faces, dtype=np.int32)
+    # orient outward: the icosahedron faces are outward-wound as (A, B, C); with the
+    # weights above (i on A, j on B) each emitted triangle has the same handedness.
+    c = v[f].mean(axis=1)
+    nrm = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    flip = (nrm * c).sum(1) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return v, f
+
+
+def torus(R: float, r: float, nu: int, nv: int):
+    """Torus around the z axis: nu x nv parametric grid, 2*nu*nv triangles, with the
+    outward unit normal per vertex."""
+    u = 2 * np.pi * np.arange(nu) / nu
+    w = 2 * np.pi * np.arange(nv) / nv
+    U, W = np.meshgrid(u, w, indexing="ij")
+    x = (R + r * np.cos(W)) * np.cos(U)
+    y = (R + r * np.cos(W)) * np.sin(U)
+    z = r * np.sin(W)
+    v = np.stack([x, y, z], -1).reshape(-1, 3)
+    nrm = np.stack([np.cos(W) * np.cos(U), np.cos(W) * np.sin(U), np.sin(W)],
+                   -1).reshape(-1, 3)
+    i = np.arange(nu)[:, None]
+    j = np.arange(nv)[None, :]
+    a = (i * nv + j)
+    b = (((i + 1) % nu) * nv + j)
+    c = (((i + 1) % nu) * nv + (j + 1) % nv)
+    d = (i * nv + (j + 1) % nv)
+    f = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3),
+                        np.stack([a, c, d], -1).reshape(-1, 3)]).astype(np.int32)
+    return v, f, nrm
+
+
+def real_sph_harm_sum(dirs: np.ndarray, lmax: int, rng: np.random.Generator):
+    """A seeded random sum of real spherical harmonics up to degree lmax evaluated at
+    unit directions, scaled so that its maximum magnitude over `dirs` is 1."""
+    from scipy.special import lpmv
+    x, y, z = dirs[:, 0], dirs[:, 1], dirs[:, 2]
+    theta = np.arccos(np.clip(z, -1, 1))
+    phi = np.arctan2(y, x)
+    ct = np.cos(theta)
+    s = np.zeros(dirs.shape[0])
+    for l in range(1, lmax + 1):
+        for m in range(-l, l + 1):
+            am = abs(m)
+            norm = math.sqrt((2 * l + 1) / (4 * math.pi) *
+                             math.factorial(l - am) / math.factorial(l + am))
+            P = lpmv(am, l, ct)
+            if m > 0:
+                y_lm = math.sqrt(2) * norm * P * np.cos(m * phi)
+            elif m < 0:
+                y_lm = math.sqrt(2) * norm * P * np.sin(am * phi)
+            else:
+                y_lm = norm * P
+            s += rng.standard_normal() * y_lm
+    return s / np.max(np.abs(s))
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """Haar-random rotation from a uniformly random unit quaternion."""
+    q = rng.standard_normal(4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def anchored(points: np.ndarray) -> np.ndarray:
+    """Append the (0,0,0) and (1,1,1) anchors so that resolution=N, pad=0 yields the
+    N^3 grid over [0,1]^3."""
+    return np.concatenate([np.asarray(points, dtype=np.float64), ANCHORS])
+
+
+def _drop_degenerate(v, f):
+    nrm = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    keep = np.abs(nrm).sum(1) > 0
+    return f[keep]
+
+
+# ---- configurations -----------------------------------------------------------------
+
+def icosphere_c1(seed: int = 1, snap: float | None = None) -> MeshCase:
+    """Icosphere L3 (642 V, 1,280 T), radius 0.45, seeded Haar rotation, centred at
+    (0.5,0.5,0.5); 64^3 grid over [0,1]^3. snap=2**-6 snaps every coordinate to that
+    lattice (rays then hit edges exactly); snap=2**-7 also puts vertices on the column
+    lines (vertex hits and the nudge path). Zero-area triangles after snapping are
+    dropped."""
+    rng = np.random.default_rng(seed)
+    v, f = icosphere(3)
+    v = v @ random_rotation(rng).T * 0.45 + 0.5
+    name = "icosphere_l3_rot"
+    if snap:
+        v = np.round(v / snap) * snap
+        f = _drop_degenerate(v, f)
+        name += f"_snap{int(round(-math.log2(snap)))}"
+    return MeshCase(name, anchored(v), f.astype(np.int32), 64, 0,
+                    {"seed": seed, "snap": snap})
+
+
+def torus_c3(seed: int = 3, res: int = 256, nu: int = 256, nv: int = 128) -> MeshCase:
+    """Torus R=0.3, r=0.12 centred at (0.5,0.5,0.5), nu x nv parametric grid
+    (32,768 V / 65,536 T at 256 x 128), each vertex moved along its normal by
+    U(-1e-3, 1e-3); res^3 grid over [0,1]^3."""
+    rng = np.random.default_rng(seed)
+    v, f, nrm = torus(0.3, 0.12, nu, nv)
+    v = v + nrm * rng.uniform(-1e-3, 1e-3, size=(v.shape[0], 1)) + 0.5
+    return MeshCase(f"torus_{nu}x{nv}", anchored(v), f, res, 0,
+                    {"seed": seed, "jitter": "U(-1e-3,1e-3) along normals"})
+
+
+def harmonic_sphere(level_or_freq: int, amplitude: float, r0: float, rng,
+                    geodesic=False):
+    v, f = geodesic_sphere(level_or_freq) if geodesic else icosphere(level_or_freq)
+    s = real_sph_harm_sum(v, 6, rng)
+    return v * (r0 * (1.0 + amplitude * s))[:, None], f
+
+
+def harmonic_c4(seed: int = 4, res: int = 128, level: int = 5) -> MeshCase:
+    """Icosphere L5 (10,242 V / 20,480 T), radius 0.35*(1 + 0.05*S) with S a seeded
+    random real spherical-harmonic sum up to degree 6 scaled to max |S| = 1, centred at
+    (0.5,0.5,0.5); res^3 grid over [0,1]^3."""
+    rng = np.random.default_rng(seed)
+    v, f = harmonic_sphere(level, 0.05, 0.35, rng)
+    return MeshCase(f"harmonic_l{level}", anchored(v + 0.5), f, res, 0, {"seed": seed})
+
+
+def upstream_grad(n: int, seed: int = 40) -> np.ndarray:
+    """Upstream dL/dSDF, i.i.d. N(0,1) float32."""
+    return np.random.default_rng(seed).standard_normal(n).astype(np.float32)
+
+
+def batch_c5(seed: int = 5, count: int = 64, tri_range=(500, 200_000),
+             res_range=(32, 256)) -> list[MeshCase]:
+    """`count` closed meshes drawn from {geodesic sphere, torus, harmonic sphere}, each
+    with a Haar rotation and a scale U[0.3, 0.9] (the mesh's bounding radius becomes
+    scale/2, centred in [0,1]^3); triangle counts and per-mesh grid resolutions are
+    log-uniform in the given ranges."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(count):
+        kind = int(rng.integers(3))
+        target = math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))
+        res = int(round(math.exp(rng.uniform(math.log(res_range[0]),
+                                             math.log(res_range[1])))))
+        scale = rng.uniform(0.3, 0.9)
+        rot = random_rotation(rng)
+        if kind == 1:
+            nv = max(3, int(round(math.sqrt(target / 4))))
+            nu = 2 * nv
+            v, f, nrm = torus(0.3, 0.12, nu, nv)
+            name = f"torus_{nu}x{nv}"
+        else:
+            freq = max(1, int(round(math.sqrt(target / 20))))
+            if kind == 0:
+                v, f = geodesic_sphere(freq)
+                name = f"sphere_f{freq}"
+            else:
+                v, f = harmonic_sphere(freq, 0.05, 1.0, rng, geodesic=True)
+                name = f"harmonic_f{freq}"
+        v = v @ rot.T
+        v = v / np.max(np.linalg.norm(v, axis=1)) * (scale / 2) + 0.5
+        out.append(MeshCase(f"c5_{m:02d}_{name}", anchored(v), f.astype(np.int32), res,
+                            0, {"kind": kind, "scale": scale}))
+    return out
+
+
+# ---- predicate tuples ---------------------------------------------------------------
+
+_NORM81 = None
+
+
+def _norm_vectors(n2: int = 81) -> np.ndarray:
+    global _NORM81
+    if _NORM81 is None:
+        r = int(math.isqrt(n2))
+        g = np.arange(-r, r + 1)
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        m = X * X + Y * Y + Z * Z == n2
+        _NORM81 = np.stack([X[m], Y[m], Z[m]], -1).astype(np.float64)
+    return _NORM81
+
+
+def _grid24(rng, shape, lo=-1.0, hi=1.0):
+    """Uniform values on the 2^-24 grid in [lo, hi]."""
+    return np.round(rng.uniform(lo, hi, size=shape) * 2.0 ** 24) / 2.0 ** 24
+
+
+def _ulp_perturb(rng, soa, cols):
+    """Move one random coordinate of each selected tuple by +-1 ulp (nextafter)."""
+    n = cols.size
+    if n == 0:
+        return
+    plane = rng.integers(soa.shape[0], size=n)
+    up = rng.integers(2, size=n).astype(bool)
+    vals = soa[plane, cols]
+    soa[plane, cols] = np.where(up, np.nextafter(vals, np.inf), np.nextafter(vals, -np.inf))
+
+
+def tuples_orient3d(n: int, seed: int = 2, degenerate: float = 0.10):
+    """(12, n) float64 SoA orient3d tuples (plane = slot*3 + axis). 90 % uniform in
+    [-1,1]^3. 10 % near-degenerate, half exactly coplanar (a, b, c on the 2^-24 grid and
+    d = a + s(b-a) + t(c-a) with s, t in {k/16}, all exact in FP64) and half the same
+    with one coordinate moved by +-1 ulp. Returns (soa, kind) with kind 0 = uniform,
+    1 = coplanar, 2 = coplanar +-1 ulp."""
+    rng = np.random.default_rng(seed)
+    soa = rng.uniform(-1.0, 1.0, size=(12, n))
+    kind = np.zeros(n, dtype=np.int8)
+    nd = int(round(n * degenerate))
+    idx = rng.choice(n, size=nd, replace=False) if nd else np.zeros(0, np.int64)
+    if nd:
+        a = _grid24(rng, (3, nd))
+        b = _grid24(rng, (3, nd))
+        c = _grid24(rng, (3, nd))
+        s = rng.integers(-16, 17, size=nd) / 16.0
+        t = rng.integers(-16, 17, size=nd) / 16.0
+        d = a + s * (b - a) + t * (c - a)
+        # the four points go to random slots so the coplanar point is not always d
+        pts = np.stack([a, b, c, d])            # (4, 3, nd)
+        perm = np.argsort(rng.random((4, nd)), axis=0)
+        pts = np.take_along_axis(pts, perm[:, None, :], axis=0)
+        soa[:, idx] = pts.reshape(12, nd)
+        kind[idx] = 1
+        half = idx[: nd // 2]
+        _ulp_perturb(rng, soa, half)
+        kind[half] = 2
+    return np.ascontiguousarray(soa), kind
+
+
+def tuples_insphere(n: int, seed: int = 22, degenerate: float = 0.10):
+    """(15, n) float64 SoA insphere tuples. 90 % uniform in [-1,1]^3. 10 % near-
+    degenerate, half exactly cospherical (centre on the 2^-24 grid plus distinct integer
+    vectors of norm 9 times m*2^-24, all exact in FP64) and half the same with one
+    coordinate moved by +-1 ulp. kind: 0 uniform, 1 cospherical, 2 cospherical +-1 ulp."""
+    rng = np.random.default_rng(seed)
+    soa = rng.uniform(-1.0, 1.0, size=(15, n))
+    kind = np.zeros(n, dtype=np.int8)
+    nd = int(round(n * degenerate))
+    idx = rng.choice(n, size=nd, replace=False) if nd else np.zeros(0, np.int64)
+    if nd:
+        vecs = _norm_vectors(81)
+        centre = _grid24(rng, (3, nd), -0.5, 0.5)
+        m = rng.integers(1, 2 ** 24 // 20, size=nd).astype(np.float64) * 2.0 ** -24
+        # five distinct vectors per tuple
+        choice = np.argsort(rng.random((vecs.shape[0], nd)), axis=0)[:5]   # (5, nd)
+        pts = centre[None, :, :] + vecs[choice].transpose(0, 2, 1) * m[None, None, :]
+        soa[:, idx] = pts.reshape(15, nd)
+        kind[idx] = 1
+        half = idx[: nd // 2]
+        _ulp_perturb(rng, soa, half)
+        kind[half] = 2
+    return np.ascontiguousarray(soa), kind

@@ -1,0 +1,173 @@
+/* cmg_hip.h — C ABI of libcmg_hip: CyberMeshGenerator's predicate and voxelization
+ * path on AMD Instinct MI355X (gfx950).
+ *
+ * Plain C types only (pointers, sizes, status codes). Three groups of entry points:
+ *
+ *  1. The stable C ABI subset the reference exports for voxelization, byte-compatible
+ *     with bindings/c/include/cmg_c/cmg_c.h (declarations cited per function). A caller
+ *     linked against libcmg_c.so for these symbols can link libcmg_hip.so instead.
+ *  2. Additive batched-predicate and SDF-adjoint entry points (host pointers), proposed
+ *     as C-ABI additions for this path; they have no counterpart in cmg_c.h.
+ *  3. The device layer that replaces src/backend/cuda (include/cmg/backend/
+ *     cuda_kernels.hpp:11-21): device-pointer, stream-ordered entry points.
+ *
+ * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
+ * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
+ * Status codes, errbuf conventions and handle ownership follow cmg_c.h:18-33 and
+ * bindings/c/src/cmg_c.cpp:34-52: errbuf (optional) receives a NUL-terminated message,
+ * no C++ exception crosses the boundary, HIP failures map to CMG_ERR_INTERNAL.
+ */
+#ifndef CMG_HIP_H
+#define CMG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. the reference's C ABI subset ---------------------------------------------- */
+
+/* cmg_c.h:18-27 */
+typedef enum cmg_status {
+    CMG_OK = 0,
+    CMG_ERR_INVALID_INPUT = 1,
+    CMG_ERR_SELF_INTERSECTION = 2,
+    CMG_ERR_UNSUPPORTED_OPTION = 3,
+    CMG_ERR_QUALITY_NOT_MET = 4,
+    CMG_ERR_NOT_IMPLEMENTED = 5,
+    CMG_ERR_INTERNAL = 6,
+    CMG_ERR_PARSE = 7
+} cmg_status;
+
+/* cmg_c.h:30-33 (opaque handles) */
+typedef struct cmg_plc cmg_plc;
+typedef struct cmg_voxels cmg_voxels;
+
+/* cmg_c.h:36-37 */
+cmg_plc* cmg_plc_create(void);
+void cmg_plc_destroy(cmg_plc*);
+
+/* cmg_c.h:44 — replaces the point set from a flat [x0,y0,z0, ...] array */
+cmg_status cmg_plc_set_points(cmg_plc*, const double* xyz, size_t count);
+
+/* cmg_c.h:51-52 — appends a one-polygon facet (count >= 3 indices) */
+cmg_status cmg_plc_add_facet(cmg_plc*, const int* poly_vertex_indices, size_t count,
+                             int marker);
+
+/* cmg_c.h:137-138 — mode 1 = signed distance, anything else = occupancy; grid as in
+ * src/voxelize/voxelize.cpp:52-74. On success *out is a handle to destroy with
+ * cmg_voxels_destroy; on failure *out is NULL and errbuf is filled. */
+cmg_status cmg_plc_voxelize(const cmg_plc*, int resolution, int mode, int pad,
+                            cmg_voxels** out, char* errbuf, size_t errbuf_len);
+
+/* cmg_c.h:141-153 */
+void cmg_voxels_destroy(cmg_voxels*);
+void cmg_voxels_dims(const cmg_voxels*, int* nx, int* ny, int* nz);
+void cmg_voxels_origin(const cmg_voxels*, double* x, double* y, double* z);
+double cmg_voxels_spacing(const cmg_voxels*);
+const unsigned char* cmg_voxels_occupancy(const cmg_voxels*);
+const float* cmg_voxels_distance(const cmg_voxels*);
+
+/* cmg_c.h:156 */
+const char* cmg_version(void);
+
+/* ---- 2. additive host-pointer entry points ---------------------------------------- */
+
+/* Predicate modes. CMG_PRED_EXACT: exact signs (Shewchuk orient3dexact/insphereexact,
+ * src/predicates/predicates.cpp:1609,3237). CMG_PRED_COMPAT: the signs
+ * cmg::robust::orient3d/insphere return as shipped (naive FP64 determinant when nonzero,
+ * exact sign when it is 0; src/predicates/robust.cpp:28-50, predicates.cpp:2230,4113). */
+enum { CMG_PRED_EXACT = 0, CMG_PRED_COMPAT = 1 };
+
+/* Batched orient3d signs over index quadruples into AoS points: the batch of
+ * cmg::robust::orient3d_signs (src/predicates/robust.cpp:68-105), int8 output. */
+cmg_status cmg_orient3d_signs(const double* xyz, size_t npts, const int* quads, size_t n,
+                              int mode, signed char* out, char* errbuf, size_t errbuf_len);
+
+/* Batched orient3d values: cmg::robust::orient3d_batch (robust.cpp:52-66). The naive
+ * FP64 determinant (bit-identical) when nonzero; when it is exactly 0 the exact
+ * determinant's leading expansion component (exact sign). */
+cmg_status cmg_orient3d_values(const double* xyz, size_t npts, const int* quads, size_t n,
+                               double* out, char* errbuf, size_t errbuf_len);
+
+/* Batched insphere signs over index quintuples (a,b,c,d,e) into AoS points; the batched
+ * form of cmg::robust::insphere (robust.cpp:44-50). */
+cmg_status cmg_insphere_signs(const double* xyz, size_t npts, const int* quints, size_t n,
+                              int mode, signed char* out, char* errbuf, size_t errbuf_len);
+
+/* SDF adjoint for a grid produced by cmg_plc_voxelize(mode=1) from `plc`: given
+ * grad_sdf (dL/dSDF per cell, flat idx = (k*ny + j)*nx + i), writes dL/dxyz
+ * (3 floats per PLC point, anchors included) to grad_xyz. The sign is held constant;
+ * cells at distance 0 carry no gradient. */
+cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,
+                                   const float* grad_sdf, float* grad_xyz, char* errbuf,
+                                   size_t errbuf_len);
+
+/* Name of the backend that served the calling thread's last predicate or voxelize
+ * call ("hip", or "none" before any call). Mirrors cmg::backend::last_backend
+ * (src/backend/dispatch.cpp:65). */
+const char* cmg_backend_last(void);
+
+/* ---- 3. device layer (device pointers, stream-ordered) ----------------------------- */
+
+/* cuda_kernels.hpp:14 — 1 when a usable HIP device is present. */
+int cmg_hip_available(void);
+
+/* cuda_kernels.hpp:16-21, same arguments: host AoS points xyz[3*npts], host quadruples
+ * tets[4*n], host out_sign[n]. The signs are final compat signs (the reference's CPU
+ * escalation of zeros, robust.cpp:97-98, becomes a no-op). Blocking. Returns 0 on
+ * success, nonzero on a HIP failure. */
+int cmg_hip_orient3d_signs(const double* xyz, int npts, const int* tets, int n,
+                           int* out_sign);
+
+/* Per-stage lane counts of the last batched-sign call on a workspace. */
+typedef struct cmg_pred_stats {
+    uint64_t lanes;        /* tuples in the batch */
+    uint64_t fp32;         /* decided by the FP32 filter (exact mode only) */
+    uint64_t fp64;         /* decided by the FP64 stage */
+    uint64_t exact;        /* resolved by expansion arithmetic */
+    uint64_t wide;         /* of those, re-run with wide expansion capacity */
+    uint64_t failed;       /* exceeded even the wide capacity (sign left 0) */
+} cmg_pred_stats;
+
+/* Workspace bytes a batched-sign call over n tuples needs (npts 4 = orient3d,
+ * 5 = insphere). */
+size_t cmg_hip_pred_workspace_size(size_t n, int npts);
+
+/* Batched signs over device-resident structure-of-arrays tuples: plane q = slot*3 + axis
+ * (orient3d: 12 planes, insphere: 15) holds that coordinate of every tuple at
+ * soa[q*ld + i]; out[i] receives -1/0/+1. `workspace` is a device buffer of at least
+ * cmg_hip_pred_workspace_size bytes, or NULL to use the library's own (then the call
+ * synchronises `stream` before returning). `stream` is a hipStream_t (NULL = default).
+ * With a caller workspace the call is stream-ordered and returns at once. */
+cmg_status cmg_hip_orient3d_signs_soa(const double* soa, size_t n, size_t ld, int mode,
+                                      signed char* out, void* workspace,
+                                      size_t workspace_bytes, void* stream, char* errbuf,
+                                      size_t errbuf_len);
+cmg_status cmg_hip_insphere_signs_soa(const double* soa, size_t n, size_t ld, int mode,
+                                      signed char* out, void* workspace,
+                                      size_t workspace_bytes, void* stream, char* errbuf,
+                                      size_t errbuf_len);
+
+/* Reads the per-stage counts of the last call that used `workspace` (synchronises
+ * `stream`). */
+cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
+                              cmg_pred_stats* out, char* errbuf, size_t errbuf_len);
+
+/* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
+ * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
+ * "sdf_voxels", "sdf_backward", "occupancy_columns", "batch_sdf"). Events are recorded on the stream
+ * the kernel is launched on. */
+void cmg_hip_kernel_timer(const char* name, int enable);
+
+/* Total milliseconds and launch count of the timed launches of `name` since the last
+ * read (synchronises on the recorded events, then clears them). */
+cmg_status cmg_hip_kernel_time(const char* name, double* total_ms, uint64_t* count,
+                               char* errbuf, size_t errbuf_len);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+#endif /* CMG_HIP_H */

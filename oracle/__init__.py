@@ -1,0 +1,294 @@
+"""CPU oracle for the predicate and voxelization path (ctypes over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against. Only tests/,
+``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg import this module;
+the shipped package never does.
+
+Every function restates the CyberMeshGenerator behaviour cited in oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+EXACT = 0
+COMPAT = 1
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    return _LIB_PATH
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    P = ctypes.c_void_p
+    sz = ctypes.c_size_t
+    i32 = ctypes.c_int
+    d = ctypes.c_double
+    lib.orc_orient3d_naive.restype = d
+    lib.orc_orient3d_naive.argtypes = [P] * 4
+    lib.orc_insphere_naive.restype = d
+    lib.orc_insphere_naive.argtypes = [P] * 5
+    lib.orc_orient3d_exact.restype = i32
+    lib.orc_orient3d_exact.argtypes = [P] * 4
+    lib.orc_insphere_exact.restype = i32
+    lib.orc_insphere_exact.argtypes = [P] * 5
+    lib.orc_orient3d_sign.restype = i32
+    lib.orc_orient3d_sign.argtypes = [P] * 4 + [i32]
+    lib.orc_insphere_sign.restype = i32
+    lib.orc_insphere_sign.argtypes = [P] * 5 + [i32]
+    lib.orc_orient3d_batch.argtypes = [P, sz, sz, i32, P, i32]
+    lib.orc_insphere_batch.argtypes = [P, sz, sz, i32, P, i32]
+    lib.orc_orient3d_indexed.argtypes = [P, P, sz, i32, P, i32]
+    lib.orc_fan_triangulate.restype = sz
+    lib.orc_fan_triangulate.argtypes = [P, sz, P, P]
+    lib.orc_make_grid.restype = i32
+    lib.orc_make_grid.argtypes = [P, sz, i32, i32, P]
+    lib.orc_occupancy.argtypes = [P, P, sz, P, P, P, i32]
+    lib.orc_point_tri_dist2.restype = d
+    lib.orc_point_tri_dist2.argtypes = [P] * 5
+    lib.orc_sdf_shell.argtypes = [P, P, sz, P, P, P, sz, sz]
+    lib.orc_sdf_brute.argtypes = [P, P, sz, P, P, P, i32]
+    lib.orc_sdf_bvh.argtypes = [P, P, sz, P, P, P, P, P, sz, sz, i32]
+    lib.orc_sdf_grad.argtypes = [P, sz, P, sz, P, P, P, P, i32]
+    lib.orc_sdf_fd.restype = d
+    lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, i32, i32, d, i32]
+    return lib
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("corner", ctypes.c_double * 3), ("spacing", ctypes.c_double),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
+
+
+@dataclass
+class Grid:
+    corner: tuple
+    spacing: float
+    dims: tuple  # (nx, ny, nz)
+
+    @property
+    def origin(self):
+        """Centre of cell (0,0,0) (voxelize.cpp:389-391)."""
+        return tuple(c + 0.5 * self.spacing for c in self.corner)
+
+    @property
+    def ncell(self) -> int:
+        nx, ny, nz = self.dims
+        return nx * ny * nz
+
+    def _c(self) -> _Grid:
+        g = _Grid()
+        for a in range(3):
+            g.corner[a] = self.corner[a]
+        g.spacing = self.spacing
+        g.nx, g.ny, g.nz = self.dims
+        return g
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---- predicates ---------------------------------------------------------------------
+
+def orient3d_naive(a, b, c, d) -> float:
+    a, b, c, d = (_f64(x) for x in (a, b, c, d))
+    return lib().orc_orient3d_naive(_p(a), _p(b), _p(c), _p(d))
+
+
+def insphere_naive(a, b, c, d, e) -> float:
+    a, b, c, d, e = (_f64(x) for x in (a, b, c, d, e))
+    return lib().orc_insphere_naive(_p(a), _p(b), _p(c), _p(d), _p(e))
+
+
+def orient3d_sign(a, b, c, d, mode=EXACT) -> int:
+    a, b, c, d = (_f64(x) for x in (a, b, c, d))
+    return lib().orc_orient3d_sign(_p(a), _p(b), _p(c), _p(d), mode)
+
+
+def insphere_sign(a, b, c, d, e, mode=EXACT) -> int:
+    a, b, c, d, e = (_f64(x) for x in (a, b, c, d, e))
+    return lib().orc_insphere_sign(_p(a), _p(b), _p(c), _p(d), _p(e), mode)
+
+
+def orient3d_batch(soa: np.ndarray, mode=EXACT, nthreads=0) -> np.ndarray:
+    """soa: (12, n) float64 planes (slot*3 + axis)."""
+    soa = _f64(soa)
+    assert soa.shape[0] == 12
+    n = soa.shape[1]
+    out = np.empty(n, dtype=np.int8)
+    lib().orc_orient3d_batch(_p(soa), n, n, mode, _p(out), nthreads)
+    return out
+
+
+def insphere_batch(soa: np.ndarray, mode=EXACT, nthreads=0) -> np.ndarray:
+    """soa: (15, n) float64 planes (slot*3 + axis)."""
+    soa = _f64(soa)
+    assert soa.shape[0] == 15
+    n = soa.shape[1]
+    out = np.empty(n, dtype=np.int8)
+    lib().orc_insphere_batch(_p(soa), n, n, mode, _p(out), nthreads)
+    return out
+
+
+def orient3d_indexed(xyz, quads, mode=COMPAT, nthreads=0) -> np.ndarray:
+    xyz = _f64(xyz).reshape(-1, 3)
+    quads = _i32(quads).reshape(-1, 4)
+    out = np.empty(quads.shape[0], dtype=np.int8)
+    lib().orc_orient3d_indexed(_p(xyz), _p(quads), quads.shape[0], mode, _p(out), nthreads)
+    return out
+
+
+# ---- voxelization -------------------------------------------------------------------
+
+def fan_triangulate(polygons) -> np.ndarray:
+    off = np.zeros(len(polygons) + 1, dtype=np.int32)
+    for i, p in enumerate(polygons):
+        off[i + 1] = off[i] + len(p)
+    idx = _i32(np.concatenate([np.asarray(p, dtype=np.int32) for p in polygons])
+               if polygons else np.zeros(0, np.int32))
+    cap = int(sum(max(len(p) - 2, 0) for p in polygons))
+    tris = np.zeros((max(cap, 1), 3), dtype=np.int32)
+    m = lib().orc_fan_triangulate(_p(off), len(polygons), _p(idx), _p(tris))
+    return tris[:m].copy()
+
+
+def make_grid(pts, resolution: int, pad: int) -> Grid:
+    pts = _f64(pts).reshape(-1, 3)
+    g = _Grid()
+    rc = lib().orc_make_grid(_p(pts), pts.shape[0], resolution, pad, ctypes.byref(g))
+    if rc != 0:
+        raise ValueError("zero-extent point set")
+    return Grid(tuple(g.corner), g.spacing, (g.nx, g.ny, g.nz))
+
+
+def occupancy(pts, tris, grid: Grid, nthreads=0, with_stats=False):
+    """uint8 occupancy, flat idx = (k*ny + j)*nx + i."""
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.zeros(grid.ncell, dtype=np.uint8)
+    stats = np.zeros(2, dtype=np.int64)
+    g = grid._c()
+    lib().orc_occupancy(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g), _p(occ),
+                        _p(stats), nthreads)
+    return (occ, stats) if with_stats else occ
+
+
+def point_tri_dist2(p, a, b, c):
+    p, a, b, c = (_f64(x) for x in (p, a, b, c))
+    bary = np.zeros(3)
+    d2 = lib().orc_point_tri_dist2(_p(p), _p(a), _p(b), _p(c), _p(bary))
+    return d2, bary
+
+
+def sdf_shell(pts, tris, grid: Grid, occ, first=0, count=0) -> np.ndarray:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    out = np.zeros(grid.ncell, dtype=np.float32)
+    g = grid._c()
+    lib().orc_sdf_shell(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g), _p(occ),
+                        _p(out), first, count)
+    return out
+
+
+def sdf_brute(pts, tris, grid: Grid, occ, nthreads=0) -> np.ndarray:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    out = np.zeros(grid.ncell, dtype=np.float32)
+    g = grid._c()
+    lib().orc_sdf_brute(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g), _p(occ),
+                        _p(out), nthreads)
+    return out
+
+
+def sdf_bvh(pts, tris, grid: Grid, occ, first=0, count=0, nthreads=0):
+    """Returns (dist float32, best FP64 d^2, argmin triangle) over cells
+    [first, first+count) (count=0: all); entries outside the range are zero."""
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    dist = np.zeros(grid.ncell, dtype=np.float32)
+    best = np.zeros(grid.ncell, dtype=np.float64)
+    arg = np.full(grid.ncell, -1, dtype=np.int32)
+    g = grid._c()
+    lib().orc_sdf_bvh(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g), _p(occ),
+                      _p(dist), _p(best), _p(arg), first, count, nthreads)
+    return dist, best, arg
+
+
+def sdf_grad(pts, tris, grid: Grid, occ, gout, nthreads=0) -> np.ndarray:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    gout = np.ascontiguousarray(gout, dtype=np.float32)
+    grad = np.zeros_like(pts)
+    g = grid._c()
+    lib().orc_sdf_grad(_p(pts), pts.shape[0], _p(tris), tris.shape[0], ctypes.byref(g),
+                       _p(occ), _p(gout), _p(grad), nthreads)
+    return grad
+
+
+def sdf_fd(pts, tris, grid: Grid, occ, gout, vert: int, axis: int, h=1e-5,
+           nthreads=0) -> float:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    occ = np.ascontiguousarray(occ, dtype=np.uint8)
+    gout = np.ascontiguousarray(gout, dtype=np.float32)
+    g = grid._c()
+    return lib().orc_sdf_fd(_p(pts), pts.shape[0], _p(tris), tris.shape[0],
+                            ctypes.byref(g), _p(occ), _p(gout), vert, axis, h, nthreads)
+
+
+def voxelize(pts, polygons_or_tris, resolution=64, pad=1, mode="occupancy", nthreads=0):
+    """cmg::voxelize::voxelize restated (voxelize.cpp:362-398): returns
+    (grid, occupancy-or-distance flat array). polygons_or_tris: (M,3) int array of
+    triangles or a list of polygons (fan-triangulated)."""
+    pts = _f64(pts).reshape(-1, 3)
+    if resolution < 1:
+        raise ValueError("voxelize: resolution must be >= 1")
+    if pts.shape[0] == 0:
+        raise ValueError("voxelize: PLC has no points")
+    if isinstance(polygons_or_tris, np.ndarray) and polygons_or_tris.ndim == 2 \
+            and polygons_or_tris.shape[1] == 3:
+        tris = _i32(polygons_or_tris)
+    else:
+        tris = fan_triangulate(list(polygons_or_tris))
+    if tris.shape[0] == 0:
+        raise ValueError("voxelize: PLC has no boundary triangles")
+    grid = make_grid(pts, resolution, pad)
+    occ = occupancy(pts, tris, grid, nthreads)
+    if mode in ("sdf", 1):
+        return grid, sdf_bvh(pts, tris, grid, occ, nthreads=nthreads)[0]
+    return grid, occ

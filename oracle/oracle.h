@@ -1,0 +1,132 @@
+/* oracle.h — CPU restatement of CyberMeshGenerator's predicate and voxelization path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker the GPU path is compared
+ * against; nothing in the shipped package links, loads or calls it. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Each function cites the reference file:line whose behaviour it restates
+ * (paths are relative to the CyberMeshGenerator source tree).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off: no FMA contraction, so every
+ * FP64 expression rounds exactly as the reference's x86-64 build does).
+ */
+#ifndef CMG_ORACLE_H
+#define CMG_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- predicates ---------------------------------------------------- */
+
+/* Predicate modes: which oracle a sign is taken from.
+ *  ORC_EXACT  — the exact sign of the determinant (Shewchuk's orient3dexact /
+ *               insphereexact semantics, src/predicates/predicates.cpp:1609,3237).
+ *  ORC_COMPAT — what cmg::robust::orient3d / insphere return as shipped: exactinit is
+ *               called with zero bounding-box extents (src/predicates/robust.cpp:28-34),
+ *               which zeroes the static filters (predicates.cpp:645-646), so the naive
+ *               FP64 determinant's sign is returned whenever it is nonzero
+ *               (predicates.cpp:2230-2231, 4113) and the exact sign only when it is 0. */
+enum { ORC_EXACT = 0, ORC_COMPAT = 1 };
+
+/* Naive FP64 determinants in the reference's operation order
+ * (orient3d: predicates.cpp:2201-2222; insphere: predicates.cpp:4063-4106). */
+double orc_orient3d_naive(const double* pa, const double* pb, const double* pc,
+                          const double* pd);
+double orc_insphere_naive(const double* pa, const double* pb, const double* pc,
+                          const double* pd, const double* pe);
+
+/* Exact signs (-1, 0, +1) by expansion arithmetic (Shewchuk 1997, CMU-CS-96-140). */
+int orc_orient3d_exact(const double* pa, const double* pb, const double* pc,
+                       const double* pd);
+int orc_insphere_exact(const double* pa, const double* pb, const double* pc,
+                       const double* pd, const double* pe);
+
+/* Signs in the given mode. */
+int orc_orient3d_sign(const double* pa, const double* pb, const double* pc,
+                      const double* pd, int mode);
+int orc_insphere_sign(const double* pa, const double* pb, const double* pc,
+                      const double* pd, const double* pe, int mode);
+
+/* Batched signs over structure-of-arrays tuples: plane q = slot*3 + axis holds
+ * coordinate `axis` of point `slot` for all n tuples at soa[q*ld + i]
+ * (orient3d: 12 planes, insphere: 15 planes). nthreads <= 0 uses OpenMP's default. */
+void orc_orient3d_batch(const double* soa, size_t n, size_t ld, int mode, int8_t* out,
+                        int nthreads);
+void orc_insphere_batch(const double* soa, size_t n, size_t ld, int mode, int8_t* out,
+                        int nthreads);
+
+/* Indexed form: xyz is AoS [3*npts], quads [4*n] (the shape of
+ * cmg::robust::orient3d_signs, src/predicates/robust.cpp:68-105). */
+void orc_orient3d_indexed(const double* xyz, const int* quads, size_t n, int mode,
+                          int8_t* out, int nthreads);
+
+/* ---------------- voxelization -------------------------------------------------- */
+
+typedef struct {
+    double corner[3]; /* world coordinate of the MIN corner of cell (0,0,0) */
+    double spacing;
+    int nx, ny, nz;
+} orc_grid;
+
+/* Fan-triangulate polygons (src/voxelize/voxelize.cpp:29-38). Polygon p has vertices
+ * idx[off[p] .. off[p+1]-1]. Writes 3 ints per triangle; returns the count. */
+size_t orc_fan_triangulate(const int* off, size_t npoly, const int* idx, int* tris);
+
+/* Bounding-box grid (voxelize.cpp:52-74). Returns 0, or -1 for zero extent. */
+int orc_make_grid(const double* pts, size_t npts, int resolution, int pad, orc_grid* g);
+
+/* Occupancy by exact vertical ray parity with the compat predicate, xy buckets, kNudge
+ * retries and the strict "height > zc" count (voxelize.cpp:78-195). occ has
+ * nx*ny*nz bytes, idx = (k*ny + j)*nx + i. stats (optional, 2 entries): columns that
+ * needed at least one nudge, columns where every nudge grazed. */
+void orc_occupancy(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
+                   uint8_t* occ, int64_t* stats, int nthreads);
+
+/* Ericson squared point-triangle distance, FP64, reference operation order
+ * (voxelize.cpp:197-247). bary (optional) receives the closest point's barycentric
+ * weights on a, b, c. */
+double orc_point_tri_dist2(const double* p, const double* a, const double* b,
+                           const double* c, double* bary);
+
+/* SDF by the reference's own 3-D bucket + Chebyshev-shell search
+ * (voxelize.cpp:251-358). Single-threaded like the reference. Cells are processed
+ * in flat order from `first` to `first+count-1` (count = 0 means all); cells outside
+ * that range are left untouched. */
+void orc_sdf_shell(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
+                   const uint8_t* occ, float* dist, size_t first, size_t count);
+
+/* SDF by testing every triangle (the brute-force check of tests/test_voxelize.cpp:
+ * 194-275). Same output as orc_sdf_shell. */
+void orc_sdf_brute(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
+                   const uint8_t* occ, float* dist, int nthreads);
+
+/* SDF by an exact BVH nearest-triangle search: the FP64 minimum over all triangles,
+ * identical to orc_sdf_brute, fast enough for 256^3 grids. Optional outputs: best
+ * (FP64 squared distance) and arg (index of a minimising triangle). Any of dist,
+ * best, arg may be NULL. Cells [first, first+count) only (count = 0: all). */
+void orc_sdf_bvh(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
+                 const uint8_t* occ, float* dist, double* best, int32_t* arg,
+                 size_t first, size_t count, int nthreads);
+
+/* FP64 SDF-magnitude adjoint: for L = sum_v g[v] * s[v] * d[v] with the sign s
+ * (+1 outside, -1 inside, from occ) held constant, accumulates dL/dxyz into
+ * grad[3*npts] (zeroed first). The closest feature is the minimising triangle's
+ * closest point; d = 0 carries zero gradient. */
+void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
+                  const orc_grid* g, const uint8_t* occ, const float* gout,
+                  double* grad, int nthreads);
+
+/* FP64 central finite difference of the same L with respect to coordinate `axis` of
+ * vertex `vert`, step h, signs held at their unperturbed values. */
+double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
+                  const orc_grid* g, const uint8_t* occ, const float* gout, int vert,
+                  int axis, double h, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

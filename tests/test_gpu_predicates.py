@@ -1,0 +1,137 @@
+"""Parity of the HIP batched predicates with the CPU oracle (bit-exact signs)."""
+import numpy as np
+import pytest
+
+import oracle
+from cybermeshgenerator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("mode", ["exact", "compat"])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 65, 1000, 1 << 16])
+def test_orient3d_soa_matches_oracle(gpu, mode, n):
+    soa, kind = W.tuples_orient3d(max(n, 1), seed=200 + n)
+    soa = soa[:, :n]
+    got = gpu.signs_soa(_dev(soa), mode).cpu().numpy()
+    want = oracle.orient3d_batch(soa, oracle.EXACT if mode == "exact" else oracle.COMPAT)
+    assert got.shape == (n,)
+    assert (got == want).all(), np.nonzero(got != want)[0][:10]
+
+
+@pytest.mark.parametrize("mode", ["exact", "compat"])
+@pytest.mark.parametrize("n", [1, 7, 4096, 1 << 15])
+def test_insphere_soa_matches_oracle(gpu, mode, n):
+    soa, kind = W.tuples_insphere(n, seed=300 + n)
+    got = gpu.signs_soa(_dev(soa), mode).cpu().numpy()
+    want = oracle.insphere_batch(soa, oracle.EXACT if mode == "exact" else oracle.COMPAT)
+    assert (got == want).all(), np.nonzero(got != want)[0][:10]
+
+
+def test_full_size_orient3d_batch_and_stage_fractions(gpu):
+    """The 2^24-tuple batch: bit-exact against the oracle in both modes, and every
+    lane accounted for by exactly one stage."""
+    import torch
+    n = 1 << 24
+    soa, kind = W.tuples_orient3d(n, seed=2)
+    d = _dev(soa)
+    ws = torch.empty(gpu.pred_workspace_size(n, 4), dtype=torch.uint8, device="cuda")
+    got = gpu.signs_soa(d, "exact", workspace=ws)
+    st = gpu.pred_stats(n, 4, workspace=ws)
+    got = got.cpu().numpy()
+    want = oracle.orient3d_batch(soa, oracle.EXACT)
+    assert (got == want).all(), int((got != want).sum())
+    assert st["fp32"] + st["fp64"] + st["exact"] == n
+    assert st["failed"] == 0
+    assert st["exact"] >= int((kind == 1).sum())     # exact zeros need the exact stage
+    gotc = gpu.signs_soa(d, "compat").cpu().numpy()
+    assert (gotc == oracle.orient3d_batch(soa, oracle.COMPAT)).all()
+
+
+def test_full_size_insphere_batch(gpu):
+    n = 1 << 22
+    soa, kind = W.tuples_insphere(n, seed=22)
+    got = gpu.signs_soa(_dev(soa), "exact").cpu().numpy()
+    want = oracle.insphere_batch(soa, oracle.EXACT)
+    assert (got == want).all(), int((got != want).sum())
+
+
+def test_strided_planes_and_odd_alignment(gpu):
+    """Planes with a leading dimension > n and an odd start (scalar-load path)."""
+    import torch
+    n = 5001
+    soa, _ = W.tuples_orient3d(n, seed=401)
+    big = torch.zeros((12, n + 7), dtype=torch.float64, device="cuda")
+    big[:, 1:n + 1] = _dev(soa)
+    view = big[:, 1:n + 1]
+    got = gpu.signs_soa(view, "exact").cpu().numpy()
+    assert (got == oracle.orient3d_batch(soa, oracle.EXACT)).all()
+
+
+def test_indexed_matches_reference_backend_test(gpu):
+    """tests/test_backend.cpp:36-54: 10^5 random candidates over 2,000 LCG points equal
+    the per-candidate CPU sign, and the backend reports the device path."""
+    s = 9
+
+    def nx():
+        nonlocal s
+        s = (s * 1664525 + 1013904223) & 0xFFFFFFFF
+        return (s >> 8) / float(1 << 24)
+
+    pts = np.array([[nx(), nx(), nx()] for _ in range(2000)])
+    s2 = 123
+    quads = np.empty((100000, 4), dtype=np.int32)
+    for i in range(100000):
+        for k in range(4):
+            s2 = (s2 * 1664525 + 1013904223) & 0xFFFFFFFF
+            quads[i, k] = (s2 >> 8) % 2000
+    got = gpu.orient3d_signs(pts, quads, "compat")
+    assert gpu.last_backend() == "hip"
+    want = oracle.orient3d_indexed(pts, quads, oracle.COMPAT)
+    assert (got == want).all()
+    # degenerate quadruples (repeated indices) are exact zeros
+    assert (got[(quads[:, 0] == quads[:, 1])] == 0).all()
+
+
+def test_indexed_insphere_and_values(gpu):
+    rng = np.random.default_rng(402)
+    pts = np.round(rng.uniform(-1, 1, (500, 3)) * 64) / 64    # many exact degeneracies
+    quints = rng.integers(0, 500, size=(20000, 5)).astype(np.int32)
+    for mode, om in (("exact", oracle.EXACT), ("compat", oracle.COMPAT)):
+        got = gpu.insphere_signs(pts, quints, mode)
+        want = np.array([oracle.insphere_sign(*pts[q], om) for q in quints[:4000]])
+        assert (got[:4000] == want).all()
+    quads = quints[:, :4].copy()
+    vals = gpu.orient3d_batch(pts, quads)
+    naive = np.array([oracle.orient3d_naive(*pts[q]) for q in quads])
+    nz = naive != 0
+    assert (vals[nz] == naive[nz]).all()                    # bit-identical when nonzero
+    sg = np.sign(vals[~nz]).astype(np.int8)
+    assert (sg == oracle.orient3d_indexed(pts, quads[~nz], oracle.COMPAT)).all()
+
+
+def test_device_layer_signature(gpu):
+    """cmg_hip_orient3d_signs keeps cuda::orient3d_signs' argument list
+    (cuda_kernels.hpp:16-21) and returns final compat signs."""
+    import ctypes
+    rng = np.random.default_rng(403)
+    pts = rng.uniform(-1, 1, (100, 3))
+    tets = rng.integers(0, 100, size=(70000, 4)).astype(np.int32)
+    out = np.zeros(70000, dtype=np.int32)
+    rc = gpu.lib().cmg_hip_orient3d_signs(pts.ctypes.data_as(ctypes.c_void_p), 100,
+                                          tets.ctypes.data_as(ctypes.c_void_p), 70000,
+                                          out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    assert (out == oracle.orient3d_indexed(pts, tets, oracle.COMPAT)).all()
+
+
+def test_invalid_inputs_raise(gpu):
+    with pytest.raises(RuntimeError, match="out of range"):
+        gpu.orient3d_signs(np.zeros((3, 3)), np.array([[0, 1, 2, 3]]))
+    with pytest.raises(ValueError):
+        gpu.orient3d_signs(np.zeros((4, 3)), np.array([[0, 1, 2, 3]]), mode="fast")
