@@ -1,11 +1,39 @@
-// voxelize.hip — occupancy and signed-distance voxelization on gfx950 (in progress).
+// voxelize.hip — occupancy voxelization on gfx950, bit-exact with
+// src/voxelize/voxelize.cpp.
+//
+// Occupancy (voxelize.cpp:78-195). The xy bucket of a column is reproduced exactly:
+// triangle t belongs to column (i, j) iff i0 <= i <= i1 and j0 <= j <= j1, with the
+// reference's clamp(floor((x - corner) / spacing)) ranges over the three vertices.
+// Columns are processed in 8x8 tiles, one 64-lane workgroup per tile, one lane per
+// column. Each tile's triangle list (every triangle whose xy range touches the tile) is
+// streamed through LDS in chunks of 64 triangles, stored as structure-of-arrays FP64.
+// A lane runs the reference's crossing test (compat orient3d, FP64 in the reference's
+// operation order) on the triangles of its own bucket.
+//
+// Heights are not stored or sorted: the reference counts, per cell, the crossing heights
+// strictly above the cell centre zc_k and keeps the parity (voxelize.cpp:185-191). A
+// crossing at height h flips exactly the cells with zc_k < h, i.e. cells 0 .. K(h)-1, so
+// each lane keeps one bit per cell boundary ("flip below K") in a per-column bitmask and
+// the final parity of cell k is the XOR of the bits above k.
+//
+// Grazing (an orient3d exactly 0) restarts the column with the next kNudge offset
+// (voxelize.cpp:148-165). If the last nudge grazes too, the reference keeps the
+// crossings found before the first grazing triangle of the bucket, which lists triangles
+// in ascending index order; pass 6 records the smallest grazing index and a final pass
+// replays that nudge counting only triangles below it.
+//
+// Compiled with -ffp-contract=off: every FP64 expression rounds as in the reference.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <vector>
 
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
+#include "expansion.hpp"
+#include "voxelize_internal.hpp"
 
 namespace cmg::hip {
 
@@ -34,14 +62,359 @@ bool make_grid(const double* xyz, size_t npts, int resolution, int pad, GridSpec
     return true;
 }
 
-void voxelize_host(const double*, size_t, const int*, size_t, const GridSpec&, bool,
-                   uint8_t*, float*) {
-    throw HipError("voxelize: not implemented yet");
+namespace {
+
+// voxelize.cpp:148-151
+__constant__ double kNudge[6][2] = {
+    {1.0e-6, 3.0e-7},   {-7.0e-7, 5.0e-7}, {4.0e-7, -9.0e-7},
+    {-2.0e-6, -1.0e-6}, {8.0e-7, 6.0e-7},  {-5.0e-7, -4.0e-6},
+};
+
+constexpr int kTile = 8;    // 8x8 columns per workgroup
+constexpr int kChunk = 64;  // triangles staged in LDS per step
+
+__device__ __forceinline__ int clamp_cell(double v, double corner, double spacing, int n) {
+    // voxelize.cpp:82-89 (floor, then clamp into the grid)
+    double f = floor((v - corner) / spacing);
+    if (!(f >= 0.0)) return 0;          // negative (or NaN)
+    if (f >= (double)n) return n - 1;
+    return (int)f;
 }
 
-void sdf_backward_host(const double*, size_t, const int*, size_t, const GridSpec&,
-                       const uint8_t*, const float*, float*) {
-    throw HipError("sdf backward: not implemented yet");
+// Exact sign for the rare lanes whose naive determinant is exactly 0; kept out of line
+// so the expansion arrays do not weigh on the common path.
+__device__ __noinline__ int o3_exact_slow(const double* pa, const double* pb,
+                                          const double* pc, const double* pd) {
+    int s = orient3d_exact_sign<40>(pa, pb, pc, pd);
+    if (s == kOverflow) s = orient3d_exact_sign<320>(pa, pb, pc, pd);
+    return s == kOverflow ? 0 : s;
+}
+
+// Sign of cmg::robust::orient3d(pa, pb, pc, pd) as shipped: the naive FP64 determinant
+// (predicates.cpp:2201-2222) when nonzero, else the exact sign.
+__device__ __forceinline__ int o3_compat(const double* pa, const double* pb, const double* pc,
+                                         const double* pd) {
+    double adx = pa[0] - pd[0], ady = pa[1] - pd[1], adz = pa[2] - pd[2];
+    double bdx = pb[0] - pd[0], bdy = pb[1] - pd[1], bdz = pb[2] - pd[2];
+    double cdx = pc[0] - pd[0], cdy = pc[1] - pd[1], cdz = pc[2] - pd[2];
+    double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    double cdxady = cdx * ady, adxcdy = adx * cdy;
+    double adxbdy = adx * bdy, bdxady = bdx * ady;
+    double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    if (det > 0.0) return 1;
+    if (det < 0.0) return -1;
+    return o3_exact_slow(pa, pb, pc, pd);
+}
+
+// voxelize.cpp:107-114
+__device__ __forceinline__ double plane_z(const double* a, const double* b, const double* c,
+                                          double x, double y) {
+    double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    double vx = c[0] - a[0], vy = c[1] - a[1], vz = c[2] - a[2];
+    double nx = uy * vz - uz * vy;
+    double ny = uz * vx - ux * vz;
+    double nz = ux * vy - uy * vx;
+    return a[2] - (nx * (x - a[0]) + ny * (y - a[1])) / nz;
+}
+
+__device__ __forceinline__ double zc_of(const GridSpec& g, int k) {
+    return g.corner[2] + (k + 0.5) * g.spacing;  // voxelize.cpp:187
+}
+
+// Number of cells k in [0, nz) with zc_k < h (the cells a crossing at h lies above).
+__device__ int cells_below(const GridSpec& g, double h) {
+    const int nz = g.nz;
+    if (!(h > zc_of(g, 0))) return 0;  // also NaN
+    if (h > zc_of(g, nz - 1)) return nz;
+    double f = (h - g.corner[2]) / g.spacing - 0.5;
+    int k = f < 0.0 ? 0 : (f > (double)(nz - 1) ? nz - 1 : (int)f);
+    while (k + 1 < nz && zc_of(g, k + 1) < h) ++k;
+    while (k > 0 && !(zc_of(g, k) < h)) --k;
+    return k + 1;
+}
+
+struct OccArgs {
+    const double* xyz;
+    const int* tris;
+    const int4* xyr;        // per triangle: i0, i1, j0, j1
+    const int* tile_off;    // CSR over tiles
+    const int* tile_items;
+    int ntx, nty;
+    GridSpec g;
+    unsigned long long* masks;  // per column: words x 64 bits of "flip below K"
+    int words;
+    uint8_t* occ;
+    unsigned* stats;        // [0] columns nudged, [1] columns where every nudge grazed
+};
+
+__global__ __launch_bounds__(256) void tri_xy_range_kernel(const double* __restrict__ xyz,
+                                                           const int* __restrict__ tris,
+                                                           int ntris, GridSpec g, int4* xyr,
+                                                           int* tile_count, int ntx) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntris) return;
+    int i0 = g.nx, i1 = 0, j0 = g.ny, j1 = 0;
+    for (int v = 0; v < 3; ++v) {
+        const double* p = xyz + 3 * (size_t)tris[3 * t + v];
+        int i = clamp_cell(p[0], g.corner[0], g.spacing, g.nx);
+        int j = clamp_cell(p[1], g.corner[1], g.spacing, g.ny);
+        i0 = min(i0, i);
+        i1 = max(i1, i);
+        j0 = min(j0, j);
+        j1 = max(j1, j);
+    }
+    xyr[t] = make_int4(i0, i1, j0, j1);
+    for (int tj = j0 / kTile; tj <= j1 / kTile; ++tj)
+        for (int ti = i0 / kTile; ti <= i1 / kTile; ++ti) atomicAdd(&tile_count[tj * ntx + ti], 1);
+}
+
+__global__ __launch_bounds__(256) void tile_fill_kernel(const int4* __restrict__ xyr, int ntris,
+                                                        const int* __restrict__ tile_off,
+                                                        int* cursor, int* items, int ntx) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntris) return;
+    int4 r = xyr[t];
+    for (int tj = r.z / kTile; tj <= r.w / kTile; ++tj)
+        for (int ti = r.x / kTile; ti <= r.y / kTile; ++ti) {
+            int tile = tj * ntx + ti;
+            int slot = atomicAdd(&cursor[tile], 1);
+            items[tile_off[tile] + slot] = t;
+        }
+}
+
+// Single-workgroup exclusive scan of n ints (n + 1 outputs; out[n] = total).
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(const int* __restrict__ in,
+                                                              int* __restrict__ out, int n) {
+    __shared__ int s[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < n ? in[i] : 0;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) out[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ __launch_bounds__(64) void occupancy_kernel(OccArgs a) {
+    const GridSpec& g = a.g;
+    const int tile = blockIdx.x;
+    const int tx = tile % a.ntx, ty = tile / a.ntx;
+    const int lane = threadIdx.x;
+    const int ci = tx * kTile + (lane % kTile), cj = ty * kTile + (lane / kTile);
+    const bool valid = ci < g.nx && cj < g.ny;
+    __shared__ double sv[9][kChunk];
+    __shared__ int4 sr[kChunk];
+    __shared__ int st[kChunk];
+    const int beg = a.tile_off[tile], end = a.tile_off[tile + 1];
+    unsigned long long* mask =
+        a.masks + ((size_t)(valid ? cj : 0) * g.nx + (valid ? ci : 0)) * a.words;
+    // voxelize.cpp:175-181
+    const double zmin = g.corner[2] - g.spacing;
+    const double zmax = g.corner[2] + (g.nz + 1) * g.spacing;
+    const double cx = g.corner[0] + (ci + 0.5) * g.spacing;
+    const double cy = g.corner[1] + (cj + 0.5) * g.spacing;
+
+    bool active = valid;
+    int cutoff = INT_MAX;
+    for (int pass = 0; pass < 8; ++pass) {
+        if (!__syncthreads_or(active)) break;
+        double x = cx, y = cy;
+        if (pass > 0) {
+            const int d = (pass < 7 ? pass : 6) - 1;
+            x = cx + kNudge[d][0] * g.spacing;
+            y = cy + kNudge[d][1] * g.spacing;
+        }
+        if (active)
+            for (int w = 0; w < a.words; ++w) mask[w] = 0ull;
+        bool grazed = false;
+        int first_graze = INT_MAX;
+        const double P[3] = {x, y, zmin};
+        const double Q[3] = {x, y, zmax};
+        for (int c0 = beg; c0 < end; c0 += kChunk) {
+            __syncthreads();
+            const int k = c0 + lane;
+            if (k < end) {
+                const int t = a.tile_items[k];
+                st[lane] = t;
+                sr[lane] = a.xyr[t];
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const double* p = a.xyz + 3 * (size_t)a.tris[3 * t + v];
+                    sv[3 * v][lane] = p[0];
+                    sv[3 * v + 1][lane] = p[1];
+                    sv[3 * v + 2][lane] = p[2];
+                }
+            }
+            __syncthreads();
+            if (!active) continue;
+            const int cnt = min(kChunk, end - c0);
+            for (int q = 0; q < cnt; ++q) {
+                const int4 r = sr[q];
+                if (ci < r.x || ci > r.y || cj < r.z || cj > r.w) continue;  // not in bucket
+                const int t = st[q];
+                if (pass == 7 && t >= cutoff) continue;
+                if (grazed && pass < 6) break;  // this attempt is already lost
+                const double A[3] = {sv[0][q], sv[1][q], sv[2][q]};
+                const double B[3] = {sv[3][q], sv[4][q], sv[5][q]};
+                const double C[3] = {sv[6][q], sv[7][q], sv[8][q]};
+                // voxelize.cpp:131-140
+                const int sa = o3_compat(P, A, B, C);
+                const int sb = o3_compat(Q, A, B, C);
+                if (sa == 0 || sb == 0) {
+                    grazed = true;
+                    first_graze = min(first_graze, t);
+                    continue;
+                }
+                if (sa == sb) continue;
+                const int s1 = o3_compat(P, Q, A, B);
+                const int s2 = o3_compat(P, Q, B, C);
+                const int s3 = o3_compat(P, Q, C, A);
+                if (s1 == 0 || s2 == 0 || s3 == 0) {
+                    grazed = true;
+                    first_graze = min(first_graze, t);
+                    continue;
+                }
+                if (s1 != s2 || s2 != s3) continue;
+                const int K = cells_below(g, plane_z(A, B, C, x, y));
+                mask[K >> 6] ^= 1ull << (K & 63);
+            }
+        }
+        if (active) {
+            if (pass == 0 && grazed) atomicAdd(&a.stats[0], 1u);
+            if (pass < 6) {
+                active = grazed;
+            } else if (pass == 6) {
+                if (grazed) {
+                    atomicAdd(&a.stats[1], 1u);
+                    cutoff = first_graze;  // replay counting only triangles below it
+                }
+                active = grazed;
+            } else {
+                active = false;
+            }
+        }
+    }
+    if (!valid) return;
+    // parity of cell k = XOR of the flip bits above k (suffix XOR from the top word)
+    unsigned long long carry = 0;
+    for (int w = a.words - 1; w >= 0; --w) {
+        unsigned long long x = mask[w];
+        unsigned long long s = x;
+        s ^= s >> 1;
+        s ^= s >> 2;
+        s ^= s >> 4;
+        s ^= s >> 8;
+        s ^= s >> 16;
+        s ^= s >> 32;  // bit i: XOR of bits i..63
+        unsigned long long above = (s >> 1) ^ (carry ? ~0ull : 0ull);
+        carry ^= s & 1ull;
+        const int k0 = w * 64;
+        const int k1 = min(g.nz, k0 + 64);
+        for (int k = k0; k < k1; ++k)
+            a.occ[((size_t)k * g.ny + cj) * g.nx + ci] = (uint8_t)((above >> (k - k0)) & 1ull);
+    }
+}
+
+}  // namespace
+
+// ---- host orchestration ---------------------------------------------------------------
+
+OccPlan occupancy_plan(const GridSpec& g) {
+    OccPlan p;
+    p.ntx = (g.nx + kTile - 1) / kTile;
+    p.nty = (g.ny + kTile - 1) / kTile;
+    p.words = (g.nz + 1 + 63) / 64;
+    return p;
+}
+
+void launch_occupancy(const double* d_xyz, const int* d_tris, int ntris, const GridSpec& g,
+                      uint8_t* d_occ, DeviceArena& arena, hipStream_t st, unsigned* h_stats) {
+    OccPlan p = occupancy_plan(g);
+    const int ntiles = p.ntx * p.nty;
+    int4* xyr = arena.alloc<int4>(ntris);
+    int* count = arena.alloc<int>(ntiles + 1);
+    int* off = arena.alloc<int>(ntiles + 1);
+    int* cursor = arena.alloc<int>(ntiles + 1);
+    unsigned* stats = arena.alloc<unsigned>(2);
+    unsigned long long* masks = arena.alloc<unsigned long long>((size_t)g.nx * g.ny * p.words);
+    CMG_HIP(hipMemsetAsync(count, 0, sizeof(int) * (ntiles + 1), st));
+    CMG_HIP(hipMemsetAsync(cursor, 0, sizeof(int) * (ntiles + 1), st));
+    CMG_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned) * 2, st));
+    const unsigned tb = 256, tg = (unsigned)((ntris + tb - 1) / tb);
+    tri_xy_range_kernel<<<tg, tb, 0, st>>>(d_xyz, d_tris, ntris, g, xyr, count, p.ntx);
+    CMG_HIP_LAUNCH();
+    exclusive_scan_kernel<<<1, 1024, 0, st>>>(count, off, ntiles);
+    CMG_HIP_LAUNCH();
+    int total = 0;
+    CMG_HIP(hipMemcpyAsync(&total, off + ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    int* items = arena.alloc<int>(total > 0 ? total : 1);
+    tile_fill_kernel<<<tg, tb, 0, st>>>(xyr, ntris, off, cursor, items, p.ntx);
+    CMG_HIP_LAUNCH();
+    OccArgs a{d_xyz, d_tris, xyr, off, items, p.ntx, p.nty, g, masks, p.words, d_occ, stats};
+    {
+        TimedLaunch timed("occupancy_columns", st);
+        occupancy_kernel<<<ntiles, 64, 0, st>>>(a);
+        CMG_HIP_LAUNCH();
+    }
+    if (h_stats) {
+        CMG_HIP(hipMemcpyAsync(h_stats, stats, sizeof(unsigned) * 2, hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+    }
+}
+
+void voxelize_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
+                   const GridSpec& g, bool sdf, uint8_t* occ, float* dist) {
+    const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+    hipStream_t st = nullptr;
+    DeviceArena arena(st);
+    double* d_xyz = arena.alloc<double>(3 * npts);
+    int* d_tris = arena.alloc<int>(3 * ntris);
+    uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
+    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
+    launch_occupancy(d_xyz, d_tris, (int)ntris, g, d_occ, arena, st, nullptr);
+    CMG_HIP(hipMemcpyAsync(occ, d_occ, ncell, hipMemcpyDeviceToHost, st));
+    if (sdf) {
+        float* d_dist = arena.alloc<float>(ncell);
+        launch_sdf_forward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_dist, arena, st);
+        CMG_HIP(hipMemcpyAsync(dist, d_dist, sizeof(float) * ncell, hipMemcpyDeviceToHost, st));
+    }
+    CMG_HIP(hipStreamSynchronize(st));
+}
+
+void sdf_backward_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
+                       const GridSpec& g, const uint8_t* occ, const float* grad_sdf,
+                       float* grad_xyz) {
+    const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+    hipStream_t st = nullptr;
+    DeviceArena arena(st);
+    double* d_xyz = arena.alloc<double>(3 * npts);
+    int* d_tris = arena.alloc<int>(3 * ntris);
+    uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
+    float* d_g = arena.alloc<float>(ncell);
+    float* d_grad = arena.alloc<float>(3 * npts);
+    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_occ, occ, ncell, hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_g, grad_sdf, sizeof(float) * ncell, hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemsetAsync(d_grad, 0, sizeof(float) * 3 * npts, st));
+    launch_sdf_backward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_g, d_grad, arena, st);
+    CMG_HIP(hipMemcpyAsync(grad_xyz, d_grad, sizeof(float) * 3 * npts, hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace cmg::hip

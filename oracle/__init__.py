@@ -63,7 +63,7 @@ def _load() -> ctypes.CDLL:
     lib.orc_sdf_bvh.argtypes = [P, P, sz, P, P, P, P, P, sz, sz, i32]
     lib.orc_sdf_grad.argtypes = [P, sz, P, sz, P, P, P, P, i32]
     lib.orc_sdf_fd.restype = d
-    lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, i32, i32, d, i32]
+    lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, P, P, i32, i32, d, i32]
     return lib
 
 
@@ -261,14 +261,22 @@ def sdf_grad(pts, tris, grid: Grid, occ, gout, nthreads=0) -> np.ndarray:
 
 
 def sdf_fd(pts, tris, grid: Grid, occ, gout, vert: int, axis: int, h=1e-5,
-           nthreads=0) -> float:
+           nthreads=0, base=None) -> float:
+    """Central difference of L = sum gout * sdf w.r.t. pts[vert, axis]; base =
+    (best, arg) from sdf_bvh lets many calls share the unperturbed search."""
     pts = _f64(pts).reshape(-1, 3)
     tris = _i32(tris).reshape(-1, 3)
     occ = np.ascontiguousarray(occ, dtype=np.uint8)
     gout = np.ascontiguousarray(gout, dtype=np.float32)
     g = grid._c()
+    bb = ba = None
+    if base is not None:
+        best = np.ascontiguousarray(base[0], dtype=np.float64)
+        arg = np.ascontiguousarray(base[1], dtype=np.int32)
+        bb, ba = _p(best), _p(arg)
     return lib().orc_sdf_fd(_p(pts), pts.shape[0], _p(tris), tris.shape[0],
-                            ctypes.byref(g), _p(occ), _p(gout), vert, axis, h, nthreads)
+                            ctypes.byref(g), _p(occ), _p(gout), bb, ba, vert, axis, h,
+                            nthreads)
 
 
 def voxelize(pts, polygons_or_tris, resolution=64, pad=1, mode="occupancy", nthreads=0):

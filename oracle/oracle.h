@@ -121,10 +121,13 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
                   double* grad, int nthreads);
 
 /* FP64 central finite difference of the same L with respect to coordinate `axis` of
- * vertex `vert`, step h, signs held at their unperturbed values. */
+ * vertex `vert`, step h, signs held at their unperturbed values. base_best/base_arg
+ * (optional, from orc_sdf_bvh) are the unperturbed squared distances and minimising
+ * triangles, so that many differences can share one search. */
 double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
-                  const orc_grid* g, const uint8_t* occ, const float* gout, int vert,
-                  int axis, double h, int nthreads);
+                  const orc_grid* g, const uint8_t* occ, const float* gout,
+                  const double* base_best, const int32_t* base_arg, int vert, int axis,
+                  double h, int nthreads);
 
 #ifdef __cplusplus
 }

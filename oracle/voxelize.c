@@ -648,8 +648,9 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
 }
 
 double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
-                  const orc_grid* g, const uint8_t* occ, const float* gout, int vert,
-                  int axis, double h, int nthreads) {
+                  const orc_grid* g, const uint8_t* occ, const float* gout,
+                  const double* base_best, const int32_t* base_arg, int vert, int axis,
+                  double h, int nthreads) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t ncell = (size_t)nx * ny * nz;
     bvh_t b = bvh_build(pts, tris, ntris);
@@ -686,7 +687,13 @@ double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
         double p[3];
         cell_center(g, i, j, k, p);
         int t0 = -1;
-        double base = bvh_nearest(&b, pts, tris, p, NULL, &t0);
+        double base;
+        if (base_best) {
+            base = base_best[idx];
+            t0 = base_arg[idx];
+        } else {
+            base = bvh_nearest(&b, pts, tris, p, NULL, &t0);
+        }
         if (ninc == 0) continue;
         if (box_d2(&box, p) > base * (1.0 + 1e-6) && !(t0 >= 0 && inc[t0])) continue;
         double others = bvh_nearest(&b, pts, tris, p, inc, NULL);

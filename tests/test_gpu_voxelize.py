@@ -1,0 +1,165 @@
+"""Parity of the HIP voxelization with the CPU oracle: occupancy bit-exact, SDF signs
+bit-exact, SDF magnitudes bit-exact within one cell of the surface (FP64 re-evaluation)
+and within a stated FP32 tolerance elsewhere, SDF adjoint against the FP64 gradient."""
+import numpy as np
+import pytest
+
+import oracle
+from cybermeshgenerator_amd import workloads as W
+from tests.meshes import brick, subdivided_cube, uv_sphere
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_occ(cm, pts, tris, res, pad):
+    plc = cm.plc_from_mesh(pts, tris)
+    return cm.voxelize(plc, resolution=res, mode="occupancy", pad=pad, flat=True).grid
+
+
+@pytest.mark.parametrize("snap", [None, 2.0 ** -6, 2.0 ** -7])
+def test_occupancy_icosphere_c1(gpu, snap):
+    case = W.icosphere_c1(seed=1, snap=snap)
+    got = gpu_occ(gpu, case.points, case.tris, 64, 0)
+    g = oracle.make_grid(case.points, 64, 0)
+    want, stats = oracle.occupancy(case.points, case.tris, g, with_stats=True)
+    assert np.array_equal(got, want), int((got != want).sum())
+    if snap:
+        assert stats[0] > 0   # the nudge path really ran
+
+
+def test_occupancy_cube_known_answer(gpu):
+    pts, tris = subdivided_cube(4)
+    got = gpu_occ(gpu, pts, tris, 16, 1).reshape(18, 18, 18)
+    want = np.zeros((18, 18, 18), np.uint8)
+    want[1:17, 1:17, 1:17] = 1
+    assert np.array_equal(got, want)
+
+
+def test_occupancy_noncubic_axes(gpu):
+    for X, Y, Z in [(2.0, 1.0, 1.0), (1.0, 2.0, 1.0), (1.0, 1.0, 2.0), (3.0, 1.0, 2.0)]:
+        pts, tris = brick(X, Y, Z)
+        got = gpu_occ(gpu, pts, tris, 20, 1)
+        g = oracle.make_grid(pts, 20, 1)
+        assert np.array_equal(got, oracle.occupancy(pts, tris, g))
+
+
+def test_occupancy_every_nudge_grazes(gpu):
+    """A zero-area (collinear) triangle grazes every attempt of the columns it covers, so
+    the reference keeps the crossings of the triangles listed before it
+    (voxelize.cpp:163-164). The replay pass must reproduce that exactly."""
+    pts, tris = subdivided_cube(4)
+    extra = np.array([[0.3, 0.3, 0.5], [0.5, 0.5, 0.5], [0.7, 0.7, 0.5]])
+    pts = np.concatenate([pts, extra])
+    k = len(pts) - 3
+    mid = tris.shape[0] // 2
+    tris = np.concatenate([tris[:mid], [[k, k + 1, k + 2]], tris[mid:]]).astype(np.int32)
+    g = oracle.make_grid(pts, 16, 1)
+    want, stats = oracle.occupancy(pts, tris, g, with_stats=True)
+    assert stats[1] > 0      # some columns exhausted every nudge
+    got = gpu_occ(gpu, pts, tris, 16, 1)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("which", ["torus", "harmonic", "sphere"])
+def test_occupancy_larger_meshes(gpu, which):
+    if which == "torus":
+        case = W.torus_c3(seed=3)          # 65,536 triangles, 256^3
+    elif which == "harmonic":
+        case = W.harmonic_c4(seed=4)       # 20,480 triangles, 128^3
+    else:
+        sp, st = uv_sphere(0.4, 40, 60)
+        case = W.MeshCase("uv", W.anchored(sp + 0.5), st, 96, 0)
+    got = gpu_occ(gpu, case.points, case.tris, case.resolution, case.pad)
+    g = oracle.make_grid(case.points, case.resolution, case.pad)
+    want = oracle.occupancy(case.points, case.tris, g)
+    assert np.array_equal(got, want), int((got != want).sum())
+
+
+# ---- signed distance ---------------------------------------------------------------
+
+def check_sdf(cm, pts, tris, res, pad, tol=None):
+    """GPU SDF vs the FP64 oracle: signs bit-exact everywhere, values bit-exact within
+    one cell of the surface, FP32 error <= tol elsewhere. Returns the far-field error."""
+    plc = cm.plc_from_mesh(pts, tris)
+    got = cm.voxelize(plc, resolution=res, mode="sdf", pad=pad, flat=True).grid
+    g = oracle.make_grid(pts, res, pad)
+    occ = oracle.occupancy(pts, tris, g)
+    want, best, _ = oracle.sdf_bvh(pts, tris, g, occ)
+    assert np.array_equal(np.signbit(got), np.signbit(want))
+    near = np.sqrt(best) <= g.spacing
+    assert np.array_equal(got[near].view(np.uint32), want[near].view(np.uint32)), \
+        int((got[near] != want[near]).sum())
+    err = np.abs(got.astype(np.float64) - want.astype(np.float64))
+    extent = max(g.dims) * g.spacing
+    if tol is None:
+        tol = 4e-6 * extent          # FP32 distance from the corner: a few ulps of extent
+    assert err.max() <= tol, (err.max(), tol)
+    return {"max": float(err.max()), "rms": float(np.sqrt((err ** 2).mean())),
+            "near_cells": int(near.sum())}
+
+
+def test_sdf_uv_sphere_reference_case(gpu):
+    """The brute-force case of tests/test_voxelize.cpp:194-275 (16x16 UV sphere, R=20,
+    pad=1), bit-exact near the surface."""
+    sp, st = uv_sphere(1.0, 16, 16)
+    check_sdf(gpu, sp, st, 20, 1)
+
+
+def test_sdf_cube_signs_and_centre(gpu):
+    pts, tris = subdivided_cube(4)
+    plc = gpu.plc_from_mesh(pts, tris)
+    d = gpu.voxelize(plc, resolution=16, mode="sdf", pad=2).grid
+    assert d[10, 10, 10] < 0 and 0.3 < abs(d[10, 10, 10]) < 0.6
+    assert d[0, 0, 0] > 0
+    check_sdf(gpu, pts, tris, 16, 2)
+
+
+@pytest.mark.parametrize("snap", [None, 2.0 ** -7])
+def test_sdf_icosphere_c1(gpu, snap):
+    case = W.icosphere_c1(seed=1, snap=snap)
+    check_sdf(gpu, case.points, case.tris, 64, 0)
+
+
+def test_sdf_harmonic_c4(gpu):
+    case = W.harmonic_c4(seed=4)
+    r = check_sdf(gpu, case.points, case.tris, 128, 0)
+    print("C4 FP32 SDF error vs FP64:", r)
+
+
+def test_sdf_torus_c3_full(gpu):
+    """C3 at full size: 65,536 triangles, 256^3 cells, every cell compared."""
+    case = W.torus_c3(seed=3)
+    r = check_sdf(gpu, case.points, case.tris, 256, 0)
+    print("C3 FP32 SDF error vs FP64:", r)
+
+
+def test_sdf_gradient_c4(gpu):
+    """SDF adjoint at C4 size against the FP64 CPU gradient and FP64 central differences
+    (h = 1e-5) on 256 random vertices."""
+    case = W.harmonic_c4(seed=4)
+    pts, tris = case.points, case.tris
+    g = oracle.make_grid(pts, 128, 0)
+    gout = W.upstream_grad(g.ncell, seed=40)
+    plc = gpu.plc_from_mesh(pts, tris)
+    sdf, grad = gpu.sdf_with_grad(plc, 128, 0, gout)
+    occ = oracle.occupancy(pts, tris, g)
+    ref = oracle.sdf_grad(pts, tris, g, occ, gout)
+    scale = np.abs(ref).max()
+    err = np.abs(grad - ref)
+    rel = np.linalg.norm(grad - ref) / np.linalg.norm(ref)
+    print(f"C4 gradient: max|err|/max|g| = {err.max() / scale:.3e}, rel L2 = {rel:.3e}")
+    assert np.abs(grad[-2:]).max() == 0            # anchors are not referenced
+    assert rel < 1e-3 and err.max() < 1e-2 * scale
+    _, best, arg = oracle.sdf_bvh(pts, tris, g, occ)
+    rng = np.random.default_rng(41)
+    nv = pts.shape[0] - 2
+    fd_err = []
+    for v in rng.choice(nv, 256, replace=False):
+        for axis in range(3):
+            fd = oracle.sdf_fd(pts, tris, g, occ, gout, int(v), axis, h=1e-5,
+                               base=(best, arg))
+            fd_err.append(abs(fd - grad[v, axis]) / (1e-3 * scale + abs(fd)))
+            assert abs(fd - ref[v, axis]) <= 1e-4 * scale + 1e-3 * abs(fd), (v, axis)
+    fd_err = np.array(fd_err)
+    print(f"C4 gradient vs FD: max rel {fd_err.max():.3e}, median {np.median(fd_err):.3e}")
+    assert np.median(fd_err) < 1e-3 and fd_err.max() < 5e-2
