@@ -124,6 +124,93 @@ def cpu_baseline_predicates(soa_np, gpu_signs):
             "seconds": dt}
 
 
+def bench_sdf(cm, W, steps, warmup):
+    """C5: 64 uneven meshes, SDF forward+backward per step, batched (one graph of
+    persistent kernels) vs plain (one mesh at a time, synchronised per mesh, same kernels).
+    Also C3/C4 single-mesh forward throughput."""
+    import torch
+    from cybermeshgenerator_amd.batch import SdfBatch
+    log("[bench] generating C5 meshes")
+    cases = W.batch_c5(seed=5)
+    meshes = [(c.points, c.tris) for c in cases]
+    res = [c.resolution for c in cases]
+    pads = [c.pad for c in cases]
+    t0 = time.perf_counter()
+    batch = SdfBatch(meshes, res, pads)
+    plan_s = time.perf_counter() - t0
+    cells = batch.cells
+    tris = int(sum(c.tris.shape[0] for c in cases))
+    gs = torch.from_numpy(W.upstream_grad(cells, seed=50)).cuda()
+    sdf = torch.empty(cells, dtype=torch.float32, device="cuda")
+    grad = torch.empty((batch.points.shape[0], 3), dtype=torch.float32, device="cuda")
+
+    def bstep():
+        batch.step(gs, sdf=sdf, grad=grad, graph=True)
+
+    reps = max(1, steps // 4)
+    ms_b = time_steps(bstep, reps, max(1, warmup), torch.cuda.synchronize)
+    st = batch.status()
+    pairs_per_step = st["pairs"]
+    # dominant kernel: the forward voxel pass
+    cm.kernel_timer("batch_sdf", True)
+    cm.kernel_timer("batch_sdf_backward", True)
+    batch.step(gs, sdf=sdf, grad=grad, graph=False)
+    torch.cuda.synchronize()
+    fwd_ms, _ = cm.kernel_time("batch_sdf")
+    bwd_ms, _ = cm.kernel_time("batch_sdf_backward")
+    cm.kernel_timer("batch_sdf", False)
+    cm.kernel_timer("batch_sdf_backward", False)
+    # plain: one engine per mesh, one mesh at a time
+    log("[bench] planning 64 single-mesh engines")
+    singles = [SdfBatch([m], [r], [p]) for m, r, p in zip(meshes, res, pads)]
+    sg = [gs[bg.offset:bg.offset + bg.ncell].contiguous() for bg in batch.grids]
+    souts = [(torch.empty(b.cells, dtype=torch.float32, device="cuda"),
+              torch.empty((b.points.shape[0], 3), dtype=torch.float32, device="cuda"))
+             for b in singles]
+
+    def pstep():
+        for b, g, (o, gr) in zip(singles, sg, souts):
+            b.step(g, sdf=o, grad=gr, graph=False)
+            torch.cuda.synchronize()
+
+    ms_p = time_steps(pstep, max(1, reps // 2), 1, torch.cuda.synchronize)
+    # the two paths agree
+    plain = torch.cat([o for o, _ in souts])
+    same = bool(torch.equal(plain, sdf))
+    for b in singles:
+        b.close()
+    out = {"meshes": len(cases), "cells": cells, "triangles": tris,
+           "batched_ms": ms_b, "plain_ms": ms_p,
+           "batched_voxels_per_s": cells / (ms_b / 1e3),
+           "plain_voxels_per_s": cells / (ms_p / 1e3),
+           "speedup_batched_vs_plain": ms_p / ms_b,
+           "batched_equals_plain": same, "plan_s": plan_s,
+           "fp32_pairs_per_step": pairs_per_step,
+           "pairs_per_voxel": pairs_per_step / (2 * cells),
+           "fwd_voxel_kernel_ms": fwd_ms, "bwd_voxel_kernel_ms": bwd_ms,
+           "overflow": st["overflow"]}
+    # fp32 flop rate of the voxel kernels: 48 flops per evaluated pair, +20 per voxel in bwd
+    kern_s = (fwd_ms + bwd_ms) / 1e3
+    if kern_s > 0:
+        out["voxel_kernels_fp32_tflops"] = (pairs_per_step * 48 + cells * 20) / kern_s / 1e12
+    batch.close()
+    # single-mesh C3 and C4 forward
+    for name, case in (("c3_torus_256", W.torus_c3(seed=3)), ("c4_harmonic_128", W.harmonic_c4(seed=4))):
+        b = SdfBatch([(case.points, case.tris)], [case.resolution], [case.pad])
+        o = torch.empty(b.cells, dtype=torch.float32, device="cuda")
+        ms = time_steps(lambda: b.forward(out=o, graph=True), max(1, reps), 1,
+                        torch.cuda.synchronize)
+        g1 = torch.from_numpy(W.upstream_grad(b.cells, seed=40)).cuda()
+        gr1 = torch.empty((b.points.shape[0], 3), dtype=torch.float32, device="cuda")
+        ms2 = time_steps(lambda: b.step(g1, sdf=o, grad=gr1, graph=True), max(1, reps), 1,
+                         torch.cuda.synchronize)
+        out[name] = {"cells": b.cells, "fwd_ms": ms, "fwd_voxels_per_s": b.cells / (ms / 1e3),
+                     "fwd_bwd_ms": ms2, "fwd_bwd_voxels_per_s": b.cells / (ms2 / 1e3),
+                     "status": b.status()}
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +250,13 @@ def main():
         "filter_hbm_gbs": INSPHERE_BYTES * n5 / (ins["filter_ms"] / 1e3) / 1e9,
         "stage_fraction": {"fp32": s5["fp32"] / n5, "fp64": s5["fp64"] / n5,
                            "exact": s5["exact"] / n5}}
+
+    if not args.no_sdf:
+        try:
+            extra["sdf"] = bench_sdf(cm, W, args.steps, args.warmup)
+        except Exception as e:  # reported, never hides the predicate line
+            log(f"[bench] sdf workloads failed: {e!r}")
+            extra["sdf"] = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu:

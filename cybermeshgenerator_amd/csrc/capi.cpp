@@ -9,10 +9,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/cmg_hip.h"
+#include "batch_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
 
@@ -20,6 +22,10 @@ struct cmg_plc {
     std::vector<double> xyz;            // 3 per point
     std::vector<std::vector<int>> polys;  // one polygon per facet (cmg_c.cpp:88-100)
     std::vector<int> markers;
+};
+
+struct cmg_batch {
+    std::unique_ptr<cmg::hip::BatchEngine> engine;
 };
 
 struct cmg_voxels {
@@ -374,6 +380,73 @@ cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
         out->wide = c.wide_n;
         out->failed = c.failed;
     });
+}
+
+// ---- 4. batched voxelization -----------------------------------------------------------
+
+cmg_status cmg_voxelize_batch_create(int nmesh, const int64_t* point_off,
+                                     const int64_t* tri_off, const int* resolution,
+                                     const int* pad, const double* xyz, const int* tris,
+                                     void* stream, cmg_batch** out, char* errbuf,
+                                     size_t len) {
+    if (out) *out = nullptr;
+    if (!out || nmesh < 1 || !point_off || !tri_off || !resolution || !pad || !xyz || !tris) {
+        set_err(errbuf, len, "batch: invalid arguments");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        std::vector<long long> ps(point_off, point_off + nmesh + 1),
+            ts(tri_off, tri_off + nmesh + 1);
+        auto b = std::make_unique<cmg_batch>();
+        b->engine = std::make_unique<cmg::hip::BatchEngine>(nmesh, ps.data(), ts.data(),
+                                                            resolution, pad, xyz, tris,
+                                                            as_stream(stream));
+        *out = b.release();
+    });
+}
+
+void cmg_voxelize_batch_destroy(cmg_batch* b) { delete b; }
+
+int64_t cmg_voxelize_batch_cells(const cmg_batch* b) {
+    return b ? b->engine->total_cells() : 0;
+}
+
+cmg_status cmg_voxelize_batch_grid(const cmg_batch* b, int mesh, int* dims, double* origin,
+                                   double* spacing, int64_t* cell_offset) {
+    if (!b || mesh < 0 || mesh >= b->engine->meshes()) return CMG_ERR_INVALID_INPUT;
+    const cmg::hip::MeshDesc& M = b->engine->mesh(mesh);
+    if (dims) dims[0] = M.g.nx, dims[1] = M.g.ny, dims[2] = M.g.nz;
+    if (origin)
+        for (int a = 0; a < 3; ++a) origin[a] = M.g.corner[a] + 0.5 * M.g.spacing;
+    if (spacing) *spacing = M.g.spacing;
+    if (cell_offset) *cell_offset = M.cell_base;
+    return CMG_OK;
+}
+
+cmg_status cmg_voxelize_batch_run(cmg_batch* b, int flags, const double* xyz, float* sdf,
+                                  const float* grad_sdf, float* grad_xyz, void* stream,
+                                  char* errbuf, size_t len) {
+    if (!b) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        g_last_backend = "hip";
+        b->engine->run(flags, xyz, sdf, grad_sdf, grad_xyz, as_stream(stream));
+    });
+}
+
+cmg_status cmg_voxelize_batch_status(cmg_batch* b, void* stream, unsigned* overflow,
+                                     uint64_t* pairs, char* errbuf, size_t len) {
+    if (!b) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        unsigned long long pr = 0;
+        b->engine->status(as_stream(stream), overflow, &pr);
+        if (pairs) *pairs = pr;
+    });
+}
+
+const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch* b) {
+    return b ? b->engine->occupancy() : nullptr;
 }
 
 void cmg_hip_kernel_timer(const char* name, int enable) {

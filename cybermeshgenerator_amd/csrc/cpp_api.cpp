@@ -1,0 +1,200 @@
+// cpp_api.cpp — the C++20 API (include/cmg/cmg_hip.hpp) over the C ABI of this library.
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cmg/cmg_hip.hpp"
+#include "../../include/cmg_hip.h"
+
+namespace cmg {
+
+namespace {
+
+thread_local backend::Backend g_last = backend::Backend::Cpu;
+
+// PLC -> C handle (points + one polygon per facet polygon)
+struct PlcHandle {
+    cmg_plc* h;
+    explicit PlcHandle(const PLC& plc) : h(cmg_plc_create()) {
+        std::vector<double> xyz;
+        xyz.reserve(plc.points.size() * 3);
+        for (const Point3& p : plc.points) {
+            xyz.push_back(p.x);
+            xyz.push_back(p.y);
+            xyz.push_back(p.z);
+        }
+        cmg_plc_set_points(h, xyz.data(), plc.points.size());
+        for (const Facet& f : plc.facets)
+            for (const Polygon& poly : f.polygons)
+                if (poly.vertices.size() >= 3)
+                    cmg_plc_add_facet(h, poly.vertices.data(), poly.vertices.size(), f.marker);
+    }
+    ~PlcHandle() { cmg_plc_destroy(h); }
+};
+
+MeshErrorCode code_of(cmg_status s) {
+    switch (s) {
+        case CMG_ERR_INVALID_INPUT: return MeshErrorCode::InvalidInput;
+        case CMG_ERR_SELF_INTERSECTION: return MeshErrorCode::SelfIntersection;
+        case CMG_ERR_UNSUPPORTED_OPTION: return MeshErrorCode::UnsupportedOption;
+        case CMG_ERR_QUALITY_NOT_MET: return MeshErrorCode::QualityNotMet;
+        case CMG_ERR_NOT_IMPLEMENTED: return MeshErrorCode::NotImplemented;
+        default: return MeshErrorCode::Internal;
+    }
+}
+
+void flatten(std::span<const Point3> pts, std::vector<double>& xyz) {
+    xyz.resize(pts.size() * 3);
+    for (std::size_t i = 0; i < pts.size(); ++i) {
+        xyz[3 * i] = pts[i].x;
+        xyz[3 * i + 1] = pts[i].y;
+        xyz[3 * i + 2] = pts[i].z;
+    }
+}
+
+}  // namespace
+
+namespace voxelize {
+
+expected<VoxelGrid, MeshError> voxelize(const PLC& plc, const VoxelOptions& opts) {
+    PlcHandle h(plc);
+    cmg_voxels* v = nullptr;
+    char err[512] = {0};
+    cmg_status st = cmg_plc_voxelize(h.h, opts.resolution,
+                                     opts.mode == VoxelMode::SignedDistance ? 1 : 0, opts.pad,
+                                     &v, err, sizeof err);
+    if (st == CMG_ERR_INVALID_INPUT)
+        return unexpected(MeshError{MeshErrorCode::InvalidInput, err});
+    if (st != CMG_OK) throw std::runtime_error(err);
+    g_last = backend::Backend::Hip;
+    VoxelGrid g;
+    cmg_voxels_dims(v, &g.nx, &g.ny, &g.nz);
+    cmg_voxels_origin(v, &g.origin.x, &g.origin.y, &g.origin.z);
+    g.spacing = cmg_voxels_spacing(v);
+    const std::size_t n = g.cell_count();
+    if (const unsigned char* o = cmg_voxels_occupancy(v)) g.occupancy.assign(o, o + n);
+    if (const float* d = cmg_voxels_distance(v)) g.distance.assign(d, d + n);
+    cmg_voxels_destroy(v);
+    return g;
+}
+
+expected<std::vector<Point3>, MeshError> sdf_backward(const PLC& plc, const VoxelOptions& opts,
+                                                      std::span<const float> grad_sdf) {
+    PlcHandle h(plc);
+    cmg_voxels* v = nullptr;
+    char err[512] = {0};
+    cmg_status st = cmg_plc_voxelize(h.h, opts.resolution, 1, opts.pad, &v, err, sizeof err);
+    if (st == CMG_ERR_INVALID_INPUT)
+        return unexpected(MeshError{MeshErrorCode::InvalidInput, err});
+    if (st != CMG_OK) throw std::runtime_error(err);
+    int nx, ny, nz;
+    cmg_voxels_dims(v, &nx, &ny, &nz);
+    if (grad_sdf.size() != static_cast<std::size_t>(nx) * ny * nz) {
+        cmg_voxels_destroy(v);
+        return unexpected(MeshError{MeshErrorCode::InvalidInput,
+                                    "sdf_backward: one gradient value per cell expected"});
+    }
+    std::vector<float> g(plc.points.size() * 3);
+    st = cmg_voxels_sdf_backward(h.h, v, grad_sdf.data(), g.data(), err, sizeof err);
+    cmg_voxels_destroy(v);
+    if (st != CMG_OK) {
+        if (st == CMG_ERR_INVALID_INPUT) return unexpected(MeshError{code_of(st), err});
+        throw std::runtime_error(err);
+    }
+    g_last = backend::Backend::Hip;
+    std::vector<Point3> out(plc.points.size());
+    for (std::size_t i = 0; i < out.size(); ++i) out[i] = {g[3 * i], g[3 * i + 1], g[3 * i + 2]};
+    return out;
+}
+
+}  // namespace voxelize
+
+namespace robust {
+
+void ensure_initialized() {}
+
+double orient3d(const Point3& a, const Point3& b, const Point3& c, const Point3& d) {
+    const Point3 p[4] = {a, b, c, d};
+    const Tetrahedron t{0, 1, 2, 3};
+    return orient3d_batch(p, std::span<const Tetrahedron>(&t, 1))[0];
+}
+
+double insphere(const Point3& a, const Point3& b, const Point3& c, const Point3& d,
+                const Point3& e) {
+    const double xyz[15] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z,
+                            d.x, d.y, d.z, e.x, e.y, e.z};
+    const int q[5] = {0, 1, 2, 3, 4};
+    signed char s = 0;
+    char err[512] = {0};
+    if (cmg_insphere_signs(xyz, 5, q, 1, CMG_PRED_COMPAT, &s, err, sizeof err) != CMG_OK)
+        throw std::runtime_error(err);
+    g_last = backend::Backend::Hip;
+    return static_cast<double>(s);  // the sign; magnitudes are not part of this path
+}
+
+std::vector<double> orient3d_batch(std::span<const Point3> pts,
+                                   std::span<const Tetrahedron> candidates) {
+    std::vector<double> xyz;
+    flatten(pts, xyz);
+    std::vector<double> out(candidates.size());
+    char err[512] = {0};
+    if (cmg_orient3d_values(xyz.data(), pts.size(),
+                            reinterpret_cast<const int*>(candidates.data()), candidates.size(),
+                            out.data(), err, sizeof err) != CMG_OK)
+        throw std::runtime_error(err);
+    g_last = backend::Backend::Hip;
+    return out;
+}
+
+std::vector<int> orient3d_signs(std::span<const Point3> pts,
+                                std::span<const Tetrahedron> candidates) {
+    std::vector<double> xyz;
+    flatten(pts, xyz);
+    std::vector<signed char> s(candidates.size());
+    char err[512] = {0};
+    if (cmg_orient3d_signs(xyz.data(), pts.size(),
+                           reinterpret_cast<const int*>(candidates.data()), candidates.size(),
+                           CMG_PRED_COMPAT, s.data(), err, sizeof err) != CMG_OK)
+        throw std::runtime_error(err);
+    g_last = backend::Backend::Hip;
+    return std::vector<int>(s.begin(), s.end());
+}
+
+}  // namespace robust
+
+namespace backend {
+
+std::size_t threshold_for(Op op) noexcept {
+    switch (op) {  // dispatch.cpp:23-30
+        case Op::OrientFilter: return 1u << 16;
+        case Op::HilbertSort: return 1u << 15;
+        case Op::PointLocation: return 1u << 16;
+        case Op::QualityScan: return 1u << 17;
+        case Op::Voxelize: return 1;
+    }
+    return static_cast<std::size_t>(-1);
+}
+
+bool should_offload(Op op, std::size_t problem_size) noexcept {
+    (void)op;
+    (void)problem_size;
+    return cmg_hip_available() != 0;
+}
+
+Backend last_backend() noexcept { return g_last; }
+
+std::string_view name(Backend b) noexcept {
+    switch (b) {
+        case Backend::Cpu: return "cpu";
+        case Backend::Cuda: return "cuda";
+        case Backend::OpenCL: return "opencl";
+        case Backend::Metal: return "metal";
+        case Backend::Hip: return "hip";
+    }
+    return "cpu";
+}
+
+}  // namespace backend
+
+}  // namespace cmg

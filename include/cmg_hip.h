@@ -156,6 +156,52 @@ cmg_status cmg_hip_insphere_signs_soa(const double* soa, size_t n, size_t ld, in
 cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
                               cmg_pred_stats* out, char* errbuf, size_t errbuf_len);
 
+/* ---- 4. batched voxelization of many meshes (no counterpart in the reference: the
+ * proposed cmg_voxelize_batch addition) ------------------------------------------------ */
+
+typedef struct cmg_batch cmg_batch;
+
+enum { CMG_BATCH_FORWARD = 1, CMG_BATCH_BACKWARD = 2, CMG_BATCH_GRAPH = 4 };
+
+/* Plans a batch of nmesh meshes stored back to back in device memory: mesh m owns
+ * points [point_off[m], point_off[m+1]) of xyz (3 doubles each) and triangles
+ * [tri_off[m], tri_off[m+1]) of tris (3 ints each, indices local to the mesh's points).
+ * Grid m is make_grid(points of m, resolution[m], pad[m]) exactly as cmg_plc_voxelize
+ * builds it (voxelize.cpp:52-74). point_off/tri_off/resolution/pad are host arrays.
+ * Planning synchronises `stream` (it sizes the candidate-list pools); xyz and tris must
+ * stay valid and at the same address for the batch's lifetime. */
+cmg_status cmg_voxelize_batch_create(int nmesh, const int64_t* point_off,
+                                     const int64_t* tri_off, const int* resolution,
+                                     const int* pad, const double* xyz, const int* tris,
+                                     void* stream, cmg_batch** out, char* errbuf,
+                                     size_t errbuf_len);
+void cmg_voxelize_batch_destroy(cmg_batch*);
+
+/* Cells over all grids (the length of the concatenated SDF), and mesh m's grid: dims,
+ * origin (centre of cell (0,0,0)), spacing and the offset of its first cell. */
+int64_t cmg_voxelize_batch_cells(const cmg_batch*);
+cmg_status cmg_voxelize_batch_grid(const cmg_batch*, int mesh, int* dims, double* origin,
+                                   double* spacing, int64_t* cell_offset);
+
+/* One step on `stream`, no host synchronisation. CMG_BATCH_FORWARD writes every mesh's
+ * signed distance to sdf (device, cells floats, each grid flat idx = (k*ny + j)*nx + i at
+ * its cell offset). CMG_BATCH_BACKWARD (after a forward on the same points) writes
+ * dL/dxyz to grad_xyz (device, 3 floats per point) for the upstream grad_sdf (device,
+ * cells floats), signs held constant. CMG_BATCH_GRAPH captures the step into a hipGraph
+ * on first use for this buffer set and replays it afterwards. */
+cmg_status cmg_voxelize_batch_run(cmg_batch*, int flags, const double* xyz, float* sdf,
+                                  const float* grad_sdf, float* grad_xyz, void* stream,
+                                  char* errbuf, size_t errbuf_len);
+
+/* Synchronises `stream`; overflow != 0 means a candidate-list pool was too small for the
+ * last step (results incomplete: create the batch again), pairs counts the FP32
+ * point-triangle evaluations of the last forward and backward passes. */
+cmg_status cmg_voxelize_batch_status(cmg_batch*, void* stream, unsigned* overflow,
+                                     uint64_t* pairs, char* errbuf, size_t errbuf_len);
+
+/* Device pointer to the occupancy bytes of every grid (valid after a forward). */
+const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch*);
+
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
  * "sdf_voxels", "sdf_backward", "occupancy_columns", "batch_sdf"). Events are recorded on the stream
