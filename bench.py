@@ -218,6 +218,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="also run one device-to-device copy of 2^30 bytes (a kernel of "
+                         "known traffic, to scale FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -229,6 +232,15 @@ def main():
     from cybermeshgenerator_amd import workloads as W
     assert cm.hip_available(), "no HIP device"
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+
+    if args.calibration_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
     n = pr["n"]

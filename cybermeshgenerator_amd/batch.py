@@ -40,6 +40,11 @@ def _bind():
     L.cmg_voxelize_batch_status.argtypes = [_P, _P, ctypes.POINTER(ctypes.c_uint),
                                             ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.c_char_p, _sz]
+    L.cmg_voxelize_batch_autotune.restype = _i
+    L.cmg_voxelize_batch_autotune.argtypes = [_P, _P, _P, _P, _P, ctypes.POINTER(_i),
+                                              ctypes.POINTER(_i), _P, ctypes.c_char_p, _sz]
+    L.cmg_voxelize_batch_set_variant.restype = _i
+    L.cmg_voxelize_batch_set_variant.argtypes = [_P, _i, _i]
     L.cmg_voxelize_batch_occupancy.restype = _P
     L.cmg_voxelize_batch_occupancy.argtypes = [_P]
     _bound = True
@@ -161,6 +166,29 @@ class SdfBatch:
         self.run(FORWARD | BACKWARD | (GRAPH if graph else 0), sdf=sdf,
                  grad_sdf=grad_sdf.contiguous(), grad=grad, stream=stream)
         return sdf, grad
+
+    def autotune(self, grad_sdf, sdf=None, grad=None, stream=None) -> dict:
+        """Times the six voxel-pass variants on this batch and keeps the fastest; returns
+        {"vpl": .., "depth": .., "ms": {(vpl, depth): ms}}."""
+        import torch
+        if sdf is None:
+            sdf = torch.empty(self.cells, dtype=torch.float32, device=self.device)
+        if grad is None:
+            grad = torch.empty((self.points.shape[0], 3), dtype=torch.float32,
+                               device=self.device)
+        vpl, depth = _i(), _i()
+        ms = np.zeros(6, dtype=np.float32)
+        err = _err()
+        _check(lib().cmg_voxelize_batch_autotune(
+            self._h, sdf.data_ptr(), grad_sdf.contiguous().data_ptr(), grad.data_ptr(),
+            _stream(stream), ctypes.byref(vpl), ctypes.byref(depth), ms.ctypes.data_as(_P),
+            err, len(err)), err)
+        keys = [(1, 0), (1, 32), (1, 64), (2, 0), (2, 32), (2, 64)]
+        return {"vpl": vpl.value, "depth": depth.value,
+                "ms": {f"{v}x{d}": float(t) for (v, d), t in zip(keys, ms)}}
+
+    def set_variant(self, vpl: int, depth: int):
+        _check(lib().cmg_voxelize_batch_set_variant(self._h, vpl, depth), None)
 
     def status(self, stream=None) -> dict:
         ov = ctypes.c_uint()

@@ -61,9 +61,27 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_native_tests()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def _build_native_tests() -> None:
+    """The C-ABI smoke (C) and the C++ API test, linked against libcmg_hip.so."""
+    tdir = os.path.join(ROOT, "tests", "native")
+    if not os.path.isdir(tdir):
+        return
+    inc = ["-I" + os.path.join(ROOT, "include")]
+    link = ["-L" + HERE, "-lcmg_hip", "-Wl,-rpath,$ORIGIN/../../cybermeshgenerator_amd"]
+    jobs = [["gcc", "-std=c11", "-O1", *inc, os.path.join(tdir, "smoke_c_abi.c"), "-o",
+             os.path.join(tdir, "smoke_c_abi"), *link],
+            ["g++", "-std=c++20", "-O1", *inc, os.path.join(tdir, "test_cpp_api.cpp"), "-o",
+             os.path.join(tdir, "test_cpp_api"), *link]]
+    for cmd in jobs:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"native test build failed:\n{' '.join(cmd)}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -139,6 +139,13 @@ __global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n,
         g[i] = 0.0f;
 }
 
+__global__ void reset_queue_kernel(BatchCounters* ctr, int a, int b) {
+    if (threadIdx.x == 0) {
+        ctr->queue[a] = 0;
+        ctr->queue[b] = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.total_tris;
@@ -394,11 +401,15 @@ __global__ __launch_bounds__(256) void cull_kernel(Ctx C) {
     }
 }
 
-// One 4^3 brick per item; same algorithm as sdf.hip's voxel kernel.
-template <bool BWD>
+// One brick per item: 4x4x4 voxels (VPL = 1 voxel per lane) or 4x4x8 (VPL = 2, the
+// second voxel of a lane 4 cells above the first). D = 0 reads each candidate's record
+// with scalar loads (SGPR operands); D > 0 stages D records at a time through LDS.
+// Same algorithm as sdf.hip's voxel kernel otherwise.
+template <bool BWD, int VPL, int D>
 __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
     __shared__ int slist[kLdsList];
-    const long long* off = phase_off(C, kPhL0);
+    __shared__ TriRec srec[D > 0 ? D : 1];
+    const long long* off = phase_off(C, VPL == 1 ? kPhL0 : kPhL0b);
     const long long total = off[C.nm];
     const int lane = threadIdx.x;
     unsigned long long* q = &C.ctr->queue[BWD ? 8 : 7];
@@ -409,16 +420,25 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         const int pos = find_pos(off, C.nm, item);
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec& g = M.g;
+        const Level L0 = VPL == 1 ? M.L0 : M.L0b;
         const int b = (int)(item - off[pos]);
-        const int ix = b % M.L0.bx, iy = (b / M.L0.bx) % M.L0.by, iz = b / (M.L0.bx * M.L0.by);
-        const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3),
-                  k = iz * kLeaf + (lane >> 4);
-        const bool valid = i < g.nx && j < g.ny && k < g.nz;
+        const int ix = b % L0.bx, iy = (b / L0.bx) % L0.by, iz = b / (L0.bx * L0.by);
+        const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3);
+        int k[VPL];
+        bool valid[VPL];
+        float pz[VPL];
         const float h = (float)g.spacing;
-        const float px = (i + 0.5f) * h, py = (j + 0.5f) * h, pz = (k + 0.5f) * h;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            k[v] = iz * L0.sz + (lane >> 4) + 4 * v;
+            valid[v] = i < g.nx && j < g.ny && k[v] < g.nz;
+            pz[v] = (k[v] + 0.5f) * h;
+        }
+        const float px = (i + 0.5f) * h, py = (j + 0.5f) * h;
         float cx, cy, cz, hd;
-        brick_geom(g, M.L0, b, cx, cy, cz, hd);
-        const int pb = M.l1_base + ((iz >> 1) * M.L1.by + (iy >> 1)) * M.L1.bx + (ix >> 1);
+        brick_geom(g, L0, b, cx, cy, cz, hd);
+        const int pb = M.l1_base +
+                       ((VPL == 1 ? iz >> 1 : iz) * M.L1.by + (iy >> 1)) * M.L1.bx + (ix >> 1);
         const long long beg = C.off1[pb];
         long long end = beg + C.cnt1[pb];
         if (end > C.cap1) end = beg;  // overflowed (flag already set)
@@ -447,99 +467,149 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         __syncthreads();
         const bool spill = n > kLdsList;
         const int count = spill ? (int)(end - beg) : n;
+        auto list_at = [&](int qq) { return spill ? C.pool1[beg + qq] : slist[qq]; };
         const float cap = dmin + hd + M.eps;
-        float best = FLT_MAX;
-        int arg = -1;
-        float bound = cap;
-        for (int qq = 0; qq < count; ++qq) {
-            const int t = uniform_int(spill ? C.pool1[beg + qq] : slist[qq]);
-            const TriRec T = C.rec[t];
-            const float dx = px - T.mx, dy = py - T.my, dz = pz - T.mz;
-            const float rr = T.r + bound;
-            const bool need = valid && fma3(dx, dx, dy, dy, dz, dz) <= rr * rr;
-            if (__ballot(need) == 0) continue;
-            ++evals;
-            const float d2 = tri_d2(T, px, py, pz);
-            if (d2 < best) {
-                best = d2;
-                arg = t;
-            }
-            bound = fminf(cap, sqrtf(best) * (1.0f + 4.0f * kU) + M.eps);
+        float best[VPL], bound[VPL];
+        int arg[VPL];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            best[v] = FLT_MAX;
+            bound[v] = cap;
+            arg[v] = -1;
         }
-        const size_t idx = valid ? ((size_t)k * g.ny + j) * g.nx + i : 0;
+        auto visit = [&](const TriRec& T, int t) {
+            bool need = false;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                const float dx = px - T.mx, dy = py - T.my, dz = pz[v] - T.mz;
+                const float rr = T.r + bound[v];
+                need |= valid[v] && fma3(dx, dx, dy, dy, dz, dz) <= rr * rr;
+            }
+            if (__ballot(need) == 0) return;
+            ++evals;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                const float d2 = tri_d2(T, px, py, pz[v]);
+                if (d2 < best[v]) {
+                    best[v] = d2;
+                    arg[v] = t;
+                }
+                bound[v] = fminf(cap, sqrtf(best[v]) * (1.0f + 4.0f * kU) + M.eps);
+            }
+        };
+        if (D == 0) {
+            for (int qq = 0; qq < count; ++qq) {
+                const int t = uniform_int(list_at(qq));
+                visit(load_uniform(C.rec, t), t);
+            }
+        } else {
+            for (int c0 = 0; c0 < count; c0 += D) {
+                const int nq = min(D, count - c0);
+                __syncthreads();
+                for (int e = lane; e < nq * 8; e += 64) {
+                    const int t = list_at(c0 + (e >> 3));
+                    reinterpret_cast<float4*>(&srec[e >> 3])[e & 7] =
+                        reinterpret_cast<const float4*>(C.rec + t)[e & 7];
+                }
+                __syncthreads();
+                for (int qq = 0; qq < nq; ++qq) visit(srec[qq], uniform_int(list_at(c0 + qq)));
+            }
+        }
         const uint8_t* occ = C.occ + M.cell_base;
         if (!BWD) {
-            const float d32 = sqrtf(best);
-            const bool near = valid && d32 <= h * (1.0f + 1e-3f) + 2.0f * M.eps;
-            double best64 = DBL_MAX;
-            if (__ballot(near) != 0) {
-                const double p64[3] = {g.corner[0] + (i + 0.5) * g.spacing,
-                                       g.corner[1] + (j + 0.5) * g.spacing,
-                                       g.corner[2] + (k + 0.5) * g.spacing};
+            bool near[VPL];
+            bool any_near = false;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                near[v] = valid[v] && sqrtf(best[v]) <= h * (1.0f + 1e-3f) + 2.0f * M.eps;
+                any_near |= near[v];
+            }
+            double best64[VPL];
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) best64[v] = DBL_MAX;
+            if (__ballot(any_near) != 0) {
+                double p64[VPL][3];
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    p64[v][0] = g.corner[0] + (i + 0.5) * g.spacing;
+                    p64[v][1] = g.corner[1] + (j + 0.5) * g.spacing;
+                    p64[v][2] = g.corner[2] + (k[v] + 0.5) * g.spacing;
+                }
                 for (int qq = 0; qq < count; ++qq) {
-                    const int t = uniform_int(spill ? C.pool1[beg + qq] : slist[qq]);
-                    const TriRec64 T = C.rec64[t];
-                    if (near) {
-                        const double d = point_tri_dist2_ref(p64, T.a, T.b, T.c);
-                        if (d < best64) best64 = d;
-                    }
+                    const int t = uniform_int(list_at(qq));
+                    const TriRec64 T = load_uniform(C.rec64, t);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v)
+                        if (near[v]) {
+                            const double d = point_tri_dist2_ref(p64[v], T.a, T.b, T.c);
+                            if (d < best64[v]) best64[v] = d;
+                        }
                 }
             }
-            if (valid) {
-                float* out = C.sdf + M.cell_base;
-                if (near) {
-                    const double d = sqrt(best64);
-                    out[idx] = (float)(occ[idx] ? -d : d);
+            float* out = C.sdf + M.cell_base;
+#pragma unroll
+            for (int v = 0; v < VPL; ++v) {
+                if (!valid[v]) continue;
+                const size_t idx = ((size_t)k[v] * g.ny + j) * g.nx + i;
+                if (near[v]) {
+                    const double d = sqrt(best64[v]);
+                    out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
                 } else {
+                    const float d32 = sqrtf(best[v]);
                     out[idx] = occ[idx] ? -d32 : d32;
                 }
             }
             __syncthreads();
             continue;
         }
-        float c[9];
-        bool act = false;
-        if (valid && arg >= 0) {
-            const TriRec T = C.rec[arg];
-            float v, w;
-            tri_bary(T, px, py, pz, v, w);
-            const float u = 1.0f - v - w;
-            const float qx = T.ax + v * T.abx + w * T.acx;
-            const float qy = T.ay + v * T.aby + w * T.acy;
-            const float qz = T.az + v * T.abz + w * T.acz;
-            const float ex = px - qx, ey = py - qy, ez = pz - qz;
-            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
-            const float gv = C.gsdf[M.cell_base + idx];
-            if (d > 0.0f && gv != 0.0f) {
-                const float s = occ[idx] ? -1.0f : 1.0f;
-                const float coef = -gv * s / d;
-                c[0] = coef * u * ex; c[1] = coef * u * ey; c[2] = coef * u * ez;
-                c[3] = coef * v * ex; c[4] = coef * v * ey; c[5] = coef * v * ez;
-                c[6] = coef * w * ex; c[7] = coef * w * ey; c[8] = coef * w * ez;
-                act = true;
-            }
-        }
-        while (true) {
-            const unsigned long long mask = __ballot(act);
-            if (mask == 0) break;
-            const int leader = __ffsll((long long)mask) - 1;
-            const int lt = __shfl(arg, leader);
-            const bool match = act && arg == lt;
-            float sum[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
-            if (lane == leader) {
-                const int* tv = C.tris + 3 * (size_t)lt;
-                float* gbase = C.grad + 3 * M.vbase;
-#pragma unroll
-                for (int vtx = 0; vtx < 3; ++vtx) {
-                    float* gp = gbase + 3 * (size_t)tv[vtx];
-                    atomicAdd(gp + 0, sum[3 * vtx + 0]);
-                    atomicAdd(gp + 1, sum[3 * vtx + 1]);
-                    atomicAdd(gp + 2, sum[3 * vtx + 2]);
+        for (int v = 0; v < VPL; ++v) {
+            float c[9];
+            bool act = false;
+            const size_t idx = valid[v] ? ((size_t)k[v] * g.ny + j) * g.nx + i : 0;
+            if (valid[v] && arg[v] >= 0) {
+                const TriRec T = C.rec[arg[v]];
+                float vv, ww;
+                tri_bary(T, px, py, pz[v], vv, ww);
+                const float u = 1.0f - vv - ww;
+                const float qx = T.ax + vv * T.abx + ww * T.acx;
+                const float qy = T.ay + vv * T.aby + ww * T.acy;
+                const float qz = T.az + vv * T.abz + ww * T.acz;
+                const float ex = px - qx, ey = py - qy, ez = pz[v] - qz;
+                const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+                const float gv = C.gsdf[M.cell_base + idx];
+                if (d > 0.0f && gv != 0.0f) {
+                    const float s = occ[idx] ? -1.0f : 1.0f;
+                    const float coef = -gv * s / d;
+                    c[0] = coef * u * ex; c[1] = coef * u * ey; c[2] = coef * u * ez;
+                    c[3] = coef * vv * ex; c[4] = coef * vv * ey; c[5] = coef * vv * ez;
+                    c[6] = coef * ww * ex; c[7] = coef * ww * ey; c[8] = coef * ww * ez;
+                    act = true;
                 }
             }
-            act = act && !match;
+            // wavefront-aggregated scatter: one group of lanes per distinct triangle
+            while (true) {
+                const unsigned long long mask = __ballot(act);
+                if (mask == 0) break;
+                const int leader = __ffsll((long long)mask) - 1;
+                const int lt = __shfl(arg[v], leader);
+                const bool match = act && arg[v] == lt;
+                float sum[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
+                if (lane == leader) {
+                    const int* tv = C.tris + 3 * (size_t)lt;
+                    float* gbase = C.grad + 3 * M.vbase;
+#pragma unroll
+                    for (int vtx = 0; vtx < 3; ++vtx) {
+                        float* gp = gbase + 3 * (size_t)tv[vtx];
+                        atomicAdd(gp + 0, sum[3 * vtx + 0]);
+                        atomicAdd(gp + 1, sum[3 * vtx + 1]);
+                        atomicAdd(gp + 2, sum[3 * vtx + 2]);
+                    }
+                }
+                act = act && !match;
+            }
         }
         __syncthreads();
     }
@@ -614,8 +684,23 @@ struct BatchEngine::Impl {
         for (void* p : owned) (void)hipFree(p);
     }
 
+    int vpl = 1, depth = 0;  // voxel-pass variant (voxels per lane, LDS tile depth)
+
     void launch_forward(hipStream_t st);
     void launch_backward(hipStream_t st);
+
+    template <bool BWD>
+    void launch_vox(hipStream_t st) {
+#define CMG_VOX(V, DD)                                                 \
+    if (vpl == V && depth == DD) {                                    \
+        vox_kernel<BWD, V, DD><<<grid64, 64, 0, st>>>(ctx);            \
+        CMG_HIP_LAUNCH();                                              \
+        return;                                                        \
+    }
+        CMG_VOX(1, 0) CMG_VOX(1, 32) CMG_VOX(1, 64) CMG_VOX(2, 0) CMG_VOX(2, 32) CMG_VOX(2, 64)
+#undef CMG_VOX
+        throw HipError("batch: unknown voxel-pass variant");
+    }
 };
 
 void BatchEngine::Impl::launch_forward(hipStream_t st) {
@@ -644,8 +729,7 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     CMG_HIP_LAUNCH();
     {
         TimedLaunch timed("batch_sdf", st);
-        vox_kernel<false><<<grid64, 64, 0, st>>>(C);
-        CMG_HIP_LAUNCH();
+        launch_vox<false>(st);
     }
 }
 
@@ -655,8 +739,7 @@ void BatchEngine::Impl::launch_backward(hipStream_t st) {
     CMG_HIP_LAUNCH();
     {
         TimedLaunch timed("batch_sdf_backward", st);
-        vox_kernel<true><<<grid64, 64, 0, st>>>(C);
-        CMG_HIP_LAUNCH();
+        launch_vox<true>(st);
     }
 }
 
@@ -715,6 +798,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.tile_base = (int)tiles;
         M.mask_base = words;
         M.L0 = make_level(g, kLeaf);
+        M.L0b = make_level(g, kLeaf, 2 * kLeaf);
         M.L1 = make_level(g, 2 * kLeaf);
         M.L2 = make_level(g, 4 * kLeaf);
         M.l2_base = (int)l2;
@@ -748,6 +832,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             if (ph == kPhL2) n = (long long)M.L2.bx * M.L2.by * M.L2.bz;
             if (ph == kPhL1) n = (long long)M.L1.bx * M.L1.by * M.L1.bz;
             if (ph == kPhL0) n = (long long)M.L0.bx * M.L0.by * M.L0.bz;
+            if (ph == kPhL0b) n = (long long)M.L0b.bx * M.L0b.by * M.L0b.bz;
             o[p + 1] = o[p] + n;
         }
     }
@@ -876,6 +961,63 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     }
     CMG_HIP(hipGraphLaunch(I.exec, st));
 }
+
+std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sdf,
+                                                float* d_grad, hipStream_t st) {
+    Impl& I = *p_;
+    I.ctx.sdf = d_sdf;
+    I.ctx.gsdf = d_gsdf;
+    I.ctx.grad = d_grad;
+    I.launch_forward(st);  // lists and occupancy for this geometry
+    std::vector<BatchVariant> out;
+    hipEvent_t e0, e1;
+    CMG_HIP(hipEventCreate(&e0));
+    CMG_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    int best_v = I.vpl, best_d = I.depth;
+    for (int v : {1, 2})
+        for (int d : {0, 32, 64}) {
+            I.vpl = v;
+            I.depth = d;
+            float total = 0;
+            for (int rep = 0; rep < 3; ++rep) {
+                reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 7, 8);
+                zero_grad_kernel<<<I.grid256, 256, 0, st>>>(I.ctx.grad, 3 * I.total_pts, I.ctx.ctr);
+                CMG_HIP(hipEventRecord(e0, st));
+                I.launch_vox<false>(st);
+                I.launch_vox<true>(st);
+                CMG_HIP(hipEventRecord(e1, st));
+                CMG_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                CMG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) total += ms;  // first run warms caches
+            }
+            const float ms = total / 2;
+            out.push_back(BatchVariant{v, d, ms});
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_v = v;
+                best_d = d;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_variant(best_v, best_d);
+    return out;
+}
+
+void BatchEngine::set_variant(int vpl, int depth) {
+    if (!((vpl == 1 || vpl == 2) && (depth == 0 || depth == 32 || depth == 64)))
+        throw InvalidInput("batch: variant must have vpl in {1,2} and depth in {0,32,64}");
+    Impl& I = *p_;
+    if (I.vpl != vpl || I.depth != depth) {
+        I.vpl = vpl;
+        I.depth = depth;
+        I.graph_flags = -1;  // recapture with the new kernels
+    }
+}
+
+BatchVariant BatchEngine::variant() const { return BatchVariant{p_->vpl, p_->depth, 0.0f}; }
 
 void BatchEngine::status(hipStream_t st, unsigned* overflow, unsigned long long* pairs) {
     BatchCounters h{};

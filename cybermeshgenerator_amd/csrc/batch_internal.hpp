@@ -21,12 +21,12 @@ struct MeshDesc {
     long long mask_base;  // first occupancy bit-mask word
     int words, ntx, nty;  // occupancy: mask words per column, column tiles per axis
     int tile_base;        // first column tile in the global tile arrays
-    sdfdev::Level L0, L1, L2;
+    sdfdev::Level L0, L0b, L1, L2;  // L0: 4x4x4 leaves, L0b: 4x4x8 leaves
     int l2_base, l1_base; // first 16^3 / 8^3 brick in the global brick arrays
     float eps;            // FP32 distance error margin for this grid
 };
 
-enum BatchPhase { kPhTile = 0, kPhL2 = 1, kPhL1 = 2, kPhL0 = 3, kNumPhase = 4 };
+enum BatchPhase { kPhTile = 0, kPhL2 = 1, kPhL1 = 2, kPhL0 = 3, kPhL0b = 4, kNumPhase = 5 };
 
 struct BatchCounters {
     unsigned long long queue[10];  // work queues of the persistent kernels
@@ -37,6 +37,13 @@ struct BatchCounters {
 };
 
 enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4 };
+
+// A voxel-pass variant: voxels per lane (1: 4x4x4 leaves, 2: 4x4x8) and the LDS tile
+// depth (0: records read with scalar loads, 32/64: staged through LDS in tiles).
+struct BatchVariant {
+    int vpl, depth;
+    float ms;  // forward + backward voxel passes, when measured by autotune
+};
 
 class BatchEngine {
   public:
@@ -62,6 +69,13 @@ class BatchEngine {
     // into a hipGraph on first use (per buffer set) and replays it.
     void run(int flags, const double* d_xyz, float* d_sdf, const float* d_gsdf, float* d_grad,
              hipStream_t st);
+
+    // Times every voxel-pass variant on this batch (forward + backward voxel passes, HIP
+    // events on st, after one forward builds the lists) and keeps the fastest.
+    std::vector<BatchVariant> autotune(const float* d_gsdf, float* d_sdf, float* d_grad,
+                                       hipStream_t st);
+    void set_variant(int vpl, int depth);
+    BatchVariant variant() const;
 
     // Synchronises st and reads the overflow flags and the evaluation count.
     void status(hipStream_t st, unsigned* overflow, unsigned long long* pairs);

@@ -445,6 +445,25 @@ cmg_status cmg_voxelize_batch_status(cmg_batch* b, void* stream, unsigned* overf
     });
 }
 
+cmg_status cmg_voxelize_batch_autotune(cmg_batch* b, float* sdf, const float* grad_sdf,
+                                       float* grad_xyz, void* stream, int* vpl, int* depth,
+                                       float* ms, char* errbuf, size_t len) {
+    if (!b || !sdf || !grad_sdf || !grad_xyz) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        auto all = b->engine->autotune(grad_sdf, sdf, grad_xyz, as_stream(stream));
+        if (ms)
+            for (size_t i = 0; i < all.size() && i < 6; ++i) ms[i] = all[i].ms;
+        cmg::hip::BatchVariant v = b->engine->variant();
+        if (vpl) *vpl = v.vpl;
+        if (depth) *depth = v.depth;
+    });
+}
+
+cmg_status cmg_voxelize_batch_set_variant(cmg_batch* b, int vpl, int depth) {
+    if (!b) return CMG_ERR_INVALID_INPUT;
+    return guarded(nullptr, 0, [&] { b->engine->set_variant(vpl, depth); });
+}
+
 const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch* b) {
     return b ? b->engine->occupancy() : nullptr;
 }

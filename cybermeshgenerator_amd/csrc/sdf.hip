@@ -229,7 +229,7 @@ __global__ __launch_bounds__(64) void sdf_voxel_kernel(VoxArgs A) {
     unsigned long long evals = 0;
     for (int q = 0; q < count; ++q) {
         const int t = uniform_int(spill ? A.items1[beg + q] : slist[q]);
-        const TriRec T = A.rec[t];
+        const TriRec T = load_uniform(A.rec, t);
         const float dx = px - T.mx, dy = py - T.my, dz = pz - T.mz;
         const float rr = T.r + bound;
         const bool need = valid && fma3(dx, dx, dy, dy, dz, dz) <= rr * rr;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(64) void sdf_voxel_kernel(VoxArgs A) {
                                    g.corner[2] + (k + 0.5) * g.spacing};
             for (int q = 0; q < count; ++q) {
                 const int t = uniform_int(spill ? A.items1[beg + q] : slist[q]);
-                const TriRec64 T = A.rec64[t];
+                const TriRec64 T = load_uniform(A.rec64, t);
                 if (near) {
                     const double d = point_tri_dist2_ref(p64, T.a, T.b, T.c);
                     if (d < best64) best64 = d;  // std::min(best, d) (voxelize.cpp:317)

@@ -37,21 +37,23 @@ struct __align__(8) TriRec64 {  // FP64 vertices for the near-surface re-evaluat
 };
 
 struct Level {
-    int s;           // brick edge in voxels
+    int s;           // brick edge in voxels along x and y
+    int sz;          // brick edge along z
     int bx, by, bz;  // bricks per axis
 };
 
-__host__ __device__ inline Level make_level(const GridSpec& g, int s) {
-    return Level{s, (g.nx + s - 1) / s, (g.ny + s - 1) / s, (g.nz + s - 1) / s};
+__host__ __device__ inline Level make_level(const GridSpec& g, int s, int sz = 0) {
+    if (sz == 0) sz = s;
+    return Level{s, sz, (g.nx + s - 1) / s, (g.ny + s - 1) / s, (g.nz + sz - 1) / sz};
 }
 
 // Centre (relative to the corner) and half-diagonal of brick b's voxel centres.
 __device__ inline void brick_geom(const GridSpec& g, const Level& L, int b, float& cx,
                                   float& cy, float& cz, float& hd) {
     const int ix = b % L.bx, iy = (b / L.bx) % L.by, iz = b / (L.bx * L.by);
-    const int lx = ix * L.s, ly = iy * L.s, lz = iz * L.s;
+    const int lx = ix * L.s, ly = iy * L.s, lz = iz * L.sz;
     const int hx = min(lx + L.s, g.nx) - 1, hy = min(ly + L.s, g.ny) - 1,
-              hz = min(lz + L.s, g.nz) - 1;
+              hz = min(lz + L.sz, g.nz) - 1;
     const float h = (float)g.spacing;
     cx = (0.5f * (lx + hx) + 0.5f) * h;
     cy = (0.5f * (ly + hy) + 0.5f) * h;
@@ -204,6 +206,30 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Loads of records every lane needs (uniform index t) through the constant address
+// space, so that they become scalar loads into SGPRs (broadcast operands of the VALU)
+// instead of 64 identical vector loads. The records are read-only during every kernel.
+typedef const __attribute__((address_space(4))) float* ConstF32Ptr;
+typedef const __attribute__((address_space(4))) double* ConstF64Ptr;
+
+__device__ __forceinline__ TriRec load_uniform(const TriRec* base, int t) {
+    ConstF32Ptr p = (ConstF32Ptr)(base + t);
+    TriRec r;
+    float* d = reinterpret_cast<float*>(&r);
+#pragma unroll
+    for (int i = 0; i < 32; ++i) d[i] = p[i];
+    return r;
+}
+
+__device__ __forceinline__ TriRec64 load_uniform(const TriRec64* base, int t) {
+    ConstF64Ptr p = (ConstF64Ptr)(base + t);
+    TriRec64 r;
+    double* d = reinterpret_cast<double*>(&r);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = p[i];
+    return r;
+}
 
 
 // Fills the FP32 and FP64 records of one triangle (vertices P[3][3], world coordinates).

@@ -83,38 +83,44 @@ def geodesic_sphere(freq: int):
     between the icosphere levels."""
     v0, f0 = _icosahedron()
     n = max(1, int(freq))
-    pts, keys, faces = [], {}, []
-
-    def vid(fa, i, j):
-        # point i*A + j*B + (n-i-j)*C on face fa, keyed by its integer weights
-        a, b, c = f0[fa]
-        w = {}
-        for vert, wt in ((a, i), (b, j), (c, n - i - j)):
-            if wt:
-                w[vert] = w.get(vert, 0) + wt
-        k = tuple(sorted(w.items()))
-        if k not in keys:
-            p = sum(v0[vv] * wt for vv, wt in k) / n
-            keys[k] = len(pts)
-            pts.append(p / np.linalg.norm(p))
-        return keys[k]
-
-    for fa in range(20):
-        for i in range(n):
-            for j in range(n - i):
-                p0, p1, p2 = vid(fa, i, j), vid(fa, i + 1, j), vid(fa, i, j + 1)
-                faces.append((p0, p1, p2))
-                if i + j + 1 < n:
-                    faces.append((p1, vid(fa, i + 1, j + 1), p2))
-    v = np.array(pts)
-    f = np.array(faces, dtype=np.int32)
-    # orient outward: the icosahedron faces are outward-wound as (A, B, C); with the
-    # weights above (i on A, j on B) each emitted triangle has the same handedness.
-    c = v[f].mean(axis=1)
-    nrm = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    # lattice points (i, j) with i + j <= n on one face: weights (i, j, n-i-j) on (A, B, C)
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    keep = ii + jj <= n
+    ii, jj = ii[keep], jj[keep]
+    lid = -np.ones((n + 1, n + 1), dtype=np.int64)
+    lid[ii, jj] = np.arange(ii.size)
+    # triangles of one face in local lattice ids
+    tri = []
+    for i in range(n):
+        for j in range(n - i):
+            tri.append((lid[i, j], lid[i + 1, j], lid[i, j + 1]))
+            if i + j + 1 < n:
+                tri.append((lid[i + 1, j], lid[i + 1, j + 1], lid[i, j + 1]))
+    tri = np.array(tri, dtype=np.int64)
+    npf = ii.size
+    # canonical integer key per point: (vertex, weight) pairs sorted by vertex id, with
+    # zero weights dropped, so points shared by faces get one id
+    verts = f0[:, None, :].repeat(npf, 1)                     # (20, npf, 3)
+    w = np.stack([ii, jj, n - ii - jj], -1)[None].repeat(20, 0)  # (20, npf, 3)
+    vid = np.where(w > 0, verts, 1 << 20)
+    order = np.argsort(vid, axis=-1)
+    vid = np.take_along_axis(vid, order, -1)
+    w = np.take_along_axis(w, order, -1)
+    key = np.concatenate([vid, w], -1).reshape(-1, 6)
+    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    pos = np.zeros((uniq.shape[0], 3))
+    for c in range(3):
+        vv = uniq[:, c]
+        ok = vv < (1 << 20)
+        pos[ok] += v0[vv[ok]] * uniq[ok, 3 + c][:, None]
+    pos /= np.linalg.norm(pos, axis=1, keepdims=True)
+    faces = (inv.reshape(20, npf)[:, tri]).reshape(-1, 3).astype(np.int32)
+    c = pos[faces].mean(axis=1)
+    nrm = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
     flip = (nrm * c).sum(1) < 0
-    f[flip] = f[flip][:, [0, 2, 1]]
-    return v, f
+    faces[flip] = faces[flip][:, [0, 2, 1]]
+    return pos, faces
 
 
 def torus(R: float, r: float, nu: int, nv: int):

@@ -199,6 +199,18 @@ cmg_status cmg_voxelize_batch_run(cmg_batch*, int flags, const double* xyz, floa
 cmg_status cmg_voxelize_batch_status(cmg_batch*, void* stream, unsigned* overflow,
                                      uint64_t* pairs, char* errbuf, size_t errbuf_len);
 
+/* Autotuner: times every voxel-pass variant on this batch — voxels per lane (1: 4x4x4
+ * bricks per wavefront, 2: 4x4x8) x LDS tile depth (0: triangle records read with scalar
+ * loads, 32 or 64: staged through LDS) — with HIP events on `stream` (forward + backward
+ * voxel passes, buffers as in cmg_voxelize_batch_run) and keeps the fastest. ms (optional,
+ * 6 entries) receives the timings in the order (1,0) (1,32) (1,64) (2,0) (2,32) (2,64). */
+cmg_status cmg_voxelize_batch_autotune(cmg_batch*, float* sdf, const float* grad_sdf,
+                                       float* grad_xyz, void* stream, int* vpl, int* depth,
+                                       float* ms, char* errbuf, size_t errbuf_len);
+
+/* Selects a voxel-pass variant explicitly (vpl 1 or 2, depth 0, 32 or 64). */
+cmg_status cmg_voxelize_batch_set_variant(cmg_batch*, int vpl, int depth);
+
 /* Device pointer to the occupancy bytes of every grid (valid after a forward). */
 const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch*);
 
