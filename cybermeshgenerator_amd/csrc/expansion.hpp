@@ -221,10 +221,80 @@ __device__ inline bool orient3d_exact(const double* pa, const double* pb, const 
     return true;
 }
 
+// Sign of the exact sum of N doubles held in registers (static indices only), by
+// repeated error-free VecSum passes (Ogita, Rump, Oishi, "Accurate sum and dot product",
+// SISC 2005): each pass replaces v by its rounded sum in v[N-1] and the rounding errors
+// below it, without changing the exact sum. The sign is certain once |v[N-1]| exceeds
+// the sum of the other magnitudes (with a margin for rounding that sum), and the sum is 0
+// once every entry is 0. Returns false if neither happened within MAXPASS passes.
+template <int N, int MAXPASS>
+__device__ inline bool sum_sign(double (&v)[N], int& sign) {
+#pragma unroll 1
+    for (int pass = 0; pass < MAXPASS; ++pass) {
+#pragma unroll
+        for (int i = 1; i < N; ++i) {
+            double s, e;
+            two_sum(v[i], v[i - 1], s, e);
+            v[i] = s;
+            v[i - 1] = e;
+        }
+        double tail = 0.0;
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i) tail += __builtin_fabs(v[i]);
+        const double top = v[N - 1];
+        if (__builtin_fabs(top) > tail * (1.0 + 0x1p-40)) {
+            sign = top > 0.0 ? 1 : -1;
+            return true;
+        }
+        if (top == 0.0 && tail == 0.0) {
+            sign = 0;
+            return true;
+        }
+    }
+    return false;
+}
+
+// orient3d sign when every difference a - d etc. is exact in FP64: the determinant is the
+// exact sum of six triple products of those differences, each split into four doubles
+// with FMA (x*y = p + e exactly, then p*z and e*z exactly). False if undecided.
+__device__ inline bool orient3d_sign_exact_diffs(const double* pa, const double* pb,
+                                                 const double* pc, const double* pd,
+                                                 int& sign) {
+    double d[9];
+    for (int s = 0; s < 3; ++s) {
+        const double* p = s == 0 ? pa : (s == 1 ? pb : pc);
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(p[a], -pd[a], x, y);
+            if (y != 0.0) return false;
+            // keep every triple product and its FMA error terms normal and finite
+            const double ax = __builtin_fabs(x);
+            if (ax != 0.0 && !(ax >= 0x1p-300 && ax <= 0x1p300)) return false;
+            d[3 * s + a] = x;
+        }
+    }
+    const double adx = d[0], ady = d[1], adz = d[2], bdx = d[3], bdy = d[4], bdz = d[5],
+                 cdx = d[6], cdy = d[7], cdz = d[8];
+    // adz*bdx*cdy - adz*cdx*bdy + bdz*cdx*ady - bdz*adx*cdy + cdz*adx*bdy - cdz*bdx*ady
+    const double t[6][3] = {{bdx, cdy, adz},  {-cdx, bdy, adz}, {cdx, ady, bdz},
+                            {-adx, cdy, bdz}, {adx, bdy, cdz},  {-bdx, ady, cdz}};
+    double v[24];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double p, e;
+        two_prod(t[k][0], t[k][1], p, e);
+        two_prod(p, t[k][2], v[4 * k], v[4 * k + 1]);
+        two_prod(e, t[k][2], v[4 * k + 2], v[4 * k + 3]);
+    }
+    return sum_sign<24, 6>(v, sign);
+}
+
 // Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small.
 template <int CAP>
 __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
                                           const double* pc, const double* pd) {
+    int fast;
+    if (orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
     double top;
     if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
     return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
