@@ -289,12 +289,89 @@ __device__ inline bool orient3d_sign_exact_diffs(const double* pa, const double*
     return sum_sign<24, 6>(v, sign);
 }
 
+// Double-word (hi + lo) arithmetic with the FMA algorithms of Joldes, Muller and Popescu,
+// "Tight and rigorous error bounds for basic building blocks of double-word arithmetic"
+// (ACM TOMS 2017): DWTimesDW3 (relative error <= 4u^2) and AccurateDWPlusDW
+// (<= 3u^2 + 13u^3), u = 2^-53.
+struct DW {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DW dw_mul(DW x, DW y) {
+    double ch, cl1;
+    two_prod(x.hi, y.hi, ch, cl1);
+    const double tl0 = x.lo * y.lo;
+    const double tl1 = __builtin_fma(x.hi, y.lo, tl0);
+    const double cl2 = __builtin_fma(x.lo, y.hi, tl1);
+    DW z;
+    fast_two_sum(ch, cl1 + cl2, z.hi, z.lo);
+    return z;
+}
+
+__device__ __forceinline__ DW dw_add(DW x, DW y) {
+    double sh, sl, th, tl;
+    two_sum(x.hi, y.hi, sh, sl);
+    two_sum(x.lo, y.lo, th, tl);
+    sl += th;
+    double vh, vl;
+    fast_two_sum(sh, sl, vh, vl);
+    vl += tl;
+    DW z;
+    fast_two_sum(vh, vl, z.hi, z.lo);
+    return z;
+}
+
+__device__ __forceinline__ DW dw_neg(DW x) { return DW{-x.hi, -x.lo}; }
+
+// orient3d sign from exact double-word differences evaluated in double-word arithmetic.
+// Each of the 15 operations adds at most ~5u^2 of the magnitudes it combines, all bounded
+// by the permanent P, so |det_dw - det| <= 2^-96 P with a wide margin; lanes inside that
+// band (in practice only exact zeros) are left to the expansion arithmetic. Inputs must
+// keep every product normal (same range check as the exact-difference path).
+__device__ inline bool orient3d_sign_dw(const double* pa, const double* pb, const double* pc,
+                                        const double* pd, int& sign) {
+    DW d[9];
+    double perm_in[9];
+    for (int s = 0; s < 3; ++s) {
+        const double* p = s == 0 ? pa : (s == 1 ? pb : pc);
+        for (int a = 0; a < 3; ++a) {
+            DW x;
+            two_sum(p[a], -pd[a], x.hi, x.lo);
+            const double ax = __builtin_fabs(x.hi);
+            if (ax != 0.0 && !(ax >= 0x1p-250 && ax <= 0x1p250)) return false;
+            d[3 * s + a] = x;
+            perm_in[3 * s + a] = ax;
+        }
+    }
+    const DW adx = d[0], ady = d[1], adz = d[2], bdx = d[3], bdy = d[4], bdz = d[5],
+             cdx = d[6], cdy = d[7], cdz = d[8];
+    const DW m1 = dw_add(dw_mul(bdx, cdy), dw_neg(dw_mul(cdx, bdy)));
+    const DW m2 = dw_add(dw_mul(cdx, ady), dw_neg(dw_mul(adx, cdy)));
+    const DW m3 = dw_add(dw_mul(adx, bdy), dw_neg(dw_mul(bdx, ady)));
+    const DW det = dw_add(dw_add(dw_mul(adz, m1), dw_mul(bdz, m2)), dw_mul(cdz, m3));
+    const double* A = perm_in;  // |adx| |ady| |adz| |bdx| ... (hi parts)
+    const double perm = (A[3] * A[7] + A[6] * A[4]) * A[2] + (A[6] * A[1] + A[0] * A[7]) * A[5] +
+                        (A[0] * A[4] + A[3] * A[1]) * A[8];
+    const double bound = 0x1p-96 * perm * (1.0 + 0x1p-40);
+    const double v = det.hi + det.lo;
+    if (v > bound) {
+        sign = 1;
+        return true;
+    }
+    if (-v > bound) {
+        sign = -1;
+        return true;
+    }
+    return false;
+}
+
 // Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small.
 template <int CAP>
 __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
                                           const double* pc, const double* pd) {
     int fast;
     if (orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
+    if (orient3d_sign_dw(pa, pb, pc, pd, fast)) return fast;
     double top;
     if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
     return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);

@@ -84,3 +84,67 @@ def test_fast_path_signs_are_exact():
         p = rng.integers(-50, 51, size=(4, 3)).astype(float)
         s = fast_sign(p)
         assert s == frac_orient3d(*p)
+
+
+def fma(a, b, c):
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def fast_two_sum(a, b):
+    s = a + b
+    return s, b - (s - a)
+
+
+def dw_mul(x, y):
+    ch, cl1 = two_prod(x[0], y[0])
+    tl0 = x[1] * y[1]
+    tl1 = fma(x[0], y[1], tl0)
+    cl2 = fma(x[1], y[0], tl1)
+    return fast_two_sum(ch, cl1 + cl2)
+
+
+def dw_add(x, y):
+    sh, sl = two_sum(x[0], y[0])
+    th, tl = two_sum(x[1], y[1])
+    sl += th
+    vh, vl = fast_two_sum(sh, sl)
+    vl += tl
+    return fast_two_sum(vh, vl)
+
+
+def dw_sign(p):
+    a, b, c, d = p
+    D = []
+    for q in (a, b, c):
+        for k in range(3):
+            D.append(two_sum(float(q[k]), -float(d[k])))
+    adx, ady, adz, bdx, bdy, bdz, cdx, cdy, cdz = D
+    neg = lambda x: (-x[0], -x[1])
+    m1 = dw_add(dw_mul(bdx, cdy), neg(dw_mul(cdx, bdy)))
+    m2 = dw_add(dw_mul(cdx, ady), neg(dw_mul(adx, cdy)))
+    m3 = dw_add(dw_mul(adx, bdy), neg(dw_mul(bdx, ady)))
+    det = dw_add(dw_add(dw_mul(adz, m1), dw_mul(bdz, m2)), dw_mul(cdz, m3))
+    A = [abs(x[0]) for x in D]
+    perm = ((A[3] * A[7] + A[6] * A[4]) * A[2] + (A[6] * A[1] + A[0] * A[7]) * A[5] +
+            (A[0] * A[4] + A[3] * A[1]) * A[8])
+    bound = 2.0 ** -96 * perm * (1.0 + 2.0 ** -40)
+    v = det[0] + det[1]
+    if v > bound:
+        return 1
+    if -v > bound:
+        return -1
+    return None
+
+
+def test_double_word_stage_signs_are_exact():
+    soa, kind = W.tuples_orient3d(3000, seed=33, degenerate=1.0)
+    decided_ulp = 0
+    for i in range(soa.shape[1]):
+        p = soa[:, i].reshape(4, 3)
+        s = dw_sign(p)
+        if s is None:
+            continue
+        assert s == frac_orient3d(*p), i
+        decided_ulp += kind[i] == 2
+    assert decided_ulp >= 0.9 * (kind == 2).sum()
