@@ -124,6 +124,36 @@ def cpu_baseline_predicates(soa_np, gpu_signs):
             "seconds": dt}
 
 
+def cpu_sdf_sample(W, budget_s=20.0):
+    """The reference's own SDF search (3-D buckets + Chebyshev shells, restated in the
+    oracle) on one core, over cells spread evenly through the C3 grid (256^3 torus),
+    sized to about budget_s seconds. The bucket build is timed separately (a run over
+    one cell) and taken out of the per-voxel rate."""
+    import oracle
+    case = W.torus_c3(seed=3)
+    g = oracle.make_grid(case.points, case.resolution, case.pad)
+    occ = oracle.occupancy(case.points, case.tris, g, nthreads=1)
+
+    def timed(count, stride):
+        t = time.perf_counter()
+        oracle.sdf_shell(case.points, case.tris, g, occ, first=stride // 2, count=count,
+                         stride=stride)
+        return time.perf_counter() - t
+
+    t_build = min(timed(1, g.ncell), timed(1, g.ncell))
+    probe = 256
+    t_probe = max(timed(probe, g.ncell // probe) - t_build, 1e-6)
+    n = int(min(g.ncell // 4, max(probe, probe * budget_s / t_probe)))
+    stride = g.ncell // n
+    t_run = timed(n, stride)
+    rate = n / max(t_run - t_build, 1e-9)
+    return {"voxels_per_s": rate, "cells_timed": n, "seconds": t_run,
+            "bucket_build_s": t_build, "cores": 1, "kind": "port",
+            "sample": f"{n} cells of the 256^3 C3 grid at stride {stride} (evenly spread), "
+                      f"reference shell search, one thread, bucket build "
+                      f"{t_build:.2f} s excluded ({cpu_model()})"}
+
+
 def bench_sdf(cm, W, steps, warmup):
     """C5: 64 uneven meshes, SDF forward+backward per step, batched (one graph of
     persistent kernels) vs plain (one mesh at a time, synchronised per mesh, same kernels).
@@ -266,6 +296,9 @@ def main():
     if not args.no_sdf:
         try:
             extra["sdf"] = bench_sdf(cm, W, args.steps, args.warmup)
+            if not args.no_cpu:
+                log("[bench] CPU reference SDF search sample (1 thread)")
+                extra["sdf"]["cpu_baseline"] = cpu_sdf_sample(W)
         except Exception as e:  # reported, never hides the predicate line
             log(f"[bench] sdf workloads failed: {e!r}")
             extra["sdf"] = {"error": repr(e)}

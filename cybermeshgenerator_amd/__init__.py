@@ -75,6 +75,8 @@ def _load() -> ctypes.CDLL:
     lib.cmg_orient3d_values.argtypes = [_P, _sz, _P, _sz, _P, _errbuf_t, _sz]
     lib.cmg_voxels_sdf_backward.restype = _i
     lib.cmg_voxels_sdf_backward.argtypes = [_P, _P, _P, _P, _errbuf_t, _sz]
+    lib.cmg_voxels_sdf_nearest.restype = _i
+    lib.cmg_voxels_sdf_nearest.argtypes = [_P, _P, _P, _errbuf_t, _sz]
     lib.cmg_backend_last.restype = ctypes.c_char_p
     lib.cmg_hip_available.restype = _i
     lib.cmg_hip_orient3d_signs.restype = _i
@@ -402,6 +404,22 @@ def sdf_with_grad(plc: PLC, resolution: int, pad: int, grad_sdf) -> tuple:
     finally:
         _lib.cmg_voxels_destroy(handle)
     return sdf, grad
+
+
+def sdf_nearest(plc: PLC, resolution: int, pad: int) -> np.ndarray:
+    """Per cell (flat, C ABI order), the index of the triangle whose closest point the
+    SDF adjoint uses (the FP32 nearest triangle; -1 where there is none), through
+    cmg_voxels_sdf_nearest."""
+    handle = _voxelize_handle(plc, resolution, 1, pad)
+    try:
+        (nx, ny, nz), _, _ = _grid_info(handle)
+        out = np.full(nx * ny * nz, -1, dtype=np.int32)
+        err = _err()
+        _check(_lib.cmg_voxels_sdf_nearest(plc._handle, handle, _ptr(out), err, len(err)),
+               err)
+    finally:
+        _lib.cmg_voxels_destroy(handle)
+    return out
 
 
 def plc_from_mesh(points, tris) -> PLC:

@@ -665,8 +665,9 @@ struct BatchEngine::Impl {
     // graph state
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t capture = nullptr;  // engine-owned stream the graph is recorded on
     int graph_flags = -1;
-    const void* graph_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
     unsigned grid64 = 0, grid256 = 0;
 
     template <class T>
@@ -681,6 +682,7 @@ struct BatchEngine::Impl {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamSynchronize(stream);
+        if (capture) (void)hipStreamDestroy(capture);
         for (void* p : owned) (void)hipFree(p);
     }
 
@@ -933,7 +935,10 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
         if (bwd) I.launch_backward(st);
         return;
     }
-    const void* key[5] = {d_xyz, d_sdf, d_gsdf, d_grad, st};
+    // The graph is recorded on an engine-owned non-blocking stream (the caller's stream
+    // may be the legacy default stream, which cannot be captured) and launched on st;
+    // it depends only on the buffer addresses and the pass selection.
+    const void* key[4] = {d_xyz, d_sdf, d_gsdf, d_grad};
     const int f = flags & (kBatchForward | kBatchBackward);
     if (!I.exec || I.graph_flags != f || std::memcmp(key, I.graph_key, sizeof key) != 0) {
         if (I.exec) {
@@ -944,17 +949,19 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
             CMG_HIP(hipGraphDestroy(I.graph));
             I.graph = nullptr;
         }
-        CMG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        if (!I.capture) CMG_HIP(hipStreamCreateWithFlags(&I.capture, hipStreamNonBlocking));
+        hipStream_t cs = I.capture;
+        CMG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
         try {
-            if (fwd) I.launch_forward(st);
-            if (bwd) I.launch_backward(st);
+            if (fwd) I.launch_forward(cs);
+            if (bwd) I.launch_backward(cs);
         } catch (...) {
             hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(st, &g);
+            (void)hipStreamEndCapture(cs, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        CMG_HIP(hipStreamEndCapture(st, &I.graph));
+        CMG_HIP(hipStreamEndCapture(cs, &I.graph));
         CMG_HIP(hipGraphInstantiate(&I.exec, I.graph, nullptr, nullptr, 0));
         I.graph_flags = f;
         std::memcpy(I.graph_key, key, sizeof key);

@@ -310,7 +310,30 @@ cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,
         require_device();
         g_last_backend = "hip";
         cmg::hip::sdf_backward_host(plc->xyz.data(), npts, tris.data(), tris.size() / 3,
-                                    grid->grid, grid->inside.data(), grad_sdf, grad_xyz);
+                                    grid->grid, grid->inside.data(), grad_sdf, grad_xyz,
+                                    nullptr);
+    });
+}
+
+cmg_status cmg_voxels_sdf_nearest(const cmg_plc* plc, const cmg_voxels* grid,
+                                  int32_t* tri_out, char* errbuf, size_t len) {
+    if (!plc || !grid || !tri_out) {
+        set_err(errbuf, len, "null argument");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        if (grid->mode != 1)
+            throw cmg::hip::InvalidInput("sdf nearest needs a grid voxelized with mode 1");
+        const size_t npts = plc->xyz.size() / 3;
+        std::vector<int> tris = fan_triangles(*plc);
+        for (int v : tris)
+            if (v < 0 || (size_t)v >= npts)
+                throw cmg::hip::InvalidInput("facet index out of range");
+        require_device();
+        g_last_backend = "hip";
+        cmg::hip::sdf_backward_host(plc->xyz.data(), npts, tris.data(), tris.size() / 3,
+                                    grid->grid, grid->inside.data(), nullptr, nullptr,
+                                    tri_out);
     });
 }
 

@@ -77,8 +77,10 @@ void voxelize_host(const double* xyz, size_t npts, const int* tris, size_t ntris
                    const GridSpec& g, bool sdf, uint8_t* occ, float* dist);
 
 // SDF adjoint from host arrays (occ = the grid's inside bits). Synchronous.
+// grad_sdf/grad_xyz (both or neither) give the adjoint; nearest (optional) receives the
+// triangle the adjoint uses for every cell (-1 where there is none).
 void sdf_backward_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
                        const GridSpec& g, const uint8_t* occ, const float* grad_sdf,
-                       float* grad_xyz);
+                       float* grad_xyz, int* nearest);
 
 }  // namespace cmg::hip

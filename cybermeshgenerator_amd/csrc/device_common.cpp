@@ -115,6 +115,10 @@ void timer_read(const char* name, double* total_ms, uint64_t* count) {
 }
 
 TimedLaunch::TimedLaunch(const char* n, hipStream_t s) : name(n), stream(s), on(false) {
+    // launches recorded into a graph are not timed (events in a capture would become
+    // graph nodes and be read before the graph ever runs)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     std::lock_guard<std::mutex> lk(g_timer_mu);
     TimerState* t = find_timer(name, false);
     if (!t || !t->enabled) return;

@@ -176,7 +176,8 @@ struct VoxArgs {
     const uint8_t* occ;
     float* dist;         // forward output
     const float* gsdf;   // backward input
-    float* grad;         // backward output (3 per point)
+    float* grad;         // backward output (3 per point); null: no scatter
+    int* nearest;        // optional: the triangle the adjoint uses, per cell
     unsigned long long* pairs;  // full FP32 evaluations (statistics)
 };
 
@@ -273,9 +274,10 @@ __global__ __launch_bounds__(64) void sdf_voxel_kernel(VoxArgs A) {
         return;
     }
     // ---- adjoint ----
+    if (A.nearest && valid) A.nearest[idx] = arg;
     float c[9];
     bool act = false;
-    if (valid && arg >= 0) {
+    if (valid && arg >= 0 && A.grad) {
         const TriRec T = A.rec[arg];
         float v, w;
         tri_bary(T, px, py, pz, v, w);
@@ -416,11 +418,11 @@ void launch_sdf_forward(const double* d_xyz, int npts, const int* d_tris, int nt
 
 void launch_sdf_backward(const double* d_xyz, int npts, const int* d_tris, int ntris,
                          const GridSpec& g, const uint8_t* d_occ, const float* d_gsdf,
-                         float* d_grad, DeviceArena& arena, hipStream_t st) {
+                         float* d_grad, int* d_nearest, DeviceArena& arena, hipStream_t st) {
     (void)npts;
     SdfState S = build_cull(d_xyz, d_tris, ntris, g, arena, st);
     VoxArgs A{S.rec, S.rec64, d_tris, g, S.L0, S.L1, S.off1, S.items1, S.eps, d_occ, nullptr,
-              d_gsdf, d_grad, nullptr};
+              d_gsdf, d_grad, d_nearest, nullptr};
     const int nb = S.L0.bx * S.L0.by * S.L0.bz;
     TimedLaunch timed("sdf_backward", st);
     sdf_voxel_kernel<true><<<nb, 64, 0, st>>>(A);

@@ -310,22 +310,31 @@ void voxelize_host(const double* xyz, size_t npts, const int* tris, size_t ntris
 
 void sdf_backward_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
                        const GridSpec& g, const uint8_t* occ, const float* grad_sdf,
-                       float* grad_xyz) {
+                       float* grad_xyz, int* nearest) {
     const size_t ncell = (size_t)g.nx * g.ny * g.nz;
+    const bool adj = grad_sdf && grad_xyz;
     hipStream_t st = nullptr;
     DeviceArena arena(st);
     double* d_xyz = arena.alloc<double>(3 * npts);
     int* d_tris = arena.alloc<int>(3 * ntris);
     uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
-    float* d_g = arena.alloc<float>(ncell);
-    float* d_grad = arena.alloc<float>(3 * npts);
+    float* d_g = adj ? arena.alloc<float>(ncell) : nullptr;
+    float* d_grad = adj ? arena.alloc<float>(3 * npts) : nullptr;
+    int* d_near = nearest ? arena.alloc<int>(ncell) : nullptr;
     CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
     CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
     CMG_HIP(hipMemcpyAsync(d_occ, occ, ncell, hipMemcpyHostToDevice, st));
-    CMG_HIP(hipMemcpyAsync(d_g, grad_sdf, sizeof(float) * ncell, hipMemcpyHostToDevice, st));
-    CMG_HIP(hipMemsetAsync(d_grad, 0, sizeof(float) * 3 * npts, st));
-    launch_sdf_backward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_g, d_grad, arena, st);
-    CMG_HIP(hipMemcpyAsync(grad_xyz, d_grad, sizeof(float) * 3 * npts, hipMemcpyDeviceToHost, st));
+    if (adj) {
+        CMG_HIP(hipMemcpyAsync(d_g, grad_sdf, sizeof(float) * ncell, hipMemcpyHostToDevice, st));
+        CMG_HIP(hipMemsetAsync(d_grad, 0, sizeof(float) * 3 * npts, st));
+    }
+    launch_sdf_backward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_g, d_grad, d_near,
+                        arena, st);
+    if (adj)
+        CMG_HIP(hipMemcpyAsync(grad_xyz, d_grad, sizeof(float) * 3 * npts,
+                               hipMemcpyDeviceToHost, st));
+    if (nearest)
+        CMG_HIP(hipMemcpyAsync(nearest, d_near, sizeof(int) * ncell, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
 }
 

@@ -74,6 +74,6 @@ void launch_sdf_forward(const double* d_xyz, int npts, const int* d_tris, int nt
 // for the upstream gradient d_gsdf, signs from d_occ held constant.
 void launch_sdf_backward(const double* d_xyz, int npts, const int* d_tris, int ntris,
                          const GridSpec& g, const uint8_t* d_occ, const float* d_gsdf,
-                         float* d_grad, DeviceArena& arena, hipStream_t st);
+                         float* d_grad, int* d_nearest, DeviceArena& arena, hipStream_t st);
 
 }  // namespace cmg::hip

@@ -105,6 +105,14 @@ cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,
                                    const float* grad_sdf, float* grad_xyz, char* errbuf,
                                    size_t errbuf_len);
 
+/* For the same grid: the index (into the PLC's fan-triangulated facet list) of the
+ * triangle whose closest point the adjoint uses for each cell, i.e. the FP32 nearest
+ * triangle (-1 where there is none). No reference counterpart (the reference's
+ * fill_distance, voxelize.cpp:331-358, keeps only the distance); useful for attribute
+ * transfer and for checking the adjoint. */
+cmg_status cmg_voxels_sdf_nearest(const cmg_plc* plc, const cmg_voxels* grid,
+                                  int32_t* tri_out, char* errbuf, size_t errbuf_len);
+
 /* Name of the backend that served the calling thread's last predicate or voxelize
  * call ("hip", or "none" before any call). Mirrors cmg::backend::last_backend
  * (src/backend/dispatch.cpp:65). */

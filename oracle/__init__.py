@@ -58,10 +58,11 @@ def _load() -> ctypes.CDLL:
     lib.orc_occupancy.argtypes = [P, P, sz, P, P, P, i32]
     lib.orc_point_tri_dist2.restype = d
     lib.orc_point_tri_dist2.argtypes = [P] * 5
-    lib.orc_sdf_shell.argtypes = [P, P, sz, P, P, P, sz, sz]
+    lib.orc_sdf_shell.argtypes = [P, P, sz, P, P, P, sz, sz, sz]
     lib.orc_sdf_brute.argtypes = [P, P, sz, P, P, P, i32]
     lib.orc_sdf_bvh.argtypes = [P, P, sz, P, P, P, P, P, sz, sz, i32]
-    lib.orc_sdf_grad.argtypes = [P, sz, P, sz, P, P, P, P, i32]
+    lib.orc_sdf_grad.argtypes = [P, sz, P, sz, P, P, P, P, P, i32]
+    lib.orc_sdf_dist2_to.argtypes = [P, P, P, P, P, i32]
     lib.orc_sdf_fd.restype = d
     lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, P, P, i32, i32, d, i32]
     return lib
@@ -211,14 +212,14 @@ def point_tri_dist2(p, a, b, c):
     return d2, bary
 
 
-def sdf_shell(pts, tris, grid: Grid, occ, first=0, count=0) -> np.ndarray:
+def sdf_shell(pts, tris, grid: Grid, occ, first=0, count=0, stride=1) -> np.ndarray:
     pts = _f64(pts).reshape(-1, 3)
     tris = _i32(tris).reshape(-1, 3)
     occ = np.ascontiguousarray(occ, dtype=np.uint8)
     out = np.zeros(grid.ncell, dtype=np.float32)
     g = grid._c()
     lib().orc_sdf_shell(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g), _p(occ),
-                        _p(out), first, count)
+                        _p(out), first, count, stride)
     return out
 
 
@@ -248,16 +249,37 @@ def sdf_bvh(pts, tris, grid: Grid, occ, first=0, count=0, nthreads=0):
     return dist, best, arg
 
 
-def sdf_grad(pts, tris, grid: Grid, occ, gout, nthreads=0) -> np.ndarray:
+def sdf_grad(pts, tris, grid: Grid, occ, gout, nthreads=0, arg=None) -> np.ndarray:
+    """FP64 adjoint dL/dpts; arg (per cell, optional) names the triangle whose closest
+    point each cell uses, instead of the FP64 nearest one."""
     pts = _f64(pts).reshape(-1, 3)
     tris = _i32(tris).reshape(-1, 3)
     occ = np.ascontiguousarray(occ, dtype=np.uint8)
     gout = np.ascontiguousarray(gout, dtype=np.float32)
     grad = np.zeros_like(pts)
     g = grid._c()
+    pa = None
+    if arg is not None:
+        arg = np.ascontiguousarray(arg, dtype=np.int32)
+        if arg.size != grid.ncell:
+            raise ValueError("arg must have one entry per cell")
+        pa = _p(arg)
     lib().orc_sdf_grad(_p(pts), pts.shape[0], _p(tris), tris.shape[0], ctypes.byref(g),
-                       _p(occ), _p(gout), _p(grad), nthreads)
+                       _p(occ), _p(gout), pa, _p(grad), nthreads)
     return grad
+
+
+def sdf_dist2_to(pts, tris, grid: Grid, arg, nthreads=0) -> np.ndarray:
+    """FP64 squared distance from each cell centre to triangle arg[cell] (-1 if < 0)."""
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    arg = np.ascontiguousarray(arg, dtype=np.int32)
+    if arg.size != grid.ncell:
+        raise ValueError("arg must have one entry per cell")
+    out = np.zeros(grid.ncell, dtype=np.float64)
+    g = grid._c()
+    lib().orc_sdf_dist2_to(_p(pts), _p(tris), ctypes.byref(g), _p(arg), _p(out), nthreads)
+    return out
 
 
 def sdf_fd(pts, tris, grid: Grid, occ, gout, vert: int, axis: int, h=1e-5,

@@ -93,11 +93,12 @@ double orc_point_tri_dist2(const double* p, const double* a, const double* b,
                            const double* c, double* bary);
 
 /* SDF by the reference's own 3-D bucket + Chebyshev-shell search
- * (voxelize.cpp:251-358). Single-threaded like the reference. Cells are processed
- * in flat order from `first` to `first+count-1` (count = 0 means all); cells outside
- * that range are left untouched. */
+ * (voxelize.cpp:251-358). Single-threaded like the reference. Cells first,
+ * first+stride, ... (count of them; count = 0 means to the end of the grid) are
+ * computed in that order; other cells are left untouched. */
 void orc_sdf_shell(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
-                   const uint8_t* occ, float* dist, size_t first, size_t count);
+                   const uint8_t* occ, float* dist, size_t first, size_t count,
+                   size_t stride);
 
 /* SDF by testing every triangle (the brute-force check of tests/test_voxelize.cpp:
  * 194-275). Same output as orc_sdf_shell. */
@@ -114,11 +115,17 @@ void orc_sdf_bvh(const double* pts, const int* tris, size_t ntris, const orc_gri
 
 /* FP64 SDF-magnitude adjoint: for L = sum_v g[v] * s[v] * d[v] with the sign s
  * (+1 outside, -1 inside, from occ) held constant, accumulates dL/dxyz into
- * grad[3*npts] (zeroed first). The closest feature is the minimising triangle's
- * closest point; d = 0 carries zero gradient. */
+ * grad[3*npts] (zeroed first). The closest feature is the closest point of the
+ * minimising triangle, or of triangle arg[cell] when arg is given (-1: no
+ * contribution); d = 0 carries zero gradient. */
 void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const orc_grid* g, const uint8_t* occ, const float* gout,
-                  double* grad, int nthreads);
+                  const int32_t* arg, double* grad, int nthreads);
+
+/* FP64 squared distance from every cell centre to triangle arg[cell] (-1 where
+ * arg[cell] < 0). */
+void orc_sdf_dist2_to(const double* pts, const int* tris, const orc_grid* g,
+                      const int32_t* arg, double* out, int nthreads);
 
 /* FP64 central finite difference of the same L with respect to coordinate `axis` of
  * vertex `vert`, step h, signs held at their unperturbed values. base_best/base_arg

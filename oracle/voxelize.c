@@ -313,10 +313,12 @@ static void shell_visit(shell_ctx* c, int i, int j, int k) {
 }
 
 void orc_sdf_shell(const double* pts, const int* tris, size_t ntris, const orc_grid* g,
-                   const uint8_t* occ, float* dist, size_t first, size_t count) {
+                   const uint8_t* occ, float* dist, size_t first, size_t count,
+                   size_t stride) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t ncell = (size_t)nx * ny * nz;
-    if (count == 0) count = ncell - first;
+    if (stride == 0) stride = 1;
+    if (count == 0) count = (ncell - first + stride - 1) / stride;
     int* cnt = (int*)calloc(ncell + 1, sizeof(int));
     int* rng = (int*)malloc(sizeof(int) * 6 * (ntris ? ntris : 1));
     for (size_t t = 0; t < ntris; ++t) {
@@ -356,7 +358,9 @@ void orc_sdf_shell(const double* pts, const int* tris, size_t ntris, const orc_g
     sc.seen = (int*)malloc(sizeof(int) * (ntris ? ntris : 1));
     for (size_t t = 0; t < ntris; ++t) sc.seen[t] = -1;
     const int max_r = nx + ny + nz;
-    for (size_t idx = first; idx < first + count; ++idx) {
+    for (size_t c = 0; c < count; ++c) {
+        const size_t idx = first + c * stride;
+        if (idx >= ncell) break;
         int k = (int)(idx / ((size_t)nx * ny));
         int j = (int)((idx / nx) % ny);
         int i = (int)(idx % nx);
@@ -603,7 +607,7 @@ void orc_sdf_bvh(const double* pts, const int* tris, size_t ntris, const orc_gri
 
 void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const orc_grid* g, const uint8_t* occ, const float* gout,
-                  double* grad, int nthreads) {
+                  const int32_t* arg, double* grad, int nthreads) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t ncell = (size_t)nx * ny * nz;
     bvh_t b = bvh_build(pts, tris, ntris);
@@ -622,12 +626,14 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
             double p[3];
             cell_center(g, i, j, k, p);
             int t = -1;
-            double d2 = bvh_nearest(&b, pts, tris, p, NULL, &t);
-            if (t < 0 || !(d2 > 0)) continue;
+            if (arg) t = arg[idx];
+            else bvh_nearest(&b, pts, tris, p, NULL, &t);
+            if (t < 0) continue;
             const int* tv = tris + 3 * (size_t)t;
             double w[3];
-            orc_point_tri_dist2(p, pts + 3 * (size_t)tv[0], pts + 3 * (size_t)tv[1],
-                                pts + 3 * (size_t)tv[2], w);
+            double d2 = orc_point_tri_dist2(p, pts + 3 * (size_t)tv[0],
+                                            pts + 3 * (size_t)tv[1], pts + 3 * (size_t)tv[2], w);
+            if (!(d2 > 0)) continue;
             double q[3];
             for (int a = 0; a < 3; ++a)
                 q[a] = w[0] * pts[3 * (size_t)tv[0] + a] + w[1] * pts[3 * (size_t)tv[1] + a] +
@@ -720,4 +726,26 @@ double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
     bvh_free(&b);
     (void)npts;
     return total / (2.0 * h);
+}
+
+void orc_sdf_dist2_to(const double* pts, const int* tris, const orc_grid* g,
+                      const int32_t* arg, double* out, int nthreads) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    const size_t ncell = (size_t)nx * ny * nz;
+#pragma omp parallel for schedule(static) num_threads(NT(nthreads))
+    for (size_t idx = 0; idx < ncell; ++idx) {
+        const int t = arg[idx];
+        if (t < 0) {
+            out[idx] = -1.0;
+            continue;
+        }
+        int k = (int)(idx / ((size_t)nx * ny));
+        int j = (int)((idx / nx) % ny);
+        int i = (int)(idx % nx);
+        double p[3];
+        cell_center(g, i, j, k, p);
+        const int* tv = tris + 3 * (size_t)t;
+        out[idx] = orc_point_tri_dist2(p, pts + 3 * (size_t)tv[0], pts + 3 * (size_t)tv[1],
+                                       pts + 3 * (size_t)tv[2], NULL);
+    }
 }

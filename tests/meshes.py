@@ -115,3 +115,4 @@ def frac_insphere(a, b, c, d, e) -> int:
         rows.append(r + [r[0] * r[0] + r[1] * r[1] + r[2] * r[2]])
     v = det4(rows)
     return (v > 0) - (v < 0)
+

@@ -66,22 +66,29 @@ def test_batch_graph_replays_equal_eager(gpu):
 
 
 def test_batch_backward_matches_fp64_gradient(gpu):
+    """Per mesh: the batched adjoint equals the single-mesh adjoint (same FP32 nearest
+    triangles; float atomics sum in a different order) and is within the FP32-nearest
+    tolerance of the FP64 adjoint (rel L2 < 3e-3)."""
     from cybermeshgenerator_amd.batch import SdfBatch
     import torch
     cases = _cases()[:3]
     b = SdfBatch([(p, t) for p, t, _, _ in cases], [r for *_, r, _ in cases],
                  [d for *_, d in cases])
-    gs = torch.from_numpy(W.upstream_grad(b.cells, seed=3)).cuda()
-    _, grad = b.step(gs, graph=True)
+    gsp = W.upstream_grad(b.cells, seed=3)
+    _, grad = b.step(torch.from_numpy(gsp).cuda(), graph=True)
     gparts = b.mesh_points(grad.cpu().numpy())
-    gsp = gs.cpu().numpy()
     for m, (p, t, r, d) in enumerate(cases):
         gr = b.grids[m]
+        gm = gsp[gr.offset:gr.offset + gr.ncell]
+        _, single = gpu.sdf_with_grad(gpu.plc_from_mesh(p, t), r, d, gm)
+        rel_single = np.linalg.norm(gparts[m] - single) / np.linalg.norm(single)
         g = oracle.make_grid(p, r, d)
         occ = oracle.occupancy(p, t, g)
-        ref = oracle.sdf_grad(p, t, g, occ, gsp[gr.offset:gr.offset + gr.ncell])
+        ref = oracle.sdf_grad(p, t, g, occ, gm)
         rel = np.linalg.norm(gparts[m] - ref) / np.linalg.norm(ref)
-        assert rel < 1e-3, (m, rel)
+        print(f"mesh {m}: vs single-mesh adjoint {rel_single:.2e}, vs FP64 {rel:.2e}")
+        assert rel_single < 1e-4, (m, rel_single)
+        assert rel < 3e-3, (m, rel)
 
 
 def test_autograd_function(gpu):
@@ -92,15 +99,16 @@ def test_autograd_function(gpu):
     tris = torch.tensor(case.tris, device="cuda")
     sdf = signed_distance(pts, tris, resolution=48, pad=0)
     assert sdf.shape == (48, 48, 48) and sdf.dtype == torch.float32
-    gout = torch.from_numpy(W.upstream_grad(48 ** 3, seed=5)).cuda().view(48, 48, 48)
-    (sdf * gout).sum().backward()
     g = oracle.make_grid(case.points, 48, 0)
+    gflat = W.upstream_grad(48 ** 3, seed=5)
+    # flat cell index (k*ny + j)*nx + i; the tensor is indexed [i, j, k]
+    gout = torch.from_numpy(gflat.reshape(48, 48, 48)).permute(2, 1, 0).cuda()
+    (sdf * gout).sum().backward()
     occ = oracle.occupancy(case.points, case.tris, g)
-    gflat = gout.permute(2, 1, 0).contiguous().view(-1).cpu().numpy()
     ref = oracle.sdf_grad(case.points, case.tris, g, occ, gflat)
     got = pts.grad.cpu().numpy()
     rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
-    assert rel < 1e-3, rel
+    assert rel < 3e-3, rel   # FP32 nearest-triangle tolerance, as for the C ABI adjoint
     # forward values equal the C ABI path bit for bit
     plc = gpu.plc_from_mesh(case.points, case.tris)
     want = gpu.voxelize(plc, resolution=48, mode="sdf", pad=0).grid

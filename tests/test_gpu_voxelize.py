@@ -133,24 +133,56 @@ def test_sdf_torus_c3_full(gpu):
     print("C3 FP32 SDF error vs FP64:", r)
 
 
+def _fp32_distance_margin(g):
+    """The device's FP32 distance error margin for a grid: 64 u times the grid diagonal
+    plus 1e-6 cell widths (u = 2^-24)."""
+    return 64.0 * 2.0 ** -24 * max(g.dims) * g.spacing * 3 ** 0.5 + 1e-6 * g.spacing
+
+
 def test_sdf_gradient_c4(gpu):
-    """SDF adjoint at C4 size against the FP64 CPU gradient and FP64 central differences
-    (h = 1e-5) on 256 random vertices."""
+    """SDF adjoint at C4 size (20,480 triangles, 128^3 cells, N(0,1) upstream gradient).
+
+    The adjoint follows the FP32 nearest triangle. Where two triangles are equidistant to
+    within the FP32 distance error (next to a shared edge, or on the medial axis) that
+    choice may differ from the FP64 one, and the cell's contribution lands on the other
+    triangle's vertices. So the check has three parts:
+      1. the triangle the adjoint uses is a nearest triangle to within twice the FP32
+         distance margin (FP64 distances, every cell);
+      2. given those triangles, the FP64 CPU adjoint equals the device adjoint to FP32
+         accumulation accuracy (rel L2 < 1e-4, max < 1e-3 of the largest entry);
+      3. against the FP64 adjoint of the FP64 nearest triangles and against FP64 central
+         differences (h = 1e-5, 256 random vertices): rel L2 < 3e-3 and every entry
+         within 3e-2 of the largest."""
     case = W.harmonic_c4(seed=4)
     pts, tris = case.points, case.tris
     g = oracle.make_grid(pts, 128, 0)
     gout = W.upstream_grad(g.ncell, seed=40)
     plc = gpu.plc_from_mesh(pts, tris)
     sdf, grad = gpu.sdf_with_grad(plc, 128, 0, gout)
+    near = gpu.sdf_nearest(plc, 128, 0)
     occ = oracle.occupancy(pts, tris, g)
-    ref = oracle.sdf_grad(pts, tris, g, occ, gout)
-    scale = np.abs(ref).max()
-    err = np.abs(grad - ref)
-    rel = np.linalg.norm(grad - ref) / np.linalg.norm(ref)
-    print(f"C4 gradient: max|err|/max|g| = {err.max() / scale:.3e}, rel L2 = {rel:.3e}")
-    assert np.abs(grad[-2:]).max() == 0            # anchors are not referenced
-    assert rel < 1e-3 and err.max() < 1e-2 * scale
     _, best, arg = oracle.sdf_bvh(pts, tris, g, occ)
+    # 1. a nearest triangle, to within the FP32 margin
+    assert (near >= 0).all() and (near < tris.shape[0]).all()
+    d_used = np.sqrt(oracle.sdf_dist2_to(pts, tris, g, near))
+    gap = d_used - np.sqrt(best)
+    assert gap.min() >= 0 and gap.max() <= 2 * _fp32_distance_margin(g), gap.max()
+    flips = (near != arg).mean()
+    # 2. the scatter itself
+    same = oracle.sdf_grad(pts, tris, g, occ, gout, arg=near)
+    scale = np.abs(same).max()
+    rel_same = np.linalg.norm(grad - same) / np.linalg.norm(same)
+    max_same = np.abs(grad - same).max() / scale
+    # 3. the FP64 adjoint and finite differences
+    ref = oracle.sdf_grad(pts, tris, g, occ, gout)
+    rel = np.linalg.norm(grad - ref) / np.linalg.norm(ref)
+    max_ref = np.abs(grad - ref).max() / scale
+    print(f"C4 gradient: cells on another nearest triangle {flips:.2e} (max distance gap "
+          f"{gap.max():.2e}); vs FP64 adjoint of the same triangles rel L2 {rel_same:.2e}, "
+          f"max {max_same:.2e}; vs FP64 adjoint rel L2 {rel:.2e}, max {max_ref:.2e}")
+    assert np.abs(grad[-2:]).max() == 0            # anchors are not referenced
+    assert rel_same < 1e-4 and max_same < 1e-3
+    assert rel < 3e-3 and max_ref < 3e-2
     rng = np.random.default_rng(41)
     nv = pts.shape[0] - 2
     fd_err = []
@@ -158,8 +190,9 @@ def test_sdf_gradient_c4(gpu):
         for axis in range(3):
             fd = oracle.sdf_fd(pts, tris, g, occ, gout, int(v), axis, h=1e-5,
                                base=(best, arg))
-            fd_err.append(abs(fd - grad[v, axis]) / (1e-3 * scale + abs(fd)))
             assert abs(fd - ref[v, axis]) <= 1e-4 * scale + 1e-3 * abs(fd), (v, axis)
+            fd_err.append(abs(fd - grad[v, axis]) / scale)
     fd_err = np.array(fd_err)
-    print(f"C4 gradient vs FD: max rel {fd_err.max():.3e}, median {np.median(fd_err):.3e}")
-    assert np.median(fd_err) < 1e-3 and fd_err.max() < 5e-2
+    print(f"C4 gradient vs FD: max |err| / max|g| {fd_err.max():.3e}, median "
+          f"{np.median(fd_err):.3e}")
+    assert fd_err.max() < 3e-2 and np.median(fd_err) < 1e-3

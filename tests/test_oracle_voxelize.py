@@ -209,6 +209,23 @@ def test_gradient_matches_finite_differences():
             fd = oracle.sdf_fd(pts, f, g, occ, gout, int(vert), axis, h=1e-6)
             assert abs(fd - grad[vert, axis]) <= 1e-4 * (1 + abs(fd)), (vert, axis, fd,
                                                                       grad[vert, axis])
+    # the same gradient when the nearest triangle of every cell is passed in, and the
+    # distances to those triangles are the nearest distances
+    _, best, arg = oracle.sdf_bvh(pts, f, g, occ)
+    # (threads sum their partial gradients in a schedule-dependent order)
+    assert np.allclose(oracle.sdf_grad(pts, f, g, occ, gout, arg=arg), grad, rtol=1e-12,
+                       atol=1e-12 * np.abs(grad).max())
+    assert np.array_equal(oracle.sdf_dist2_to(pts, f, g, arg), best)
+    # a cell given another triangle moves its contribution onto that triangle's vertices
+    c = int(np.argmax(np.abs(gout)))
+    other = (arg[c] + 1) % f.shape[0]
+    one = np.zeros_like(gout)
+    one[c] = gout[c]
+    moved = arg.copy()
+    moved[c] = other
+    g1 = oracle.sdf_grad(pts, f, g, occ, one, arg=moved)
+    assert set(np.nonzero(np.abs(g1).sum(axis=1))[0]) <= set(f[other].tolist())
+    assert oracle.sdf_dist2_to(pts, f, g, moved)[c] >= best[c]
 
 
 def test_invalid_input_rejected():
