@@ -280,7 +280,9 @@ def main():
     extra = {
         "orient3d": {"tuples": n, "ms_per_batch": pr["ms"], "filter_kernel_ms": pr["filter_ms"],
                      "stage_fraction": {"fp32": st["fp32"] / n, "fp64": st["fp64"] / n,
-                                        "exact": st["exact"] / n},
+                                        "exact": st["exact"] / n,
+                                        "exact_inline": st["exact_inline"] / n,
+                                        "expansion": (st["exact"] - st["exact_inline"]) / n},
                      "wide_lanes": st["wide"], "failed_lanes": st["failed"]},
     }
     ins = pr["insphere"]

@@ -37,7 +37,8 @@ class PredStats(ctypes.Structure):
     """Per-stage lane counts of a batched-sign call (cmg_pred_stats)."""
     _fields_ = [("lanes", ctypes.c_uint64), ("fp32", ctypes.c_uint64),
                 ("fp64", ctypes.c_uint64), ("exact", ctypes.c_uint64),
-                ("wide", ctypes.c_uint64), ("failed", ctypes.c_uint64)]
+                ("wide", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("exact_inline", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}

@@ -399,7 +399,8 @@ cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
         out->lanes = n;
         out->fp32 = c.fp32;
         out->fp64 = c.fp64;
-        out->exact = c.exact;
+        out->exact = c.exact + c.inline_;
+        out->exact_inline = c.inline_;
         out->wide = c.wide_n;
         out->failed = c.failed;
     });

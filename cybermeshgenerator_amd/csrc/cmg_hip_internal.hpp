@@ -15,11 +15,12 @@ constexpr int kModeCompat = 1;  // what cmg::robust::orient3d / insphere return 
 // ---- batched predicates ---------------------------------------------------------------
 
 struct PredCounters {
-    unsigned long long fp32;   // lanes decided by the FP32 filter
-    unsigned long long fp64;   // lanes decided by the FP64 stage
-    unsigned long long exact;  // lanes sent to expansion arithmetic
-    unsigned int wide_n;       // lanes re-run with wide expansion capacity
-    unsigned int failed;       // lanes whose expansions outgrew even the wide capacity
+    unsigned long long fp32;    // lanes decided by the FP32 filter
+    unsigned long long fp64;    // lanes decided by the FP64 stage
+    unsigned long long exact;   // lanes sent to expansion arithmetic
+    unsigned int wide_n;        // lanes re-run with wide expansion capacity
+    unsigned int failed;        // lanes whose expansions outgrew even the wide capacity
+    unsigned long long inline_; // lanes decided exactly inside the streaming kernel
 };
 
 // Launch shape of a filter pass: `iters` grid-stride iterations per thread, one queue
@@ -33,6 +34,7 @@ struct PredWorkspace {
     uint32_t* wave_q = nullptr;   // per-wave queued-lane count
     uint32_t* wave_32 = nullptr;  // per-wave FP32-decided count
     uint32_t* wave_64 = nullptr;  // per-wave FP64-decided count
+    uint32_t* wave_x = nullptr;   // per-wave count decided by the inline exact stage
     uint32_t* queue = nullptr;    // nwaves segments of seg entries
     uint32_t* wide = nullptr;     // wide-capacity queue (n entries)
     unsigned seg = 0, nwaves = 0;

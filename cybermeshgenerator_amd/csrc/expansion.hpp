@@ -365,13 +365,14 @@ __device__ inline bool orient3d_sign_dw(const double* pa, const double* pb, cons
     return false;
 }
 
-// Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small.
-template <int CAP>
+// Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small. FAST = false
+// skips the register-only stages (for lanes that already failed them).
+template <int CAP, bool FAST = true>
 __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
                                           const double* pc, const double* pd) {
     int fast;
-    if (orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
-    if (orient3d_sign_dw(pa, pb, pc, pd, fast)) return fast;
+    if (FAST && orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
+    if (FAST && orient3d_sign_dw(pa, pb, pc, pd, fast)) return fast;
     double top;
     if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
     return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);

@@ -220,6 +220,23 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
     }
 }
 
+// Register-only exact stage run inside the streaming kernels for the lanes the filters
+// leave undecided (orient3d only): an exact sum of the triple products when the
+// coordinate differences are exact, else double-word arithmetic with a rigorous bound.
+// Lanes it cannot decide (exact zeros with inexact differences, extreme exponents) go to
+// the expansion pass.
+template <int NPTS>
+__device__ __forceinline__ bool decide_inline_exact(const double* v, int& s) {
+    if constexpr (NPTS == 4) {
+        if (orient3d_sign_exact_diffs(v, v + 3, v + 6, v + 9, s)) return true;
+        return orient3d_sign_dw(v, v + 3, v + 6, v + 9, s);
+    } else {
+        (void)v;
+        (void)s;
+        return false;
+    }
+}
+
 // ---- tuple sources -------------------------------------------------------------------
 
 template <int NPTS>
@@ -273,29 +290,100 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 // Grid-stride kernels run the same number of iterations in every lane (`iters`), so the
 // ballots and the per-wave fill count stay wave-uniform.
 
+// Intra-wave ordering of LDS traffic: the staging ring below is written by some lanes
+// and read by others of the same wave, with no workgroup barrier (waves progress
+// independently).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-wave LDS ring of tuples the filters left undecided (orient3d): they are pushed as
+// the stream goes by and handed to the register-only exact stage 64 at a time, so that
+// stage runs on full wavefronts instead of on the ~10 % of lanes that need it.
+constexpr unsigned kRing = 192;  // >= 63 pending + 128 pushed per iteration
+
+template <int NPL>
+struct StageRing {
+    double* v;      // [NPL][kRing] for this wave
+    uint32_t* idx;  // [kRing]
+    unsigned head = 0, count = 0;  // wave-uniform
+
+    __device__ __forceinline__ void push(bool flag, const double* t, uint32_t i) {
+        const unsigned long long mask = __ballot(flag);
+        if (flag) {
+            unsigned pos = head + count + (unsigned)__popcll(mask & ((1ull << __lane_id()) - 1ull));
+            if (pos >= kRing) pos -= kRing;
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) v[q * kRing + pos] = t[q];
+            idx[pos] = i;
+        }
+        count += (unsigned)__popcll(mask);
+    }
+
+    // Pops min(count, 64) entries, one per lane; returns whether this lane got one.
+    __device__ __forceinline__ bool pop(double* t, uint32_t& i) {
+        wave_sync();
+        const unsigned take = count < 64 ? count : 64;
+        const bool has = __lane_id() < take;
+        if (has) {
+            unsigned pos = head + __lane_id();
+            if (pos >= kRing) pos -= kRing;
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) t[q] = v[q * kRing + pos];
+            i = idx[pos];
+        }
+        wave_sync();
+        head += take;
+        if (head >= kRing) head -= kRing;
+        count -= take;
+        return has;
+    }
+};
+
 // Structure-of-arrays filter: two tuples per thread per iteration, 16-byte
 // non-temporal loads (requires 16-byte aligned planes and an even leading dimension).
+// Every output byte is written exactly once: by this kernel when a filter or the inline
+// exact stage decides it, otherwise by the expansion pass.
 template <int NPTS, int MODE, bool VEC>
 __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restrict__ soa,
                                                          size_t n, size_t ld,
                                                          int8_t* __restrict__ out,
                                                          PredWorkspace ws, unsigned iters) {
     constexpr int NPL = NPTS * 3;
-    unsigned n32 = 0, n64 = 0;
+    constexpr bool STAGE = NPTS == 4;
+    __shared__ double s_ring[STAGE ? 4 : 1][STAGE ? NPL * kRing : 1];
+    __shared__ uint32_t s_idx[STAGE ? 4 : 1][STAGE ? kRing : 1];
+    unsigned n32 = 0, n64 = 0, nx = 0;
     const size_t npairs = (n + 1) / 2;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned wave = (unsigned)(tid / 64);
     uint32_t* seg = ws.queue + (size_t)wave * ws.seg;
     unsigned off = 0;
+    StageRing<NPL> ring;
+    ring.v = &s_ring[STAGE ? threadIdx.x / 64 : 0][0];
+    ring.idx = &s_idx[STAGE ? threadIdx.x / 64 : 0][0];
+    // the inline exact stage over up to 64 ring entries
+    auto drain = [&]() {
+        double t[NPL];
+        uint32_t i = 0;
+        const bool has = ring.pop(t, i);
+        int s = kUndecided;
+        const bool dec = has && decide_inline_exact<NPTS>(t, s);
+        if (dec) out[i] = (int8_t)s;
+        nx += dec;
+        wave_push(has && !dec, i, seg, off);
+    };
     for (unsigned it = 0; it < iters; ++it) {
         const size_t p = tid + (size_t)it * stride;
         const bool valid = p < npairs;
         const size_t i0 = 2 * p;
         const bool two = valid && i0 + 1 < n;
         bool u0 = false, u1 = false;
+        double v0[NPL], v1[NPL];
         if (valid) {
-            double v0[NPL], v1[NPL];
             if (VEC && two) {
 #pragma unroll
                 for (int q = 0; q < NPL; ++q) {
@@ -312,30 +400,40 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                 }
             }
             int st0 = 0, st1 = 0;
-            int s0 = decide<NPTS, MODE>(v0, st0);
-            int s1 = decide<NPTS, MODE>(v1, st1);
+            const int s0 = decide<NPTS, MODE>(v0, st0);
+            const int s1 = decide<NPTS, MODE>(v1, st1);
             u0 = s0 == kUndecided;
             u1 = two && s1 == kUndecided;
-            int8_t o0 = u0 ? 0 : (int8_t)s0, o1 = (u1 || !two) ? 0 : (int8_t)s1;
-            if (two && ((reinterpret_cast<uintptr_t>(out + i0) & 1) == 0)) {
+            const int8_t o0 = (int8_t)s0, o1 = (int8_t)s1;
+            if (two && !u0 && !u1 && ((reinterpret_cast<uintptr_t>(out + i0) & 1) == 0)) {
                 uint16_t packed = (uint16_t)(uint8_t)o0 | ((uint16_t)(uint8_t)o1 << 8);
                 __builtin_nontemporal_store(packed, reinterpret_cast<uint16_t*>(out + i0));
             } else {
-                out[i0] = o0;
-                if (two) out[i0 + 1] = o1;
+                if (!u0) out[i0] = o0;
+                if (two && !u1) out[i0 + 1] = o1;
             }
             n32 += (!u0 && st0 == 1) + (two && !u1 && st1 == 1);
             n64 += (!u0 && st0 == 2) + (two && !u1 && st1 == 2);
         }
-        wave_push(u0, (uint32_t)i0, seg, off);
-        wave_push(u1, (uint32_t)(i0 + 1), seg, off);
+        if (STAGE) {
+            ring.push(u0, v0, (uint32_t)i0);
+            ring.push(u1, v1, (uint32_t)(i0 + 1));
+            while (ring.count >= 64) drain();
+        } else {
+            wave_push(u0, (uint32_t)i0, seg, off);
+            wave_push(u1, (uint32_t)(i0 + 1), seg, off);
+        }
     }
+    if (STAGE)
+        while (ring.count > 0) drain();
     n32 = wave_sum(n32);
     n64 = wave_sum(n64);
+    nx = wave_sum(nx);
     if (lane_id() == 0) {
         ws.wave_q[wave] = off;
         ws.wave_32[wave] = n32;
         ws.wave_64[wave] = n64;
+        ws.wave_x[wave] = nx;
     }
 }
 
@@ -344,7 +442,7 @@ template <int NPTS, int MODE, class Src>
 __global__ __launch_bounds__(256) void filter_kernel(Src src, size_t n,
                                                      int8_t* __restrict__ out,
                                                      PredWorkspace ws, unsigned iters) {
-    unsigned n32 = 0, n64 = 0;
+    unsigned n32 = 0, n64 = 0, nx = 0;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned wave = (unsigned)(tid / 64);
@@ -358,19 +456,23 @@ __global__ __launch_bounds__(256) void filter_kernel(Src src, size_t n,
             src.load(i, v);
             int st = 0;
             int s = decide<NPTS, MODE>(v, st);
+            if (s == kUndecided && decide_inline_exact<NPTS>(v, s)) st = 3;
             u = s == kUndecided;
             out[i] = u ? 0 : (int8_t)s;
             n32 += !u && st == 1;
             n64 += !u && st == 2;
+            nx += !u && st == 3;
         }
         wave_push(u, (uint32_t)i, seg, off);
     }
     n32 = wave_sum(n32);
     n64 = wave_sum(n64);
+    nx = wave_sum(nx);
     if (lane_id() == 0) {
         ws.wave_q[wave] = off;
         ws.wave_32[wave] = n32;
         ws.wave_64[wave] = n64;
+        ws.wave_x[wave] = nx;
     }
 }
 
@@ -389,7 +491,7 @@ __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__
                 uint32_t i = q[k];
                 double v[NPTS * 3];
                 src.load(i, v);
-                int s = NPTS == 4 ? orient3d_exact_sign<40>(v, v + 3, v + 6, v + 9)
+                int s = NPTS == 4 ? orient3d_exact_sign<40, false>(v, v + 3, v + 6, v + 9)
                                   : insphere_exact_sign<40, 96>(v, v + 3, v + 6, v + 9, v + 12);
                 if (s == kOverflow) {
                     unsigned slot = atomicAdd(&ws.ctr->wide_n, 1u);
@@ -406,7 +508,7 @@ __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__
             uint32_t i = ws.wide[k];
             double v[NPTS * 3];
             src.load(i, v);
-            int s = NPTS == 4 ? orient3d_exact_sign<320>(v, v + 3, v + 6, v + 9)
+            int s = NPTS == 4 ? orient3d_exact_sign<320, false>(v, v + 3, v + 6, v + 9)
                               : insphere_exact_sign<200, 1024>(v, v + 3, v + 6, v + 9, v + 12);
             if (s == kOverflow) {
                 atomicAdd(&ws.ctr->failed, 1u);
@@ -440,7 +542,12 @@ __global__ __launch_bounds__(256) void o3_values_kernel(Src src, size_t n,
         }
         wave_push(u, (uint32_t)i, seg, off);
     }
-    if (lane_id() == 0) ws.wave_q[wave] = off;
+    if (lane_id() == 0) {
+        ws.wave_q[wave] = off;
+        ws.wave_32[wave] = 0;
+        ws.wave_64[wave] = 0;
+        ws.wave_x[wave] = 0;
+    }
 }
 
 template <class Src>
@@ -466,22 +573,25 @@ __global__ __launch_bounds__(64) void o3_values_exact_kernel(Src src,
 
 // Sums the per-wave statistics into ctr (only when the caller asks for them).
 __global__ __launch_bounds__(256) void reduce_stats_kernel(PredWorkspace ws) {
-    unsigned long long q = 0, a = 0, b = 0;
+    unsigned long long q = 0, a = 0, b = 0, x = 0;
     for (unsigned w = threadIdx.x; w < ws.nwaves; w += blockDim.x) {
         q += ws.wave_q[w];
         a += ws.wave_32[w];
         b += ws.wave_64[w];
+        x += ws.wave_x[w];
     }
-    __shared__ unsigned long long sq[256], sa[256], sb[256];
+    __shared__ unsigned long long sq[256], sa[256], sb[256], sx[256];
     sq[threadIdx.x] = q;
     sa[threadIdx.x] = a;
     sb[threadIdx.x] = b;
+    sx[threadIdx.x] = x;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             sq[threadIdx.x] += sq[threadIdx.x + o];
             sa[threadIdx.x] += sa[threadIdx.x + o];
             sb[threadIdx.x] += sb[threadIdx.x + o];
+            sx[threadIdx.x] += sx[threadIdx.x + o];
         }
         __syncthreads();
     }
@@ -489,6 +599,7 @@ __global__ __launch_bounds__(256) void reduce_stats_kernel(PredWorkspace ws) {
         ws.ctr->exact = sq[0];
         ws.ctr->fp32 = sa[0];
         ws.ctr->fp64 = sb[0];
+        ws.ctr->inline_ = sx[0];
     }
 }
 
@@ -522,7 +633,7 @@ PredPlan pred_plan(size_t items, unsigned per_thread) {
 
 size_t pred_workspace_bytes(const PredPlan& p, size_t n) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return 256 + 3 * al((size_t)p.nwaves * 4) + al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
+    return 256 + 4 * al((size_t)p.nwaves * 4) + al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
 }
 
 PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
@@ -536,6 +647,8 @@ PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
     ws.wave_32 = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * 4);
     ws.wave_64 = reinterpret_cast<uint32_t*>(c);
+    c += al((size_t)p.nwaves * 4);
+    ws.wave_x = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * 4);
     ws.queue = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * p.seg * 4);

@@ -135,9 +135,12 @@ typedef struct cmg_pred_stats {
     uint64_t lanes;        /* tuples in the batch */
     uint64_t fp32;         /* decided by the FP32 filter (exact mode only) */
     uint64_t fp64;         /* decided by the FP64 stage */
-    uint64_t exact;        /* resolved by expansion arithmetic */
+    uint64_t exact;        /* left undecided by the filters, resolved exactly */
     uint64_t wide;         /* of those, re-run with wide expansion capacity */
     uint64_t failed;       /* exceeded even the wide capacity (sign left 0) */
+    uint64_t exact_inline; /* of `exact`, decided inside the streaming kernel (exact
+                              difference sums or double-word bound); the rest went
+                              through expansion arithmetic */
 } cmg_pred_stats;
 
 /* Workspace bytes a batched-sign call over n tuples needs (npts 4 = orient3d,

@@ -49,6 +49,7 @@ def test_full_size_orient3d_batch_and_stage_fractions(gpu):
     assert st["fp32"] + st["fp64"] + st["exact"] == n
     assert st["failed"] == 0
     assert st["exact"] >= int((kind == 1).sum())     # exact zeros need the exact stage
+    assert 0 < st["exact_inline"] <= st["exact"]
     gotc = gpu.signs_soa(d, "compat").cpu().numpy()
     assert (gotc == oracle.orient3d_batch(soa, oracle.COMPAT)).all()
 
