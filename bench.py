@@ -154,6 +154,29 @@ def cpu_sdf_sample(W, budget_s=20.0):
                       f"{t_build:.2f} s excluded ({cpu_model()})"}
 
 
+# FP32 operations per lane of the voxel passes' three kinds of work (FMA = 2): a cylinder
+# (or disk) lower bound ~30, a point-triangle distance ~80.
+FLOP_CYL, FLOP_TRI = 30, 80
+FP32_PEAK_TFLOPS = 157.3
+
+
+def voxel_work(counters, cells, vpl, kernel_ms):
+    """Work of the forward + backward voxel passes from the engine's counters (each
+    counted once per wavefront of 64 lanes x vpl voxels) and the FP32 rate it implies."""
+    lanes = 64 * vpl
+    cyl = (counters["node_tests"] + counters["tri_tests"]) * lanes
+    tri = counters["tri_evals"] * lanes
+    flops = cyl * FLOP_CYL + tri * FLOP_TRI
+    out = {"cylinder_tests_per_voxel": cyl / (2 * cells),
+           "triangle_distances_per_voxel": tri / (2 * cells),
+           "counted_fp32_gflop_per_step": flops / 1e9}
+    if kernel_ms > 0:
+        tf = flops / (kernel_ms / 1e3) / 1e12
+        out["voxel_kernels_fp32_tflops"] = tf
+        out["voxel_kernels_fp32_frac_of_peak"] = tf / FP32_PEAK_TFLOPS
+    return out
+
+
 def bench_sdf(cm, W, steps, warmup):
     """C5: 64 uneven meshes, SDF forward+backward per step, batched (one graph of
     persistent kernels) vs plain (one mesh at a time, synchronised per mesh, same kernels).
@@ -174,13 +197,15 @@ def bench_sdf(cm, W, steps, warmup):
     sdf = torch.empty(cells, dtype=torch.float32, device="cuda")
     grad = torch.empty((batch.points.shape[0], 3), dtype=torch.float32, device="cuda")
 
+    log("[bench] autotuning the voxel passes")
+    tune = batch.autotune(gs, sdf=sdf, grad=grad)
+
     def bstep():
         batch.step(gs, sdf=sdf, grad=grad, graph=True)
 
     reps = max(1, steps // 4)
     ms_b = time_steps(bstep, reps, max(1, warmup), torch.cuda.synchronize)
-    st = batch.status()
-    pairs_per_step = st["pairs"]
+    st = batch.counters()
     # dominant kernel: the forward voxel pass
     cm.kernel_timer("batch_sdf", True)
     cm.kernel_timer("batch_sdf_backward", True)
@@ -193,6 +218,8 @@ def bench_sdf(cm, W, steps, warmup):
     # plain: one engine per mesh, one mesh at a time
     log("[bench] planning 64 single-mesh engines")
     singles = [SdfBatch([m], [r], [p]) for m, r, p in zip(meshes, res, pads)]
+    for b in singles:  # the same voxel-pass variant as the batch
+        b.set_variant(tune["vpl"], tune["depth"])
     sg = [gs[bg.offset:bg.offset + bg.ncell].contiguous() for bg in batch.grids]
     souts = [(torch.empty(b.cells, dtype=torch.float32, device="cuda"),
               torch.empty((b.points.shape[0], 3), dtype=torch.float32, device="cuda"))
@@ -215,14 +242,10 @@ def bench_sdf(cm, W, steps, warmup):
            "plain_voxels_per_s": cells / (ms_p / 1e3),
            "speedup_batched_vs_plain": ms_p / ms_b,
            "batched_equals_plain": same, "plan_s": plan_s,
-           "fp32_pairs_per_step": pairs_per_step,
-           "pairs_per_voxel": pairs_per_step / (2 * cells),
            "fwd_voxel_kernel_ms": fwd_ms, "bwd_voxel_kernel_ms": bwd_ms,
+           "variant": batch.variant_info(), "autotune_ms": tune["ms"],
            "overflow": st["overflow"]}
-    # fp32 flop rate of the voxel kernels: 48 flops per evaluated pair, +20 per voxel in bwd
-    kern_s = (fwd_ms + bwd_ms) / 1e3
-    if kern_s > 0:
-        out["voxel_kernels_fp32_tflops"] = (pairs_per_step * 48 + cells * 20) / kern_s / 1e12
+    out.update(voxel_work(st, cells, batch.vpl(), fwd_ms + bwd_ms))
     batch.close()
     # single-mesh C3 and C4 forward
     for name, case in (("c3_torus_256", W.torus_c3(seed=3)), ("c4_harmonic_128", W.harmonic_c4(seed=4))):
@@ -236,7 +259,7 @@ def bench_sdf(cm, W, steps, warmup):
                          torch.cuda.synchronize)
         out[name] = {"cells": b.cells, "fwd_ms": ms, "fwd_voxels_per_s": b.cells / (ms / 1e3),
                      "fwd_bwd_ms": ms2, "fwd_bwd_voxels_per_s": b.cells / (ms2 / 1e3),
-                     "status": b.status()}
+                     "counters": b.counters()}
         b.close()
     return out
 

@@ -47,6 +47,10 @@ def _bind():
     L.cmg_voxelize_batch_set_variant.argtypes = [_P, _i, _i]
     L.cmg_voxelize_batch_occupancy.restype = _P
     L.cmg_voxelize_batch_occupancy.argtypes = [_P]
+    L.cmg_voxelize_batch_nearest.restype = _i
+    L.cmg_voxelize_batch_nearest.argtypes = [_P, _P, _P, ctypes.c_char_p, _sz]
+    L.cmg_voxelize_batch_counters.restype = _i
+    L.cmg_voxelize_batch_counters.argtypes = [_P, _P, _P, ctypes.c_char_p, _sz]
     _bound = True
 
 
@@ -168,7 +172,7 @@ class SdfBatch:
         return sdf, grad
 
     def autotune(self, grad_sdf, sdf=None, grad=None, stream=None) -> dict:
-        """Times the six voxel-pass variants on this batch and keeps the fastest; returns
+        """Times the four voxel-pass variants on this batch and keeps the fastest; returns
         {"vpl": .., "depth": .., "ms": {(vpl, depth): ms}}."""
         import torch
         if sdf is None:
@@ -177,18 +181,28 @@ class SdfBatch:
             grad = torch.empty((self.points.shape[0], 3), dtype=torch.float32,
                                device=self.device)
         vpl, depth = _i(), _i()
-        ms = np.zeros(6, dtype=np.float32)
+        ms = np.zeros(4, dtype=np.float32)
         err = _err()
         _check(lib().cmg_voxelize_batch_autotune(
             self._h, sdf.data_ptr(), grad_sdf.contiguous().data_ptr(), grad.data_ptr(),
             _stream(stream), ctypes.byref(vpl), ctypes.byref(depth), ms.ctypes.data_as(_P),
             err, len(err)), err)
-        keys = [(1, 0), (1, 32), (1, 64), (2, 0), (2, 32), (2, 64)]
+        keys = [(1, 0), (1, 8), (2, 0), (2, 8)]
+        self._variant = (vpl.value, depth.value)
         return {"vpl": vpl.value, "depth": depth.value,
                 "ms": {f"{v}x{d}": float(t) for (v, d), t in zip(keys, ms)}}
 
     def set_variant(self, vpl: int, depth: int):
         _check(lib().cmg_voxelize_batch_set_variant(self._h, vpl, depth), None)
+        self._variant = (vpl, depth)
+
+    def variant_info(self) -> dict:
+        """The voxel-pass variant in use: voxels per lane and LDS tile depth."""
+        v, d = getattr(self, "_variant", (1, 0))
+        return {"vpl": v, "depth": d}
+
+    def vpl(self) -> int:
+        return getattr(self, "_variant", (1, 0))[0]
 
     def status(self, stream=None) -> dict:
         ov = ctypes.c_uint()
@@ -197,6 +211,26 @@ class SdfBatch:
         _check(lib().cmg_voxelize_batch_status(self._h, _stream(stream), ctypes.byref(ov),
                                                ctypes.byref(pairs), err, len(err)), err)
         return {"overflow": int(ov.value), "pairs": int(pairs.value)}
+
+    def counters(self, stream=None) -> dict:
+        """Work counters of the last voxel passes (per wavefront): node cylinder tests,
+        triangle disk tests, FP32 triangle distances; and the overflow flags."""
+        out = np.zeros(4, dtype=np.uint64)
+        err = _err()
+        _check(lib().cmg_voxelize_batch_counters(self._h, _stream(stream),
+                                                 out.ctypes.data_as(_P), err, len(err)), err)
+        return {"node_tests": int(out[0]), "tri_tests": int(out[1]),
+                "tri_evals": int(out[2]), "overflow": int(out[3])}
+
+    def nearest(self, stream=None):
+        """After a forward: per cell (device int32, sdf layout) the mesh-local index of
+        the triangle the adjoint uses (-1 where none)."""
+        import torch
+        out = torch.empty(self.cells, dtype=torch.int32, device=self.device)
+        err = _err()
+        _check(lib().cmg_voxelize_batch_nearest(self._h, out.data_ptr(), _stream(stream),
+                                                err, len(err)), err)
+        return out
 
     def split(self, flat):
         """Per-mesh (nx, ny, nz) views of a concatenated cell array."""

@@ -1,26 +1,38 @@
-// batch.hip — voxelization engine for many meshes of uneven size in one pass.
+// batch.hip — voxelization engine for one or many meshes of uneven size in one pass.
 //
 // All meshes live in one set of device arrays (points, triangles, cells concatenated).
-// A step is a fixed sequence of persistent kernels; each kernel's workgroups pull work
-// items — (mesh, tile) or (mesh, brick) — from a device work queue (one atomic counter
-// per kernel), with the items of the most expensive meshes first, so that small meshes
-// fill the gaps left by large ones instead of each mesh occupying the GPU alone.
-// Brick and tile lists are allocated from device pools by atomic bump allocation, so a
-// step needs no host round trip; the sequence is captured once into a hipGraph and
-// replayed. Pool capacities are sized by a planning pass at creation; a step that would
-// exceed them sets an overflow flag instead of writing out of bounds.
+// A step is a fixed sequence of kernels; the persistent ones pull work items — (mesh,
+// column tile) or (mesh, voxel leaf) — from a device work queue (one atomic counter per
+// kernel), most expensive meshes first, so small meshes fill the gaps left by large ones
+// instead of each occupying the GPU alone. The sequence is captured once into a hipGraph
+// and replayed; a step needs no host round trip.
+//
+// Nearest-triangle search. At creation every mesh's triangles are put in Morton order of
+// their centroids. A level-l tree node covers 8^l consecutive triangles (an implicit
+// 8-ary tree, no pointers) and carries a bounding cylinder: centre, axis (aligned area-
+// weighted normal), lateral radius, half-height. For a surface patch seen from afar the
+// cylinder is a thin disk, and the distance to it bounds the distance to the patch far
+// more tightly than a sphere or a box would; every triangle likewise has a bounding disk.
+// Each step rebuilds the cylinders bottom-up from the current vertices. A wavefront owns
+// 64 or 128 neighbouring voxels and walks the tree once for all of them: a node is opened
+// when any voxel's lower bound beats that voxel's best distance so far, the child nearest
+// the leaf centre first (a wave-uniform stack in LDS). The minimum of the FP32 distance
+// over the triangles that survive is the minimum over all triangles (every rejection is a
+// rigorous lower bound), so results do not depend on the tree. Voxels within one cell of
+// the surface walk again with the bound fixed at the FP32 minimum plus twice the FP32
+// error margin and take the reference's FP64 distance over what survives.
 //
 // Phases of a forward step (all meshes at once):
-//   reset        zero the queues, pool cursors and tile counters (a fill kernel: no
+//   reset        zero the queues, pool cursor and tile counters (a fill kernel: no
 //                memset nodes in the graph)
-//   prep         per triangle: FP32/FP64 records, xy bucket range, tile counts
+//   prep         per tree slot: FP32/FP64 triangle records; per triangle: xy bucket
+//                range and tile counts
 //   tile alloc   per tile: list offset from the tile pool
 //   tile fill    per triangle: append to the lists of the tiles its range touches
 //   occupancy    per (mesh, 8x8 column tile): exact column parity, as voxelize.hip
-//   cull         per (mesh, 16^3 brick) from all the mesh's triangles, then per
-//                (mesh, 8^3 brick) from the parent list: min, count+alloc, write
-//   voxels       per (mesh, 4^3 brick): FP32 distance, FP64 near the surface, signed
-// A backward step reuses the forward's lists and occupancy and runs the adjoint voxel
+//   tree         level 1 from the triangles, then each level from the one below
+//   voxels       per (mesh, voxel leaf): FP32 nearest, FP64 near the surface, signed
+// A backward step reuses the forward's occupancy and tree and runs the adjoint voxel
 // pass (closest features recomputed) after zeroing the gradient.
 //
 // Compiled with -ffp-contract=off like the single-mesh kernels it shares code with.
@@ -28,6 +40,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -45,11 +58,23 @@ using namespace occdev;
 
 namespace {
 
+constexpr int kStack = 96;  // >= 7 * kMaxLevels + 1 entries of the traversal stack
+
 // ---- work queue -----------------------------------------------------------------------
 
 // Global item -> position in processing order (binary search over the phase prefix).
 __device__ __forceinline__ int find_pos(const long long* off, int nm, long long item) {
     int lo = 0, hi = nm;  // off[lo] <= item < off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int find_int(const int* off, int nm, int item) {
+    int lo = 0, hi = nm;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (off[mid] <= item) lo = mid;
@@ -67,19 +92,29 @@ __device__ __forceinline__ long long next_item(unsigned long long* q) {
     return s_item;
 }
 
+// Orders LDS traffic between lanes of one wavefront (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct Ctx {  // kernel arguments shared by every phase
     const MeshDesc* mesh;
     int nm;
     const int* order;           // processing order -> mesh id
     const long long* item_off;  // [kNumPhase][nm + 1] prefix of items in processing order
     const long long* tri_start; // [nm + 1] first global triangle per mesh id
+    const int* lvl_off;         // [kMaxLevels + 1][nm + 1] prefix of nodes per level
     long long total_tris;
     const double* xyz;
     const int* tris;            // local point indices of each mesh
+    const int* perm;            // tree slot -> global triangle
     BatchCounters* ctr;
-    TriRec* rec;
+    TriRec* rec;                // per tree slot
     TriRec64* rec64;
-    int4* xyr;
+    NodeRec* node;
+    int4* xyr;                  // per triangle
     int* tile_count;
     int* tile_off;
     int* tile_cursor;
@@ -87,19 +122,10 @@ struct Ctx {  // kernel arguments shared by every phase
     long long tile_cap;
     unsigned long long* masks;
     uint8_t* occ;
-    float* dmin2;
-    int* cnt2;
-    long long* off2;
-    int* pool2;
-    long long cap2;
-    float* dmin1;
-    int* cnt1;
-    long long* off1;
-    int* pool1;
-    long long cap1;
     float* sdf;
     const float* gsdf;
     float* grad;
+    int* nearest;
     int total_tiles;
 };
 
@@ -117,9 +143,24 @@ __device__ __forceinline__ int mesh_of_tri(const Ctx& C, long long t) {
     return lo;
 }
 
+typedef const __attribute__((address_space(4))) float* ConstNodePtr;
+
+__device__ __forceinline__ NodeRec load_node(const NodeRec* base, int i) {
+    ConstNodePtr p = (ConstNodePtr)(base + i);
+    NodeRec r;
+    float* d = reinterpret_cast<float*>(&r);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d[k] = p[k];
+    return r;
+}
+
+__device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, float pz) {
+    return cyl_lb2(N.cx, N.cy, N.cz, N.nx, N.ny, N.nz, N.r, N.t, px, py, pz);
+}
+
 // ---- phases ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void reset_kernel(Ctx C, int zero_grad, long long npts3) {
+__global__ __launch_bounds__(256) void reset_kernel(Ctx C) {
     const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long stride = (long long)gridDim.x * blockDim.x;
     if (tid == 0) *C.ctr = BatchCounters{};
@@ -127,14 +168,12 @@ __global__ __launch_bounds__(256) void reset_kernel(Ctx C, int zero_grad, long l
         C.tile_count[i] = 0;
         C.tile_cursor[i] = 0;
     }
-    if (zero_grad)
-        for (long long i = tid; i < npts3; i += stride) C.grad[i] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n,
                                                         BatchCounters* ctr) {
     const long long stride = (long long)gridDim.x * blockDim.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->queue[8] = 0;  // backward queue
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->queue[2] = 0;  // backward queue
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         g[i] = 0.0f;
 }
@@ -146,20 +185,50 @@ __global__ void reset_queue_kernel(BatchCounters* ctr, int a, int b) {
     }
 }
 
-__global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
+// Morton code (10 bits per axis) of each triangle's centroid within its mesh's grid.
+__device__ __forceinline__ unsigned spread10(unsigned x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(Ctx C, unsigned* code) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.total_tris;
          t += stride) {
         const MeshDesc& M = C.mesh[mesh_of_tri(C, t)];
         const double* xyz = C.xyz + 3 * M.vbase;
+        const double ext = fmax(fmax(M.g.nx, M.g.ny), M.g.nz) * M.g.spacing;
+        unsigned q[3];
+        for (int a = 0; a < 3; ++a) {
+            double c = 0.0;
+            for (int v = 0; v < 3; ++v) c += xyz[3 * (size_t)C.tris[3 * t + v] + a];
+            const double u = (c / 3.0 - M.g.corner[a]) / ext;
+            q[a] = (unsigned)fmin(fmax(u * 1024.0, 0.0), 1023.0);
+        }
+        code[t] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    }
+}
+
+// Per tree slot: the triangle records. Per triangle: the xy bucket range and tile counts.
+__global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < C.total_tris;
+         s += stride) {
+        const MeshDesc& M = C.mesh[mesh_of_tri(C, s)];
+        const double* xyz = C.xyz + 3 * M.vbase;
+        const int t = C.perm[s];
         double P[3][3];
         for (int v = 0; v < 3; ++v)
-            for (int a = 0; a < 3; ++a) P[v][a] = xyz[3 * (size_t)C.tris[3 * t + v] + a];
+            for (int a = 0; a < 3; ++a) P[v][a] = xyz[3 * (size_t)C.tris[3 * (size_t)t + v] + a];
         TriRec r;
         TriRec64 r64;
         prep_triangle(P, M.g, r, r64);
-        C.rec[t] = r;
-        C.rec64[t] = r64;
+        C.rec[s] = r;
+        C.rec64[s] = r64;
         const int4 x = xy_range(P, M.g);
         C.xyr[t] = x;
         for (int tj = x.z / kTile; tj <= x.w / kTile; ++tj)
@@ -303,117 +372,161 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
     }
 }
 
-// Brick lists. LEVEL 2: 16^3 bricks, candidates = the mesh's triangles; LEVEL 1: 8^3
-// bricks, candidates = the parent 16^3 brick's list. PASS 0: min centre distance;
-// PASS 1: survivors counted and their list allocated from the pool; PASS 2: written.
-template <int LEVEL, int PASS>
-__global__ __launch_bounds__(256) void cull_kernel(Ctx C) {
-    __shared__ float sred[4];
-    __shared__ int scount[4];
-    __shared__ long long sbase;
-    const int ph = LEVEL == 2 ? kPhL2 : kPhL1;
-    const long long* off = phase_off(C, ph);
-    const long long total = off[C.nm];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    unsigned long long* q = &C.ctr->queue[1 + (LEVEL == 2 ? 0 : 3) + PASS];
-    float* dmin = LEVEL == 2 ? C.dmin2 : C.dmin1;
-    int* cnt = LEVEL == 2 ? C.cnt2 : C.cnt1;
-    long long* loff = LEVEL == 2 ? C.off2 : C.off1;
-    int* pool = LEVEL == 2 ? C.pool2 : C.pool1;
-    const long long cap = LEVEL == 2 ? C.cap2 : C.cap1;
-    while (true) {
-        const long long item = next_item(q);
-        if (item >= total) break;
-        const int pos = find_pos(off, C.nm, item);
-        const MeshDesc& M = C.mesh[C.order[pos]];
-        const int b = (int)(item - off[pos]);
-        const Level& L = LEVEL == 2 ? M.L2 : M.L1;
-        const int gb = (LEVEL == 2 ? M.l2_base : M.l1_base) + b;
-        float cx, cy, cz, hd;
-        brick_geom(M.g, L, b, cx, cy, cz, hd);
-        long long beg, end;
-        const int* cand = nullptr;  // nullptr: candidates are triangles tbase + k
-        if (LEVEL == 2) {
-            beg = M.tbase;
-            end = M.tbase + M.ntris;
+// ---- tree of bounding cylinders -----------------------------------------------------------
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Rounds a bounding cylinder computed in FP64 to float, outward: the radius and the
+// half-height grow by the rounding of the centre and of the axis (a few ulps of the
+// coordinates) so that the float cylinder still contains the geometry.
+__device__ inline NodeRec round_node(const double* c, const double* n, double r, double t) {
+    const double ext = fmax(fmax(fabs(c[0]), fabs(c[1])), fabs(c[2])) + r + t;
+    const double slack = 8.0 * 0x1p-24 * ext;
+    NodeRec N;
+    N.cx = (float)c[0];
+    N.cy = (float)c[1];
+    N.cz = (float)c[2];
+    N.nx = (float)n[0];
+    N.ny = (float)n[1];
+    N.nz = (float)n[2];
+    N.r = (float)(r * (1.0 + 1e-6) + slack);
+    N.t = (float)(t * (1.0 + 1e-6) + slack);
+    return N;
+}
+
+// Level 1: the cylinder of up to 8 consecutive triangles, from their FP64 vertices.
+__global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
+    const int* lo = C.lvl_off + (C.nm + 1);
+    const int stride = gridDim.x * blockDim.x;
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < total; gi += stride) {
+        const int m = find_int(lo, C.nm, gi);
+        const MeshDesc& M = C.mesh[m];
+        const int loc = gi - lo[m];
+        const long long s0 = M.tbase + 8LL * loc;
+        const int cnt = min(8, M.ntris - 8 * loc);
+        double N[3] = {0.0, 0.0, 0.0}, ref[3] = {0.0, 0.0, 0.0};
+        double bl[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bh[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        bool have_ref = false;
+        for (int k = 0; k < cnt; ++k) {
+            const TriRec64& T = C.rec64[s0 + k];
+            double e1[3], e2[3], nt[3];
+            for (int a = 0; a < 3; ++a) {
+                e1[a] = T.b[a] - T.a[a];
+                e2[a] = T.c[a] - T.a[a];
+                bl[a] = fmin(bl[a], fmin(T.a[a], fmin(T.b[a], T.c[a])) - M.g.corner[a]);
+                bh[a] = fmax(bh[a], fmax(T.a[a], fmax(T.b[a], T.c[a])) - M.g.corner[a]);
+            }
+            cross3(e1, e2, nt);
+            if (!have_ref && dot3(nt, nt) > 0.0) {
+                for (int a = 0; a < 3; ++a) ref[a] = nt[a];
+                have_ref = true;
+            }
+            const double sg = dot3(nt, ref) < 0.0 ? -1.0 : 1.0;
+            for (int a = 0; a < 3; ++a) N[a] += sg * nt[a];
+        }
+        double c[3], n[3];
+        for (int a = 0; a < 3; ++a) c[a] = 0.5 * (bl[a] + bh[a]);
+        const double nn = sqrt(dot3(N, N));
+        if (nn > 0.0) {
+            for (int a = 0; a < 3; ++a) n[a] = N[a] / nn;
         } else {
-            const int ix = b % L.bx, iy = (b / L.bx) % L.by, iz = b / (L.bx * L.by);
-            const int pb = M.l2_base + ((iz >> 1) * M.L2.by + (iy >> 1)) * M.L2.bx + (ix >> 1);
-            beg = C.off2[pb];
-            end = beg + C.cnt2[pb];
-            if (end > C.cap2) end = beg;  // parent list overflowed (flag already set)
-            cand = C.pool2;
+            n[0] = n[1] = 0.0;
+            n[2] = 1.0;
         }
-        if (PASS == 0) {
-            float m = FLT_MAX;
-            for (long long k = beg + threadIdx.x; k < end; k += blockDim.x)
-                m = fminf(m, tri_d2(C.rec[cand ? cand[k] : k], cx, cy, cz));
-            m = wave_min(m);
-            if (lane == 0) sred[wave] = m;
-            __syncthreads();
-            if (threadIdx.x == 0)
-                dmin[gb] = sqrtf(fminf(fminf(sred[0], sred[1]), fminf(sred[2], sred[3])));
-            continue;
-        }
-        const float thr = dmin[gb] + 2.0f * hd + M.eps;
-        const float thr2 = thr * thr;
-        if (PASS == 1) {
-            int c = 0;
-            for (long long k = beg + threadIdx.x; k < end; k += blockDim.x)
-                c += tri_d2(C.rec[cand ? cand[k] : k], cx, cy, cz) <= thr2;
-            for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-            if (lane == 0) scount[wave] = c;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                const int total_c = scount[0] + scount[1] + scount[2] + scount[3];
-                const long long at =
-                    (long long)atomicAdd(LEVEL == 2 ? &C.ctr->l2_used : &C.ctr->l1_used,
-                                         (unsigned long long)total_c);
-                cnt[gb] = total_c;
-                loff[gb] = at;
-                if (at + total_c > cap) atomicOr(&C.ctr->overflow, LEVEL == 2 ? 2u : 4u);
+        double r = 0.0, t = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const TriRec64& T = C.rec64[s0 + k];
+            for (int v = 0; v < 3; ++v) {
+                const double* P = v == 0 ? T.a : (v == 1 ? T.b : T.c);
+                double d[3], w[3];
+                for (int a = 0; a < 3; ++a) d[a] = P[a] - M.g.corner[a] - c[a];
+                cross3(n, d, w);
+                t = fmax(t, fabs(dot3(n, d)));
+                r = fmax(r, sqrt(dot3(w, w)));
             }
-            continue;
         }
-        // PASS 2
-        if (threadIdx.x == 0) sbase = loff[gb];
-        __syncthreads();
-        const bool fits = loff[gb] + cnt[gb] <= cap;
-        for (long long k0 = beg; k0 < end; k0 += blockDim.x) {
-            const long long k = k0 + threadIdx.x;
-            int t = -1;
-            bool keep = false;
-            if (k < end) {
-                t = cand ? cand[k] : (int)k;
-                keep = tri_d2(C.rec[t], cx, cy, cz) <= thr2;
-            }
-            const unsigned long long mask = __ballot(keep);
-            if (lane == 0) scount[wave] = __popcll(mask);
-            __syncthreads();
-            int before = 0;
-            for (int w = 0; w < wave; ++w) before += scount[w];
-            const long long base = sbase;
-            if (keep && fits) pool[base + before + __popcll(mask & ((1ull << lane) - 1ull))] = t;
-            __syncthreads();
-            if (threadIdx.x == 0) sbase = base + scount[0] + scount[1] + scount[2] + scount[3];
-            __syncthreads();
-        }
+        C.node[M.node_base[1] + loc] = round_node(c, n, r, t);
     }
 }
 
-// One brick per item: 4x4x4 voxels (VPL = 1 voxel per lane) or 4x4x8 (VPL = 2, the
-// second voxel of a lane 4 cells above the first). D = 0 reads each candidate's record
-// with scalar loads (SGPR operands); D > 0 stages D records at a time through LDS.
-// Same algorithm as sdf.hip's voxel kernel otherwise.
+// Level l >= 2: a cylinder around up to 8 child cylinders. For a child (centre c_k, axis
+// n_k, radius r_k, half-height t_k) and the new axis n through c, with cos = |n.n_k|,
+// sin = |n x n_k|, d = c_k - c: every child point lies within |n.d| + t_k cos + r_k sin
+// of the new mid-plane and within |n x d| + t_k sin + r_k of the new axis.
+__global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int total) {
+    const int* lo = C.lvl_off + (size_t)level * (C.nm + 1);
+    const int stride = gridDim.x * blockDim.x;
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < total; gi += stride) {
+        const int m = find_int(lo, C.nm, gi);
+        const MeshDesc& M = C.mesh[m];
+        const int loc = gi - lo[m];
+        const int first = 8 * loc;
+        const int cnt = min(8, M.nnodes[level - 1] - first);
+        const NodeRec* ch = C.node + M.node_base[level - 1] + first;
+        double N[3] = {0.0, 0.0, 0.0}, ref[3] = {(double)ch[0].nx, (double)ch[0].ny,
+                                                  (double)ch[0].nz};
+        double bl[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bh[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int k = 0; k < cnt; ++k) {
+            const NodeRec K = ch[k];
+            const double nk[3] = {K.nx, K.ny, K.nz}, ck[3] = {K.cx, K.cy, K.cz};
+            const double w = (double)K.r * K.r + (double)K.t * K.t;
+            const double sg = dot3(nk, ref) < 0.0 ? -1.0 : 1.0;
+            const double ext = (double)K.r + K.t;
+            for (int a = 0; a < 3; ++a) {
+                N[a] += sg * w * nk[a];
+                bl[a] = fmin(bl[a], ck[a] - ext);
+                bh[a] = fmax(bh[a], ck[a] + ext);
+            }
+        }
+        double c[3], n[3];
+        for (int a = 0; a < 3; ++a) c[a] = 0.5 * (bl[a] + bh[a]);
+        const double nn = sqrt(dot3(N, N));
+        if (nn > 0.0) {
+            for (int a = 0; a < 3; ++a) n[a] = N[a] / nn;
+        } else {
+            n[0] = n[1] = 0.0;
+            n[2] = 1.0;
+        }
+        double r = 0.0, t = 0.0;
+        for (int k = 0; k < cnt; ++k) {
+            const NodeRec K = ch[k];
+            const double nk[3] = {K.nx, K.ny, K.nz};
+            double d[3] = {K.cx - c[0], K.cy - c[1], K.cz - c[2]}, w[3], x[3];
+            cross3(n, nk, x);
+            const double sn = fmin(1.0, sqrt(dot3(x, x)));
+            const double cs = fmin(1.0, fabs(dot3(n, nk)));
+            cross3(n, d, w);
+            t = fmax(t, fabs(dot3(n, d)) + K.t * cs + K.r * sn);
+            r = fmax(r, sqrt(dot3(w, w)) + K.t * sn + K.r);
+        }
+        C.node[M.node_base[level] + loc] = round_node(c, n, r, t);
+    }
+}
+
+// ---- voxel pass ---------------------------------------------------------------------------
+
+// One voxel leaf per item: 4x4x4 voxels (VPL = 1 voxel per lane) or 4x4x8 (VPL = 2, the
+// second voxel of a lane 4 cells above the first). D = 0 reads a tree leaf's triangle
+// records with scalar loads (SGPR operands); D = 8 stages the leaf's 8 records through LDS
+// as one 1 KiB tile. BWD: the adjoint (and, when C.nearest is set, the nearest triangle
+// of every cell) instead of the distance.
 template <bool BWD, int VPL, int D>
 __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
-    __shared__ int slist[kLdsList];
-    __shared__ TriRec srec[D > 0 ? D : 1];
+    __shared__ int s_stack[kStack];
+    __shared__ TriRec s_rec[D > 0 ? D : 1];
     const long long* off = phase_off(C, VPL == 1 ? kPhL0 : kPhL0b);
     const long long total = off[C.nm];
     const int lane = threadIdx.x;
-    unsigned long long* q = &C.ctr->queue[BWD ? 8 : 7];
-    unsigned long long evals = 0;
+    unsigned long long* q = &C.ctr->queue[BWD ? 2 : 1];
+    unsigned long long nvis = 0, ntest = 0, neval = 0;
     while (true) {
         const long long item = next_item(q);
         if (item >= total) break;
@@ -435,117 +548,186 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             pz[v] = (k[v] + 0.5f) * h;
         }
         const float px = (i + 0.5f) * h, py = (j + 0.5f) * h;
-        float cx, cy, cz, hd;
-        brick_geom(g, L0, b, cx, cy, cz, hd);
-        const int pb = M.l1_base +
-                       ((VPL == 1 ? iz >> 1 : iz) * M.L1.by + (iy >> 1)) * M.L1.bx + (ix >> 1);
-        const long long beg = C.off1[pb];
-        long long end = beg + C.cnt1[pb];
-        if (end > C.cap1) end = beg;  // overflowed (flag already set)
-        float m = FLT_MAX;
-        for (long long qq = beg + lane; qq < end; qq += 64)
-            m = fminf(m, tri_d2(C.rec[C.pool1[qq]], cx, cy, cz));
-        const float dmin = sqrtf(wave_min(m));
-        const float thr = dmin + 2.0f * hd + M.eps;
-        const float thr2 = thr * thr;
-        int n = 0;
-        for (long long q0 = beg; q0 < end; q0 += 64) {
-            const long long qq = q0 + lane;
-            int t = -1;
-            bool keep = false;
-            if (qq < end) {
-                t = C.pool1[qq];
-                keep = tri_d2(C.rec[t], cx, cy, cz) <= thr2;
+        const float eps = M.eps;
+
+        // Walks the tree for the lanes in act[] with squared bounds b2[] (which on_leaf may
+        // tighten); on_leaf(loc) visits the triangles of level-1 node loc.
+        auto walk = [&](const bool* act, const float* b2, auto&& on_leaf) {
+            int sp = 0;
+            if (lane == 0) s_stack[0] = M.nlev << 27;
+            sp = 1;
+            while (sp > 0) {
+                wave_sync();
+                const int ref = uniform_int(s_stack[sp - 1]);
+                --sp;
+                const int lev = ref >> 27, loc = ref & 0x7ffffff;
+                ++nvis;
+                {  // the bounds may have tightened since this node was pushed
+                    const NodeRec N = load_node(C.node, M.node_base[lev] + loc);
+                    bool need = false;
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v)
+                        need |= act[v] && node_lb2(N, px, py, pz[v]) <= b2[v];
+                    if (__ballot(need) == 0) continue;
+                }
+                if (lev == 1) {
+                    on_leaf(loc);
+                    continue;
+                }
+                const int first = 8 * loc;
+                const int cnt = min(8, M.nnodes[lev - 1] - first);
+                unsigned mask = 0;
+                int near_c = -1;
+                float near_k = FLT_MAX;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    if (c < cnt) {
+                        ++nvis;
+                        const NodeRec K = load_node(C.node, M.node_base[lev - 1] + first + c);
+                        bool need = false;
+                        float lb0 = 0.0f;
+#pragma unroll
+                        for (int v = 0; v < VPL; ++v) {
+                            const float lb = node_lb2(K, px, py, pz[v]);
+                            need |= act[v] && lb <= b2[v];
+                            if (v == 0) lb0 = lb;
+                        }
+                        if (__ballot(need) != 0) {
+                            mask |= 1u << c;
+                            // order key: the bound seen from a voxel near the leaf centre
+                            const float key = __int_as_float(
+                                __builtin_amdgcn_readlane(__float_as_int(lb0), 21));
+                            if (key < near_k) {
+                                near_k = key;
+                                near_c = c;
+                            }
+                        }
+                    }
+                }
+                const int child = (lev - 1) << 27;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    if (((mask >> c) & 1u) && c != near_c) {
+                        if (lane == 0) s_stack[sp] = child | (first + c);
+                        ++sp;
+                    }
+                if (near_c >= 0) {
+                    if (lane == 0) s_stack[sp] = child | (first + near_c);
+                    ++sp;
+                }
             }
-            const unsigned long long mask = __ballot(keep);
-            if (keep) {
-                const int p = n + __popcll(mask & ((1ull << lane) - 1ull));
-                if (p < kLdsList) slist[p] = t;
+        };
+
+        // triangles of one tree leaf, through LDS (D > 0) or scalar loads
+        auto leaf_rec = [&](int loc, int& cnt) -> long long {
+            const long long s0 = M.tbase + 8LL * loc;
+            cnt = min(8, M.ntris - 8 * loc);
+            if (D > 0) {
+                wave_sync();
+                if (lane < cnt * 8)
+                    reinterpret_cast<float4*>(s_rec)[lane] =
+                        reinterpret_cast<const float4*>(C.rec + s0)[lane];
+                wave_sync();
             }
-            n += __popcll(mask);
-        }
-        __syncthreads();
-        const bool spill = n > kLdsList;
-        const int count = spill ? (int)(end - beg) : n;
-        auto list_at = [&](int qq) { return spill ? C.pool1[beg + qq] : slist[qq]; };
-        const float cap = dmin + hd + M.eps;
-        float best[VPL], bound[VPL];
+            return s0;
+        };
+
+        // ---- FP32 nearest triangle
+        float best[VPL], b2[VPL];
         int arg[VPL];
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             best[v] = FLT_MAX;
-            bound[v] = cap;
+            b2[v] = FLT_MAX;
             arg[v] = -1;
         }
-        auto visit = [&](const TriRec& T, int t) {
-            bool need = false;
+        walk(valid, b2, [&](int loc) {
+            int cnt;
+            const long long s0 = leaf_rec(loc, cnt);
+            for (int c = 0; c < cnt; ++c) {
+                const TriRec T = D > 0 ? s_rec[c] : load_uniform(C.rec, (int)(s0 + c));
+                ++ntest;
+                bool need = false;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v)
+                    need |= valid[v] && tri_lb2(T, px, py, pz[v]) <= b2[v];
+                if (__ballot(need) == 0) continue;
+                ++neval;
+#pragma unroll
+                for (int v = 0; v < VPL; ++v) {
+                    // equal distances go to the lower slot, so the choice does not
+                    // depend on the order the tree is walked in
+                    const float d2 = tri_d2(T, px, py, pz[v]);
+                    const int s = (int)(s0 + c);
+                    if (d2 < best[v] || (d2 == best[v] && s < arg[v])) {
+                        best[v] = d2;
+                        arg[v] = s;
+                    }
+                    const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + eps;
+                    b2[v] = bb * bb;
+                }
+            }
+        });
+
+        // ---- within one cell of the surface: the reference's FP64 distance over every
+        // triangle within the FP32 minimum plus twice the error margin
+        bool near[VPL];
+        bool any_near = false;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            near[v] = valid[v] && sqrtf(best[v]) <= h * (1.0f + 1e-3f) + 2.0f * eps;
+            any_near |= near[v];
+        }
+        double best64[VPL];
+        int arg64[VPL];
+        double p64[VPL][3];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            best64[v] = DBL_MAX;
+            arg64[v] = -1;
+            p64[v][0] = g.corner[0] + (i + 0.5) * g.spacing;
+            p64[v][1] = g.corner[1] + (j + 0.5) * g.spacing;
+            p64[v][2] = g.corner[2] + (k[v] + 0.5) * g.spacing;
+        }
+        if (__ballot(any_near) != 0) {
+            float b2n[VPL];
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
-                const float dx = px - T.mx, dy = py - T.my, dz = pz[v] - T.mz;
-                const float rr = T.r + bound[v];
-                need |= valid[v] && fma3(dx, dx, dy, dy, dz, dz) <= rr * rr;
+                const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + 2.0f * eps;
+                b2n[v] = bb * bb;
             }
-            if (__ballot(need) == 0) return;
-            ++evals;
+            walk(near, b2n, [&](int loc) {
+                int cnt;
+                const long long s0 = leaf_rec(loc, cnt);
+                for (int c = 0; c < cnt; ++c) {
+                    const TriRec T = D > 0 ? s_rec[c] : load_uniform(C.rec, (int)(s0 + c));
+                    bool need[VPL];
+                    bool anyn = false;
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                const float d2 = tri_d2(T, px, py, pz[v]);
-                if (d2 < best[v]) {
-                    best[v] = d2;
-                    arg[v] = t;
+                    for (int v = 0; v < VPL; ++v) {
+                        need[v] = near[v] && tri_lb2(T, px, py, pz[v]) <= b2n[v];
+                        anyn |= need[v];
+                    }
+                    if (__ballot(anyn) == 0) continue;
+                    const int s = (int)(s0 + c);
+                    const TriRec64 T64 = load_uniform(C.rec64, s);
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v)
+                        if (need[v]) {
+                            const double d = point_tri_dist2_ref(p64[v], T64.a, T64.b, T64.c);
+                            // std::min(best, d) (voxelize.cpp:317); ties between distinct
+                            // triangles go to the lower input index
+                            if (d < best64[v] ||
+                                (d == best64[v] && C.perm[s] < C.perm[arg64[v]])) {
+                                best64[v] = d;
+                                arg64[v] = s;
+                            }
+                        }
                 }
-                bound[v] = fminf(cap, sqrtf(best[v]) * (1.0f + 4.0f * kU) + M.eps);
-            }
-        };
-        if (D == 0) {
-            for (int qq = 0; qq < count; ++qq) {
-                const int t = uniform_int(list_at(qq));
-                visit(load_uniform(C.rec, t), t);
-            }
-        } else {
-            for (int c0 = 0; c0 < count; c0 += D) {
-                const int nq = min(D, count - c0);
-                __syncthreads();
-                for (int e = lane; e < nq * 8; e += 64) {
-                    const int t = list_at(c0 + (e >> 3));
-                    reinterpret_cast<float4*>(&srec[e >> 3])[e & 7] =
-                        reinterpret_cast<const float4*>(C.rec + t)[e & 7];
-                }
-                __syncthreads();
-                for (int qq = 0; qq < nq; ++qq) visit(srec[qq], uniform_int(list_at(c0 + qq)));
-            }
+            });
         }
         const uint8_t* occ = C.occ + M.cell_base;
         if (!BWD) {
-            bool near[VPL];
-            bool any_near = false;
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                near[v] = valid[v] && sqrtf(best[v]) <= h * (1.0f + 1e-3f) + 2.0f * M.eps;
-                any_near |= near[v];
-            }
-            double best64[VPL];
-#pragma unroll
-            for (int v = 0; v < VPL; ++v) best64[v] = DBL_MAX;
-            if (__ballot(any_near) != 0) {
-                double p64[VPL][3];
-#pragma unroll
-                for (int v = 0; v < VPL; ++v) {
-                    p64[v][0] = g.corner[0] + (i + 0.5) * g.spacing;
-                    p64[v][1] = g.corner[1] + (j + 0.5) * g.spacing;
-                    p64[v][2] = g.corner[2] + (k[v] + 0.5) * g.spacing;
-                }
-                for (int qq = 0; qq < count; ++qq) {
-                    const int t = uniform_int(list_at(qq));
-                    const TriRec64 T = load_uniform(C.rec64, t);
-#pragma unroll
-                    for (int v = 0; v < VPL; ++v)
-                        if (near[v]) {
-                            const double d = point_tri_dist2_ref(p64[v], T.a, T.b, T.c);
-                            if (d < best64[v]) best64[v] = d;
-                        }
-                }
-            }
             float* out = C.sdf + M.cell_base;
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
@@ -559,32 +741,52 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     out[idx] = occ[idx] ? -d32 : d32;
                 }
             }
-            __syncthreads();
             continue;
         }
-#pragma unroll
+        // ---- adjoint: d = |p - q| with q the closest point, dd/dv_m = -w_m (p - q) / d
         for (int v = 0; v < VPL; ++v) {
             float c[9];
             bool act = false;
             const size_t idx = valid[v] ? ((size_t)k[v] * g.ny + j) * g.nx + i : 0;
-            if (valid[v] && arg[v] >= 0) {
-                const TriRec T = C.rec[arg[v]];
-                float vv, ww;
-                tri_bary(T, px, py, pz[v], vv, ww);
-                const float u = 1.0f - vv - ww;
-                const float qx = T.ax + vv * T.abx + ww * T.acx;
-                const float qy = T.ay + vv * T.aby + ww * T.acy;
-                const float qz = T.az + vv * T.abz + ww * T.acz;
-                const float ex = px - qx, ey = py - qy, ez = pz[v] - qz;
-                const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+            const int tri = near[v] ? arg64[v] : arg[v];
+            if (valid[v] && C.nearest)
+                C.nearest[M.cell_base + idx] = tri >= 0 ? (int)(C.perm[tri] - M.tbase) : -1;
+            if (valid[v] && tri >= 0 && C.grad) {
                 const float gv = C.gsdf[M.cell_base + idx];
-                if (d > 0.0f && gv != 0.0f) {
-                    const float s = occ[idx] ? -1.0f : 1.0f;
-                    const float coef = -gv * s / d;
-                    c[0] = coef * u * ex; c[1] = coef * u * ey; c[2] = coef * u * ez;
-                    c[3] = coef * vv * ex; c[4] = coef * vv * ey; c[5] = coef * vv * ez;
-                    c[6] = coef * ww * ex; c[7] = coef * ww * ey; c[8] = coef * ww * ez;
-                    act = true;
+                const float sg = occ[idx] ? -1.0f : 1.0f;
+                if (near[v]) {  // FP64 closest point near the surface
+                    const TriRec64& T64 = C.rec64[tri];
+                    double w[3];
+                    const double d2 = point_tri_dist2_ref(p64[v], T64.a, T64.b, T64.c, w);
+                    double e[3];
+                    for (int a = 0; a < 3; ++a)
+                        e[a] = p64[v][a] - (w[0] * T64.a[a] + w[1] * T64.b[a] + w[2] * T64.c[a]);
+                    const double d = sqrt(d2);
+                    if (d > M.dzero && gv != 0.0f) {
+                        const double coef = -(double)gv * sg / d;
+#pragma unroll
+                        for (int m = 0; m < 3; ++m)
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) c[3 * m + a] = (float)(coef * w[m] * e[a]);
+                        act = true;
+                    }
+                } else {
+                    const TriRec T = C.rec[tri];
+                    float vv, ww;
+                    tri_bary(T, px, py, pz[v], vv, ww);
+                    const float u = 1.0f - vv - ww;
+                    const float qx = T.ax + vv * T.abx + ww * T.acx;
+                    const float qy = T.ay + vv * T.aby + ww * T.acy;
+                    const float qz = T.az + vv * T.abz + ww * T.acz;
+                    const float ex = px - qx, ey = py - qy, ez = pz[v] - qz;
+                    const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+                    if (d > 0.0f && gv != 0.0f) {
+                        const float coef = -gv * sg / d;
+                        c[0] = coef * u * ex; c[1] = coef * u * ey; c[2] = coef * u * ez;
+                        c[3] = coef * vv * ex; c[4] = coef * vv * ey; c[5] = coef * vv * ez;
+                        c[6] = coef * ww * ex; c[7] = coef * ww * ey; c[8] = coef * ww * ez;
+                        act = true;
+                    }
                 }
             }
             // wavefront-aggregated scatter: one group of lanes per distinct triangle
@@ -592,13 +794,13 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 const unsigned long long mask = __ballot(act);
                 if (mask == 0) break;
                 const int leader = __ffsll((long long)mask) - 1;
-                const int lt = __shfl(arg[v], leader);
-                const bool match = act && arg[v] == lt;
+                const int lt = __shfl(tri, leader);
+                const bool match = act && tri == lt;
                 float sum[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
                 if (lane == leader) {
-                    const int* tv = C.tris + 3 * (size_t)lt;
+                    const int* tv = C.tris + 3 * (size_t)C.perm[lt];
                     float* gbase = C.grad + 3 * M.vbase;
 #pragma unroll
                     for (int vtx = 0; vtx < 3; ++vtx) {
@@ -611,9 +813,12 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 act = act && !match;
             }
         }
-        __syncthreads();
     }
-    if (lane == 0 && evals) atomicAdd(&C.ctr->pairs, evals);
+    if (lane == 0) {
+        if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
+        if (ntest) atomicAdd(&C.ctr->tri_tests, ntest);
+        if (neval) atomicAdd(&C.ctr->tri_evals, neval);
+    }
 }
 
 __global__ __launch_bounds__(256) void bbox_kernel(const double* xyz, const long long* vstart,
@@ -648,15 +853,41 @@ __global__ __launch_bounds__(256) void bbox_kernel(const double* xyz, const long
 
 // ---- host side ---------------------------------------------------------------------------
 
+namespace {
+
+// Stable LSD radix sort of (code, index) pairs by the 30-bit code, 3 passes of 10 bits.
+void sort_by_code(std::vector<unsigned>& code, std::vector<int>& idx) {
+    const size_t n = code.size();
+    std::vector<unsigned> c2(n);
+    std::vector<int> i2(n);
+    for (int pass = 0; pass < 3; ++pass) {
+        const int sh = 10 * pass;
+        size_t cnt[1025] = {0};
+        for (size_t k = 0; k < n; ++k) ++cnt[((code[k] >> sh) & 1023u) + 1];
+        for (int b = 0; b < 1024; ++b) cnt[b + 1] += cnt[b];
+        for (size_t k = 0; k < n; ++k) {
+            const size_t at = cnt[(code[k] >> sh) & 1023u]++;
+            c2[at] = code[k];
+            i2[at] = idx[k];
+        }
+        code.swap(c2);
+        idx.swap(i2);
+    }
+}
+
+}  // namespace
+
 struct BatchEngine::Impl {
     hipStream_t stream = nullptr;
     int nm = 0;
     std::vector<MeshDesc> h_mesh;
     std::vector<int> h_order;
     std::vector<long long> h_item_off;
-    std::vector<long long> h_tri_start, h_pt_start, h_cell_start;
+    std::vector<int> h_lvl_off;  // [kMaxLevels + 1][nm + 1]
+    std::vector<long long> h_tri_start, h_pt_start;
     long long total_tris = 0, total_pts = 0, total_cells = 0, total_tiles = 0;
-    long long total_cols_words = 0, n_l2 = 0, n_l1 = 0;
+    long long total_cols_words = 0;
+    int max_lev = 1;
     // device buffers (owned)
     std::vector<void*> owned;
     Ctx ctx{};
@@ -690,6 +921,7 @@ struct BatchEngine::Impl {
 
     void launch_forward(hipStream_t st);
     void launch_backward(hipStream_t st);
+    void launch_tree(hipStream_t st);
 
     template <bool BWD>
     void launch_vox(hipStream_t st) {
@@ -699,15 +931,27 @@ struct BatchEngine::Impl {
         CMG_HIP_LAUNCH();                                              \
         return;                                                        \
     }
-        CMG_VOX(1, 0) CMG_VOX(1, 32) CMG_VOX(1, 64) CMG_VOX(2, 0) CMG_VOX(2, 32) CMG_VOX(2, 64)
+        CMG_VOX(1, 0) CMG_VOX(1, 8) CMG_VOX(2, 0) CMG_VOX(2, 8)
 #undef CMG_VOX
         throw HipError("batch: unknown voxel-pass variant");
     }
 };
 
+void BatchEngine::Impl::launch_tree(hipStream_t st) {
+    const int n1 = h_lvl_off[(size_t)1 * (nm + 1) + nm];
+    node_leaf_kernel<<<grid256, 256, 0, st>>>(ctx, n1);
+    CMG_HIP_LAUNCH();
+    for (int l = 2; l <= max_lev; ++l) {
+        const int nl = h_lvl_off[(size_t)l * (nm + 1) + nm];
+        const unsigned g = (unsigned)std::min<long long>(grid256, (nl + 255) / 256);
+        node_merge_kernel<<<g ? g : 1, 256, 0, st>>>(ctx, l, nl);
+        CMG_HIP_LAUNCH();
+    }
+}
+
 void BatchEngine::Impl::launch_forward(hipStream_t st) {
     Ctx& C = ctx;
-    reset_kernel<<<grid256, 256, 0, st>>>(C, 0, 0);
+    reset_kernel<<<grid256, 256, 0, st>>>(C);
     CMG_HIP_LAUNCH();
     prep_kernel<<<grid256, 256, 0, st>>>(C);
     CMG_HIP_LAUNCH();
@@ -717,18 +961,7 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     CMG_HIP_LAUNCH();
     occ_kernel<<<grid64, 64, 0, st>>>(C);
     CMG_HIP_LAUNCH();
-    cull_kernel<2, 0><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    cull_kernel<2, 1><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    cull_kernel<2, 2><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    cull_kernel<1, 0><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    cull_kernel<1, 1><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    cull_kernel<1, 2><<<grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
+    launch_tree(st);
     {
         TimedLaunch timed("batch_sdf", st);
         launch_vox<false>(st);
@@ -750,6 +983,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
                          const int* d_tris, hipStream_t st)
     : p_(new Impl) {
     Impl& I = *p_;
+    if (nm < 1) throw InvalidInput("batch: no meshes");
     I.stream = st;
     I.nm = nm;
     I.xyz = d_xyz;
@@ -758,10 +992,13 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     I.h_tri_start.assign(tri_start, tri_start + nm + 1);
     I.total_pts = pt_start[nm];
     I.total_tris = tri_start[nm];
+    if (I.total_tris >= (1LL << 31)) throw InvalidInput("batch: more than 2^31 triangles");
     for (int m = 0; m < nm; ++m) {
         if (pt_start[m + 1] <= pt_start[m]) throw InvalidInput("batch: a mesh has no points");
         if (tri_start[m + 1] <= tri_start[m])
             throw InvalidInput("batch: a mesh has no triangles");
+        if (tri_start[m + 1] - tri_start[m] >= (1LL << 30))
+            throw InvalidInput("batch: a mesh has 2^30 triangles or more");
         if (resolution[m] < 1) throw InvalidInput("batch: resolution must be >= 1");
     }
     // grids from device bounding boxes (min/max are exact, so make_grid matches the
@@ -775,19 +1012,21 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     CMG_HIP(hipMemcpyAsync(lohi.data(), d_lohi, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
     I.h_mesh.resize(nm);
-    long long cells = 0, tiles = 0, words = 0, l2 = 0, l1 = 0;
+    long long cells = 0, tiles = 0, words = 0;
     std::vector<double> cost(nm);
+    std::vector<long long> nodes_per_level(kMaxLevels + 1, 0);
+    I.h_lvl_off.assign((size_t)(kMaxLevels + 1) * (nm + 1), 0);
     for (int m = 0; m < nm; ++m) {
-        double box[6] = {lohi[6 * m], lohi[6 * m + 1], lohi[6 * m + 2],
+        double two[6] = {lohi[6 * m], lohi[6 * m + 1], lohi[6 * m + 2],
                          lohi[6 * m + 3], lohi[6 * m + 4], lohi[6 * m + 5]};
         // make_grid over the two corners reproduces make_grid over all points
-        double two[6] = {box[0], box[1], box[2], box[3], box[4], box[5]};
         GridSpec g;
         if (!make_grid(two, 2, resolution[m], pad[m], g))
             throw InvalidInput("batch: degenerate (zero-extent) mesh");
         if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0)
             throw InvalidInput("batch: grid has no cells (negative pad)");
         MeshDesc& M = I.h_mesh[m];
+        M = MeshDesc{};
         M.g = g;
         M.vbase = pt_start[m];
         M.tbase = tri_start[m];
@@ -801,24 +1040,44 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.mask_base = words;
         M.L0 = make_level(g, kLeaf);
         M.L0b = make_level(g, kLeaf, 2 * kLeaf);
-        M.L1 = make_level(g, 2 * kLeaf);
-        M.L2 = make_level(g, 4 * kLeaf);
-        M.l2_base = (int)l2;
-        M.l1_base = (int)l1;
         M.eps = distance_eps(g);
+        M.dzero = zero_distance(g);
+        // tree levels: level l has ceil(ntris / 8^l) nodes, up to the single root
+        M.nnodes[0] = M.ntris;
+        int l = 0;
+        long long n = M.ntris;
+        do {
+            ++l;
+            n = (n + 7) / 8;
+            M.nnodes[l] = (int)n;
+        } while (n > 1);
+        M.nlev = l;
+        I.max_lev = std::max(I.max_lev, l);
         const long long nc = (long long)g.nx * g.ny * g.nz;
         cells += nc;
         tiles += (long long)op.ntx * op.nty;
         words += (long long)g.nx * g.ny * op.words;
-        l2 += (long long)M.L2.bx * M.L2.by * M.L2.bz;
-        l1 += (long long)M.L1.bx * M.L1.by * M.L1.bz;
         cost[m] = (double)nc * std::log2(2.0 + M.ntris);
     }
+    // node numbering: level by level, meshes in id order; lo[] holds each level's
+    // per-mesh prefix counted from that level's first node
+    long long node_total = 0;
+    for (int l = 1; l <= kMaxLevels; ++l) {
+        int* lo = &I.h_lvl_off[(size_t)l * (nm + 1)];
+        long long rel = 0;
+        for (int m = 0; m < nm; ++m) {
+            MeshDesc& M = I.h_mesh[m];
+            lo[m] = (int)rel;
+            M.node_base[l] = (int)(node_total + rel);
+            rel += l <= M.nlev ? M.nnodes[l] : 0;
+        }
+        lo[nm] = (int)rel;
+        node_total += rel;
+    }
+    if (node_total >= (1LL << 31)) throw InvalidInput("batch: tree too large");
     I.total_cells = cells;
     I.total_tiles = tiles;
     I.total_cols_words = words;
-    I.n_l2 = l2;
-    I.n_l1 = l1;
     // processing order: most expensive meshes first
     I.h_order.resize(nm);
     std::iota(I.h_order.begin(), I.h_order.end(), 0);
@@ -831,8 +1090,6 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             const MeshDesc& M = I.h_mesh[I.h_order[p]];
             long long n = 0;
             if (ph == kPhTile) n = (long long)M.ntx * M.nty;
-            if (ph == kPhL2) n = (long long)M.L2.bx * M.L2.by * M.L2.bz;
-            if (ph == kPhL1) n = (long long)M.L1.bx * M.L1.by * M.L1.bz;
             if (ph == kPhL0) n = (long long)M.L0.bx * M.L0.by * M.L0.bz;
             if (ph == kPhL0b) n = (long long)M.L0b.bx * M.L0b.by * M.L0b.bz;
             o[p + 1] = o[p] + n;
@@ -845,72 +1102,72 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     int* d_order = I.dalloc<int>(nm);
     long long* d_item = I.dalloc<long long>(I.h_item_off.size());
     long long* d_ts = I.dalloc<long long>(nm + 1);
+    int* d_lvl = I.dalloc<int>(I.h_lvl_off.size());
     CMG_HIP(hipMemcpyAsync(d_mesh, I.h_mesh.data(), sizeof(MeshDesc) * nm, hipMemcpyHostToDevice, st));
     CMG_HIP(hipMemcpyAsync(d_order, I.h_order.data(), sizeof(int) * nm, hipMemcpyHostToDevice, st));
     CMG_HIP(hipMemcpyAsync(d_item, I.h_item_off.data(), sizeof(long long) * I.h_item_off.size(),
                            hipMemcpyHostToDevice, st));
     CMG_HIP(hipMemcpyAsync(d_ts, tri_start, sizeof(long long) * (nm + 1), hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_lvl, I.h_lvl_off.data(), sizeof(int) * I.h_lvl_off.size(),
+                           hipMemcpyHostToDevice, st));
     C.mesh = d_mesh;
     C.order = d_order;
     C.item_off = d_item;
     C.tri_start = d_ts;
+    C.lvl_off = d_lvl;
     C.total_tris = I.total_tris;
     C.xyz = d_xyz;
     C.tris = d_tris;
     C.ctr = I.dalloc<BatchCounters>(1);
     C.rec = I.dalloc<TriRec>(I.total_tris);
     C.rec64 = I.dalloc<TriRec64>(I.total_tris);
+    C.node = I.dalloc<NodeRec>(node_total);
     C.xyr = I.dalloc<int4>(I.total_tris);
     C.tile_count = I.dalloc<int>(tiles);
     C.tile_off = I.dalloc<int>(tiles);
     C.tile_cursor = I.dalloc<int>(tiles);
     C.masks = I.dalloc<unsigned long long>(words);
     C.occ = I.dalloc<uint8_t>(cells);
-    C.dmin2 = I.dalloc<float>(l2);
-    C.cnt2 = I.dalloc<int>(l2);
-    C.off2 = I.dalloc<long long>(l2);
-    C.dmin1 = I.dalloc<float>(l1);
-    C.cnt1 = I.dalloc<int>(l1);
-    C.off1 = I.dalloc<long long>(l1);
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
     I.grid256 = (unsigned)cu_count() * 4;
-    // planning pass: run the forward once with empty pools to learn the list sizes
-    C.tile_cap = C.cap2 = C.cap1 = 0;
-    C.tile_pool = C.pool2 = C.pool1 = nullptr;
+    // Morton order of every mesh's triangles (once, from the creation-time positions)
+    {
+        unsigned* d_code = I.dalloc<unsigned>(I.total_tris);
+        morton_kernel<<<I.grid256, 256, 0, st>>>(C, d_code);
+        CMG_HIP_LAUNCH();
+        std::vector<unsigned> code(I.total_tris);
+        CMG_HIP(hipMemcpyAsync(code.data(), d_code, sizeof(unsigned) * I.total_tris,
+                               hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+        std::vector<int> perm(I.total_tris);
+        for (int m = 0; m < nm; ++m) {
+            const long long t0 = tri_start[m], t1 = tri_start[m + 1];
+            std::vector<unsigned> c(code.begin() + t0, code.begin() + t1);
+            std::vector<int> idx(t1 - t0);
+            std::iota(idx.begin(), idx.end(), (int)t0);
+            sort_by_code(c, idx);
+            std::copy(idx.begin(), idx.end(), perm.begin() + t0);
+        }
+        int* d_perm = I.dalloc<int>(I.total_tris);
+        CMG_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * I.total_tris,
+                               hipMemcpyHostToDevice, st));
+        C.perm = d_perm;
+        CMG_HIP(hipStreamSynchronize(st));
+    }
+    // planning pass: the tile-list sizes
+    C.tile_cap = 0;
+    C.tile_pool = nullptr;
     C.sdf = nullptr;
-    // sizes come from the count passes; writes are skipped while the caps are 0
-    reset_kernel<<<I.grid256, 256, 0, st>>>(C, 0, 0);
+    reset_kernel<<<I.grid256, 256, 0, st>>>(C);
     prep_kernel<<<I.grid256, 256, 0, st>>>(C);
     tile_alloc_kernel<<<I.grid256, 256, 0, st>>>(C);
     CMG_HIP_LAUNCH();
     BatchCounters h{};
     CMG_HIP(hipMemcpyAsync(&h, C.ctr, sizeof h, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
-    auto margin = [](unsigned long long n) { return (long long)(n + n / 4 + 4096); };
-    C.tile_cap = margin(h.tile_used);
+    C.tile_cap = (long long)(h.tile_used + h.tile_used / 4 + 4096);
     C.tile_pool = I.dalloc<int>(C.tile_cap);
-    // second planning step: full occupancy and level-2 counts
-    reset_kernel<<<I.grid256, 256, 0, st>>>(C, 0, 0);
-    prep_kernel<<<I.grid256, 256, 0, st>>>(C);
-    tile_alloc_kernel<<<I.grid256, 256, 0, st>>>(C);
-    tile_fill_kernel<<<I.grid256, 256, 0, st>>>(C);
-    occ_kernel<<<I.grid64, 64, 0, st>>>(C);
-    cull_kernel<2, 0><<<I.grid256, 256, 0, st>>>(C);
-    cull_kernel<2, 1><<<I.grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    CMG_HIP(hipMemcpyAsync(&h, C.ctr, sizeof h, hipMemcpyDeviceToHost, st));
-    CMG_HIP(hipStreamSynchronize(st));
-    C.cap2 = margin(h.l2_used);
-    C.pool2 = I.dalloc<int>(C.cap2);
-    cull_kernel<2, 2><<<I.grid256, 256, 0, st>>>(C);
-    cull_kernel<1, 0><<<I.grid256, 256, 0, st>>>(C);
-    cull_kernel<1, 1><<<I.grid256, 256, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
-    CMG_HIP(hipMemcpyAsync(&h, C.ctr, sizeof h, hipMemcpyDeviceToHost, st));
-    CMG_HIP(hipStreamSynchronize(st));
-    C.cap1 = margin(h.l1_used);
-    C.pool1 = I.dalloc<int>(C.cap1);
 }
 
 BatchEngine::~BatchEngine() = default;
@@ -930,6 +1187,7 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     I.ctx.sdf = d_sdf;
     I.ctx.gsdf = d_gsdf;
     I.ctx.grad = d_grad;
+    I.ctx.nearest = nullptr;
     if (!(flags & kBatchGraph)) {
         if (fwd) I.launch_forward(st);
         if (bwd) I.launch_backward(st);
@@ -969,13 +1227,26 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     CMG_HIP(hipGraphLaunch(I.exec, st));
 }
 
+void BatchEngine::nearest(int* d_tri, hipStream_t st) {
+    Impl& I = *p_;
+    Ctx saved = I.ctx;
+    I.ctx.gsdf = nullptr;
+    I.ctx.grad = nullptr;
+    I.ctx.nearest = d_tri;
+    reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 2, 2);
+    CMG_HIP_LAUNCH();
+    I.launch_vox<true>(st);
+    I.ctx = saved;
+}
+
 std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sdf,
                                                 float* d_grad, hipStream_t st) {
     Impl& I = *p_;
     I.ctx.sdf = d_sdf;
     I.ctx.gsdf = d_gsdf;
     I.ctx.grad = d_grad;
-    I.launch_forward(st);  // lists and occupancy for this geometry
+    I.ctx.nearest = nullptr;
+    I.launch_forward(st);  // occupancy and tree for this geometry
     std::vector<BatchVariant> out;
     hipEvent_t e0, e1;
     CMG_HIP(hipEventCreate(&e0));
@@ -983,12 +1254,12 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
     float best_ms = 1e30f;
     int best_v = I.vpl, best_d = I.depth;
     for (int v : {1, 2})
-        for (int d : {0, 32, 64}) {
+        for (int d : {0, 8}) {
             I.vpl = v;
             I.depth = d;
             float total = 0;
             for (int rep = 0; rep < 3; ++rep) {
-                reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 7, 8);
+                reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 1, 2);
                 zero_grad_kernel<<<I.grid256, 256, 0, st>>>(I.ctx.grad, 3 * I.total_pts, I.ctx.ctr);
                 CMG_HIP(hipEventRecord(e0, st));
                 I.launch_vox<false>(st);
@@ -1014,8 +1285,8 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
 }
 
 void BatchEngine::set_variant(int vpl, int depth) {
-    if (!((vpl == 1 || vpl == 2) && (depth == 0 || depth == 32 || depth == 64)))
-        throw InvalidInput("batch: variant must have vpl in {1,2} and depth in {0,32,64}");
+    if (!((vpl == 1 || vpl == 2) && (depth == 0 || depth == 8)))
+        throw InvalidInput("batch: variant must have vpl in {1,2} and depth in {0,8}");
     Impl& I = *p_;
     if (I.vpl != vpl || I.depth != depth) {
         I.vpl = vpl;
@@ -1026,12 +1297,17 @@ void BatchEngine::set_variant(int vpl, int depth) {
 
 BatchVariant BatchEngine::variant() const { return BatchVariant{p_->vpl, p_->depth, 0.0f}; }
 
-void BatchEngine::status(hipStream_t st, unsigned* overflow, unsigned long long* pairs) {
+BatchCounters BatchEngine::counters(hipStream_t st) {
     BatchCounters h{};
     CMG_HIP(hipMemcpyAsync(&h, p_->ctx.ctr, sizeof h, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
+    return h;
+}
+
+void BatchEngine::status(hipStream_t st, unsigned* overflow, unsigned long long* evals) {
+    BatchCounters h = counters(st);
     if (overflow) *overflow = h.overflow;
-    if (pairs) *pairs = h.pairs;
+    if (evals) *evals = h.tri_evals;
 }
 
 const uint8_t* BatchEngine::occupancy() const { return p_->ctx.occ; }

@@ -1,9 +1,10 @@
-// batch_internal.hpp — the many-mesh voxelization engine (batch.hip).
+// batch_internal.hpp — the voxelization engine (batch.hip): one or many meshes per step.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <vector>
 
 #include "cmg_hip_internal.hpp"
 #include "sdf_device.hpp"
@@ -11,35 +12,52 @@
 
 namespace cmg::hip {
 
+// Levels of the per-mesh triangle tree: level 0 are the triangles (in Morton order of
+// their centroids), a level-l node covers 8^l consecutive triangles. 8^11 > 2^31.
+constexpr int kMaxLevels = 12;
+
+// Bounding cylinder of a tree node (coordinates relative to the grid corner).
+struct __align__(16) NodeRec {
+    float cx, cy, cz, r;  // centre, lateral radius
+    float nx, ny, nz, t;  // unit axis, half-height
+};
+static_assert(sizeof(NodeRec) == 32, "node size");
+
 // Per-mesh description, uploaded once at creation.
 struct MeshDesc {
     GridSpec g;
     long long vbase;      // first point of the mesh in the concatenated point array
-    long long tbase;      // first triangle
+    long long tbase;      // first triangle (and first tree slot)
     int ntris;
+    int nlev;             // root level (>= 1)
     long long cell_base;  // first cell in the concatenated outputs
     long long mask_base;  // first occupancy bit-mask word
     int words, ntx, nty;  // occupancy: mask words per column, column tiles per axis
     int tile_base;        // first column tile in the global tile arrays
-    sdfdev::Level L0, L0b, L1, L2;  // L0: 4x4x4 leaves, L0b: 4x4x8 leaves
-    int l2_base, l1_base; // first 16^3 / 8^3 brick in the global brick arrays
+    sdfdev::Level L0, L0b;         // voxel leaves: 4x4x4 and 4x4x8
+    int node_base[kMaxLevels + 1]; // first node of each level (1..nlev) in the node array
+    int nnodes[kMaxLevels + 1];    // nodes per level (level 0: triangles)
     float eps;            // FP32 distance error margin for this grid
+    double dzero;         // distances at or below this carry no gradient
 };
 
-enum BatchPhase { kPhTile = 0, kPhL2 = 1, kPhL1 = 2, kPhL0 = 3, kPhL0b = 4, kNumPhase = 5 };
+enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kNumPhase = 3 };
 
 struct BatchCounters {
-    unsigned long long queue[10];  // work queues of the persistent kernels
-    unsigned long long tile_used, l2_used, l1_used;  // pool cursors
-    unsigned long long pairs;      // full FP32 distance evaluations (statistics)
-    unsigned overflow;             // bit 0 tiles, bit 1 16^3 lists, bit 2 8^3 lists
+    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 backward
+    unsigned long long tile_used;  // tile-list pool cursor
+    // work of the voxel passes, counted per wavefront: cylinder tests of tree nodes,
+    // bounding-disk tests of triangles, FP32 point-triangle distances
+    unsigned long long node_tests, tri_tests, tri_evals;
+    unsigned overflow;             // bit 0: tile lists
     unsigned pad;
 };
 
 enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4 };
 
 // A voxel-pass variant: voxels per lane (1: 4x4x4 leaves, 2: 4x4x8) and the LDS tile
-// depth (0: records read with scalar loads, 32/64: staged through LDS in tiles).
+// depth of a tree leaf's triangle records (0: read with scalar loads, 8: staged through
+// LDS as one 1 KiB tile).
 struct BatchVariant {
     int vpl, depth;
     float ms;  // forward + backward voxel passes, when measured by autotune
@@ -49,7 +67,8 @@ class BatchEngine {
   public:
     // pt_start/tri_start: host prefix arrays (nm + 1 entries) into the concatenated
     // device arrays d_xyz (3 doubles per point) and d_tris (3 local point indices per
-    // triangle). Plans every grid and sizes the list pools (synchronises `st`).
+    // triangle). Plans every grid, orders every mesh's triangles along a Morton curve
+    // and sizes the tile pool (synchronises `st`).
     BatchEngine(int nm, const long long* pt_start, const long long* tri_start,
                 const int* resolution, const int* pad, const double* d_xyz,
                 const int* d_tris, hipStream_t st);
@@ -70,15 +89,20 @@ class BatchEngine {
     void run(int flags, const double* d_xyz, float* d_sdf, const float* d_gsdf, float* d_grad,
              hipStream_t st);
 
+    // After a forward: for every cell, the mesh-local index of the triangle whose closest
+    // point the adjoint uses (-1 where there is none), into d_tri (total_cells ints).
+    void nearest(int* d_tri, hipStream_t st);
+
     // Times every voxel-pass variant on this batch (forward + backward voxel passes, HIP
-    // events on st, after one forward builds the lists) and keeps the fastest.
+    // events on st, after one forward) and keeps the fastest.
     std::vector<BatchVariant> autotune(const float* d_gsdf, float* d_sdf, float* d_grad,
                                        hipStream_t st);
     void set_variant(int vpl, int depth);
     BatchVariant variant() const;
 
-    // Synchronises st and reads the overflow flags and the evaluation count.
-    void status(hipStream_t st, unsigned* overflow, unsigned long long* pairs);
+    // Synchronises st and reads the counters of the last step.
+    void status(hipStream_t st, unsigned* overflow, unsigned long long* evals);
+    BatchCounters counters(hipStream_t st);
 
   private:
     struct Impl;

@@ -476,7 +476,7 @@ cmg_status cmg_voxelize_batch_autotune(cmg_batch* b, float* sdf, const float* gr
     return guarded(errbuf, len, [&] {
         auto all = b->engine->autotune(grad_sdf, sdf, grad_xyz, as_stream(stream));
         if (ms)
-            for (size_t i = 0; i < all.size() && i < 6; ++i) ms[i] = all[i].ms;
+            for (size_t i = 0; i < all.size() && i < 4; ++i) ms[i] = all[i].ms;
         cmg::hip::BatchVariant v = b->engine->variant();
         if (vpl) *vpl = v.vpl;
         if (depth) *depth = v.depth;
@@ -490,6 +490,24 @@ cmg_status cmg_voxelize_batch_set_variant(cmg_batch* b, int vpl, int depth) {
 
 const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch* b) {
     return b ? b->engine->occupancy() : nullptr;
+}
+
+cmg_status cmg_voxelize_batch_nearest(cmg_batch* b, int32_t* tri, void* stream,
+                                      char* errbuf, size_t len) {
+    if (!b || !tri) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] { b->engine->nearest(tri, as_stream(stream)); });
+}
+
+cmg_status cmg_voxelize_batch_counters(cmg_batch* b, void* stream, uint64_t* out,
+                                       char* errbuf, size_t len) {
+    if (!b || !out) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        const cmg::hip::BatchCounters c = b->engine->counters(as_stream(stream));
+        out[0] = c.node_tests;
+        out[1] = c.tri_tests;
+        out[2] = c.tri_evals;
+        out[3] = c.overflow;
+    });
 }
 
 void cmg_hip_kernel_timer(const char* name, int enable) {

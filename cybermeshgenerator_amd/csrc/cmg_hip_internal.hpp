@@ -67,6 +67,7 @@ struct GridSpec {
     double corner[3] = {0, 0, 0};
     double spacing = 0;
     int nx = 0, ny = 0, nz = 0;
+    int resolution = 0, pad = 0;  // the make_grid arguments that produced it
 };
 
 // make_grid (voxelize.cpp:52-74), host side, same FP64 operations. False for a

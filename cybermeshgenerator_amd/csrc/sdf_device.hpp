@@ -1,7 +1,7 @@
-// sdf_device.hpp — device-side pieces of the signed-distance path shared by the
-// single-mesh launchers (sdf.hip) and the batched engine (batch.hip): the FP32 triangle
-// record, brick geometry, the FP32 point-triangle distance and closest-point
-// barycentrics, and the reference's FP64 distance (src/voxelize/voxelize.cpp:197-247).
+// sdf_device.hpp — device-side pieces of the signed-distance path (batch.hip): the FP32
+// triangle record with its bounding disk, voxel-leaf geometry, the bounding-cylinder
+// lower bound, the FP32 point-triangle distance and closest-point barycentrics, and the
+// reference's FP64 distance (src/voxelize/voxelize.cpp:197-247).
 // Every TU that includes it is compiled with -ffp-contract=off.
 #pragma once
 
@@ -16,7 +16,6 @@ namespace cmg::hip {
 namespace sdfdev {
 
 constexpr int kLeaf = 4;          // 4^3 voxels per wavefront
-constexpr int kLdsList = 2048;    // candidate indices kept in LDS per leaf brick
 constexpr float kU = 0x1p-24f;    // FP32 unit roundoff
 
 // FP32 triangle record, coordinates relative to the grid corner (128 bytes).
@@ -28,7 +27,8 @@ struct __align__(16) TriRec {
     float bcx, bcy, bcz, ibc;  // c - b and 1/|c - b|^2
     float g1x, g1y, g1z, pad0; // dual basis: v = (p - a).g1, w = (p - a).g2
     float g2x, g2y, g2z, pad1;
-    float nx, ny, nz, pad2;    // unit normal (0 for a degenerate triangle)
+    float nx, ny, nz, t;       // unit normal and half-thickness of the bounding disk
+                               // (degenerate triangle: normal (0,0,1), t = r)
 };
 static_assert(sizeof(TriRec) == 128, "record size");
 
@@ -154,29 +154,48 @@ __device__ inline double dist2_to(const double* p, const double* q) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+// With bary given, also the barycentric weights (of a, b, c) of the closest point.
 __device__ double point_tri_dist2_ref(const double* p, const double* a, const double* b,
-                                      const double* c) {
+                                      const double* c, double* bary = nullptr) {
     double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
     double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
     double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
     double d1 = dot3d(ab, ap), d2 = dot3d(ac, ap);
-    if (d1 <= 0 && d2 <= 0) return dist2_to(p, a);
+    auto set = [&](double u, double v, double w) {
+        if (bary) {
+            bary[0] = u;
+            bary[1] = v;
+            bary[2] = w;
+        }
+    };
+    if (d1 <= 0 && d2 <= 0) {
+        set(1.0, 0.0, 0.0);
+        return dist2_to(p, a);
+    }
     double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
     double d3 = dot3d(ab, bp), d4 = dot3d(ac, bp);
-    if (d3 >= 0 && d4 <= d3) return dist2_to(p, b);
+    if (d3 >= 0 && d4 <= d3) {
+        set(0.0, 1.0, 0.0);
+        return dist2_to(p, b);
+    }
     double vc = d1 * d4 - d3 * d2;
     if (vc <= 0 && d1 >= 0 && d3 <= 0) {
         double v = d1 / (d1 - d3);
         double q[3] = {a[0] + v * ab[0], a[1] + v * ab[1], a[2] + v * ab[2]};
+        set(1.0 - v, v, 0.0);
         return dist2_to(p, q);
     }
     double cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
     double d5 = dot3d(ab, cp), d6 = dot3d(ac, cp);
-    if (d6 >= 0 && d5 <= d6) return dist2_to(p, c);
+    if (d6 >= 0 && d5 <= d6) {
+        set(0.0, 0.0, 1.0);
+        return dist2_to(p, c);
+    }
     double vb = d5 * d2 - d1 * d6;
     if (vb <= 0 && d2 >= 0 && d6 <= 0) {
         double w = d2 / (d2 - d6);
         double q[3] = {a[0] + w * ac[0], a[1] + w * ac[1], a[2] + w * ac[2]};
+        set(1.0 - w, 0.0, w);
         return dist2_to(p, q);
     }
     double va = d3 * d6 - d5 * d4;
@@ -184,12 +203,14 @@ __device__ double point_tri_dist2_ref(const double* p, const double* a, const do
         double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
         double q[3] = {b[0] + w * (c[0] - b[0]), b[1] + w * (c[1] - b[1]),
                        b[2] + w * (c[2] - b[2])};
+        set(0.0, 1.0 - w, w);
         return dist2_to(p, q);
     }
     double denom = 1.0 / (va + vb + vc);
     double v = vb * denom, w = vc * denom;
     double q[3] = {a[0] + ab[0] * v + ac[0] * w, a[1] + ab[1] * v + ac[1] * w,
                    a[2] + ab[2] * v + ac[2] * w};
+    set(1.0 - v - w, v, w);
     return dist2_to(p, q);
 }
 
@@ -287,17 +308,51 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     r.bcy = (float)BC[1];
     r.bcz = (float)BC[2];
     r.ibc = a33 > 0.0 ? (float)(1.0 / a33) : 0.0f;
+    // bounding disk: centre m, radius r, unit normal; its half-thickness covers the float
+    // rounding of the vertices and of the normal (a degenerate triangle gets a cylinder
+    // around its bounding sphere instead)
     if (degen) {
         r.g1x = r.g1y = r.g1z = r.g2x = r.g2y = r.g2z = 0.0f;
-        r.nx = r.ny = r.nz = 0.0f;
+        r.nx = r.ny = 0.0f;
+        r.nz = 1.0f;
+        r.t = r.r;
     } else {
         for (int a = 0; a < 3; ++a) {
             (&r.g1x)[a] = (float)((a22 * AB[a] - a12 * AC[a]) / det);
             (&r.g2x)[a] = (float)((a11 * AC[a] - a12 * AB[a]) / det);
             (&r.nx)[a] = (float)(N[a] / nn);
         }
+        r.t = (float)(8.0 * 0x1p-24 * (ext + rad));
     }
-    r.pad0 = r.pad1 = r.pad2 = 0.0f;
+    r.pad0 = r.pad1 = 0.0f;
+}
+
+// Squared distance from p to the solid cylinder with centre c, unit axis n, radius r and
+// half-height t (a disk when t is 0): a lower bound on the distance to anything inside.
+// The lateral distance comes from |n x v| (no cancellation).
+__device__ __forceinline__ float cyl_lb2(float cx, float cy, float cz, float nx, float ny,
+                                         float nz, float r, float t, float px, float py,
+                                         float pz) {
+    const float vx = px - cx, vy = py - cy, vz = pz - cz;
+    const float h = fma3(vx, nx, vy, ny, vz, nz);
+    const float wx = __builtin_fmaf(vy, nz, -vz * ny);
+    const float wy = __builtin_fmaf(vz, nx, -vx * nz);
+    const float wz = __builtin_fmaf(vx, ny, -vy * nx);
+    const float rho = sqrtf(fma3(wx, wx, wy, wy, wz, wz));
+    const float dh = fmaxf(fabsf(h) - t, 0.0f), dr = fmaxf(rho - r, 0.0f);
+    return __builtin_fmaf(dh, dh, dr * dr);
+}
+
+__device__ __forceinline__ float tri_lb2(const TriRec& T, float px, float py, float pz) {
+    return cyl_lb2(T.mx, T.my, T.mz, T.nx, T.ny, T.nz, T.r, T.t, px, py, pz);
+}
+
+// Distances at or below this are zero up to the FP64 rounding of the coordinates: such
+// a cell lies on the surface, where the unsigned distance has a kink, and carries no
+// gradient (2^-40 of the grid extent).
+inline double zero_distance(const GridSpec& g) {
+    const int m = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+    return std::ldexp(m * g.spacing, -40);
 }
 
 // FP32 error margin for distances measured from the grid corner: a few ulps of the

@@ -25,6 +25,8 @@
 // Compiled with -ffp-contract=off: every FP64 expression rounds as in the reference.
 #include <hip/hip_runtime.h>
 
+#include <memory>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -34,6 +36,7 @@
 #include "device_common.hpp"
 #include "expansion.hpp"
 #include "occ_device.hpp"
+#include "batch_internal.hpp"
 #include "voxelize_internal.hpp"
 
 namespace cmg::hip {
@@ -50,6 +53,8 @@ bool make_grid(const double* xyz, size_t npts, int resolution, int pad, GridSpec
     double ext = 0;
     for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
     g = GridSpec{};
+    g.resolution = resolution;
+    g.pad = pad;
     g.spacing = ext / resolution;
     if (!(g.spacing > 0)) return false;
     int* dims[3] = {&g.nx, &g.ny, &g.nz};
@@ -288,53 +293,89 @@ void launch_occupancy(const double* d_xyz, const int* d_tris, int ntris, const G
     }
 }
 
+namespace {
+
+// One mesh through the voxelization engine (device copies of the points and triangles
+// live in the arena; the engine holds them until it is destroyed, before the arena).
+struct SingleMesh {
+    DeviceArena arena;
+    double* d_xyz;
+    int* d_tris;
+    std::unique_ptr<BatchEngine> engine;
+
+    SingleMesh(const double* xyz, size_t npts, const int* tris, size_t ntris,
+               const GridSpec& g, hipStream_t st)
+        : arena(st) {
+        d_xyz = arena.alloc<double>(3 * npts);
+        d_tris = arena.alloc<int>(3 * ntris);
+        CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
+        CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
+        const long long ps[2] = {0, (long long)npts}, ts[2] = {0, (long long)ntris};
+        const int res = g.resolution, pad = g.pad;
+        engine = std::make_unique<BatchEngine>(1, ps, ts, &res, &pad, d_xyz, d_tris, st);
+        const GridSpec& e = engine->mesh(0).g;
+        if (e.nx != g.nx || e.ny != g.ny || e.nz != g.nz || e.spacing != g.spacing ||
+            e.corner[0] != g.corner[0] || e.corner[1] != g.corner[1] ||
+            e.corner[2] != g.corner[2])
+            throw HipError("voxelize: device grid differs from the host grid");
+    }
+    ~SingleMesh() {
+        (void)hipStreamSynchronize(arena.stream());
+        engine.reset();
+    }
+};
+
+}  // namespace
+
 void voxelize_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
                    const GridSpec& g, bool sdf, uint8_t* occ, float* dist) {
     const size_t ncell = (size_t)g.nx * g.ny * g.nz;
     hipStream_t st = nullptr;
-    DeviceArena arena(st);
-    double* d_xyz = arena.alloc<double>(3 * npts);
-    int* d_tris = arena.alloc<int>(3 * ntris);
-    uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
-    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
-    CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
-    launch_occupancy(d_xyz, d_tris, (int)ntris, g, d_occ, arena, st, nullptr);
-    CMG_HIP(hipMemcpyAsync(occ, d_occ, ncell, hipMemcpyDeviceToHost, st));
-    if (sdf) {
-        float* d_dist = arena.alloc<float>(ncell);
-        launch_sdf_forward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_dist, arena, st);
-        CMG_HIP(hipMemcpyAsync(dist, d_dist, sizeof(float) * ncell, hipMemcpyDeviceToHost, st));
+    if (!sdf) {
+        DeviceArena arena(st);
+        double* d_xyz = arena.alloc<double>(3 * npts);
+        int* d_tris = arena.alloc<int>(3 * ntris);
+        uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
+        CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
+        CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
+        launch_occupancy(d_xyz, d_tris, (int)ntris, g, d_occ, arena, st, nullptr);
+        CMG_HIP(hipMemcpyAsync(occ, d_occ, ncell, hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+        return;
     }
+    SingleMesh one(xyz, npts, tris, ntris, g, st);
+    float* d_dist = one.arena.alloc<float>(ncell);
+    one.engine->run(kBatchForward, one.d_xyz, d_dist, nullptr, nullptr, st);
+    CMG_HIP(hipMemcpyAsync(occ, one.engine->occupancy(), ncell, hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipMemcpyAsync(dist, d_dist, sizeof(float) * ncell, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
 }
 
 void sdf_backward_host(const double* xyz, size_t npts, const int* tris, size_t ntris,
                        const GridSpec& g, const uint8_t* occ, const float* grad_sdf,
                        float* grad_xyz, int* nearest) {
+    // the occupancy of the grid is recomputed by the forward (it is a function of the
+    // same points and triangles)
+    (void)occ;
     const size_t ncell = (size_t)g.nx * g.ny * g.nz;
     const bool adj = grad_sdf && grad_xyz;
     hipStream_t st = nullptr;
-    DeviceArena arena(st);
-    double* d_xyz = arena.alloc<double>(3 * npts);
-    int* d_tris = arena.alloc<int>(3 * ntris);
-    uint8_t* d_occ = arena.alloc<uint8_t>(ncell);
-    float* d_g = adj ? arena.alloc<float>(ncell) : nullptr;
-    float* d_grad = adj ? arena.alloc<float>(3 * npts) : nullptr;
-    int* d_near = nearest ? arena.alloc<int>(ncell) : nullptr;
-    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, st));
-    CMG_HIP(hipMemcpyAsync(d_tris, tris, sizeof(int) * 3 * ntris, hipMemcpyHostToDevice, st));
-    CMG_HIP(hipMemcpyAsync(d_occ, occ, ncell, hipMemcpyHostToDevice, st));
-    if (adj) {
+    SingleMesh one(xyz, npts, tris, ntris, g, st);
+    float* d_dist = one.arena.alloc<float>(ncell);
+    float* d_g = adj ? one.arena.alloc<float>(ncell) : nullptr;
+    float* d_grad = adj ? one.arena.alloc<float>(3 * npts) : nullptr;
+    if (adj)
         CMG_HIP(hipMemcpyAsync(d_g, grad_sdf, sizeof(float) * ncell, hipMemcpyHostToDevice, st));
-        CMG_HIP(hipMemsetAsync(d_grad, 0, sizeof(float) * 3 * npts, st));
-    }
-    launch_sdf_backward(d_xyz, (int)npts, d_tris, (int)ntris, g, d_occ, d_g, d_grad, d_near,
-                        arena, st);
+    one.engine->run(kBatchForward | (adj ? kBatchBackward : 0), one.d_xyz, d_dist, d_g, d_grad,
+                    st);
     if (adj)
         CMG_HIP(hipMemcpyAsync(grad_xyz, d_grad, sizeof(float) * 3 * npts,
                                hipMemcpyDeviceToHost, st));
-    if (nearest)
+    if (nearest) {
+        int* d_near = one.arena.alloc<int>(ncell);
+        one.engine->nearest(d_near, st);
         CMG_HIP(hipMemcpyAsync(nearest, d_near, sizeof(int) * ncell, hipMemcpyDeviceToHost, st));
+    }
     CMG_HIP(hipStreamSynchronize(st));
 }
 

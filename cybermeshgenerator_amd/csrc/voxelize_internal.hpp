@@ -65,15 +65,4 @@ void launch_occupancy(const double* d_xyz, const int* d_tris, int ntris, const G
                       uint8_t* d_occ, DeviceArena& arena, hipStream_t st,
                       unsigned* h_stats);
 
-// Signed distance of one mesh into d_dist, signs from d_occ (1 = inside).
-void launch_sdf_forward(const double* d_xyz, int npts, const int* d_tris, int ntris,
-                        const GridSpec& g, const uint8_t* d_occ, float* d_dist,
-                        DeviceArena& arena, hipStream_t st);
-
-// Adjoint of the SDF magnitude: accumulates dL/dxyz (float, 3 per point) into d_grad
-// for the upstream gradient d_gsdf, signs from d_occ held constant.
-void launch_sdf_backward(const double* d_xyz, int npts, const int* d_tris, int ntris,
-                         const GridSpec& g, const uint8_t* d_occ, const float* d_gsdf,
-                         float* d_grad, int* d_nearest, DeviceArena& arena, hipStream_t st);
-
 }  // namespace cmg::hip

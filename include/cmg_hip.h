@@ -100,7 +100,7 @@ cmg_status cmg_insphere_signs(const double* xyz, size_t npts, const int* quints,
 /* SDF adjoint for a grid produced by cmg_plc_voxelize(mode=1) from `plc`: given
  * grad_sdf (dL/dSDF per cell, flat idx = (k*ny + j)*nx + i), writes dL/dxyz
  * (3 floats per PLC point, anchors included) to grad_xyz. The sign is held constant;
- * cells at distance 0 carry no gradient. */
+ * cells on the surface (distance at most 2^-40 of the grid extent) carry no gradient. */
 cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,
                                    const float* grad_sdf, float* grad_xyz, char* errbuf,
                                    size_t errbuf_len);
@@ -179,8 +179,10 @@ enum { CMG_BATCH_FORWARD = 1, CMG_BATCH_BACKWARD = 2, CMG_BATCH_GRAPH = 4 };
  * [tri_off[m], tri_off[m+1]) of tris (3 ints each, indices local to the mesh's points).
  * Grid m is make_grid(points of m, resolution[m], pad[m]) exactly as cmg_plc_voxelize
  * builds it (voxelize.cpp:52-74). point_off/tri_off/resolution/pad are host arrays.
- * Planning synchronises `stream` (it sizes the candidate-list pools); xyz and tris must
- * stay valid and at the same address for the batch's lifetime. */
+ * Planning synchronises `stream` (it orders each mesh's triangles for the nearest-
+ * triangle search and sizes the column-tile pool); xyz and tris must stay valid and at
+ * the same address for the batch's lifetime (the point values may change between steps:
+ * every step rebuilds the search bounds from them). */
 cmg_status cmg_voxelize_batch_create(int nmesh, const int64_t* point_off,
                                      const int64_t* tri_off, const int* resolution,
                                      const int* pad, const double* xyz, const int* tris,
@@ -204,31 +206,44 @@ cmg_status cmg_voxelize_batch_run(cmg_batch*, int flags, const double* xyz, floa
                                   const float* grad_sdf, float* grad_xyz, void* stream,
                                   char* errbuf, size_t errbuf_len);
 
-/* Synchronises `stream`; overflow != 0 means a candidate-list pool was too small for the
+/* Synchronises `stream`; overflow != 0 means the column-tile pool was too small for the
  * last step (results incomplete: create the batch again), pairs counts the FP32
- * point-triangle evaluations of the last forward and backward passes. */
+ * point-triangle evaluations (per wavefront of 64 lanes) of the last passes. */
 cmg_status cmg_voxelize_batch_status(cmg_batch*, void* stream, unsigned* overflow,
                                      uint64_t* pairs, char* errbuf, size_t errbuf_len);
 
 /* Autotuner: times every voxel-pass variant on this batch — voxels per lane (1: 4x4x4
- * bricks per wavefront, 2: 4x4x8) x LDS tile depth (0: triangle records read with scalar
- * loads, 32 or 64: staged through LDS) — with HIP events on `stream` (forward + backward
- * voxel passes, buffers as in cmg_voxelize_batch_run) and keeps the fastest. ms (optional,
- * 6 entries) receives the timings in the order (1,0) (1,32) (1,64) (2,0) (2,32) (2,64). */
+ * voxels per wavefront, 2: 4x4x8) x LDS tile depth of a search leaf's triangle records
+ * (0: read with scalar loads, 8: staged through LDS as one tile) — with HIP events on
+ * `stream` (forward + backward voxel passes, buffers as in cmg_voxelize_batch_run) and
+ * keeps the fastest. ms (optional, 4 entries) receives the timings in the order (1,0)
+ * (1,8) (2,0) (2,8). */
 cmg_status cmg_voxelize_batch_autotune(cmg_batch*, float* sdf, const float* grad_sdf,
                                        float* grad_xyz, void* stream, int* vpl, int* depth,
                                        float* ms, char* errbuf, size_t errbuf_len);
 
-/* Selects a voxel-pass variant explicitly (vpl 1 or 2, depth 0, 32 or 64). */
+/* Selects a voxel-pass variant explicitly (vpl 1 or 2, depth 0 or 8). */
 cmg_status cmg_voxelize_batch_set_variant(cmg_batch*, int vpl, int depth);
 
 /* Device pointer to the occupancy bytes of every grid (valid after a forward). */
 const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch*);
 
+/* After a forward: for every cell (device, cells ints, same layout as sdf) the
+ * mesh-local index of the triangle whose closest point the adjoint uses, -1 where there
+ * is none. Stream-ordered. */
+cmg_status cmg_voxelize_batch_nearest(cmg_batch*, int32_t* tri, void* stream,
+                                      char* errbuf, size_t errbuf_len);
+
+/* Synchronises `stream` and reads the work counters of the last voxel passes, per
+ * wavefront: out[0] cylinder tests of search-tree nodes, out[1] bounding-disk tests of
+ * triangles, out[2] FP32 point-triangle distance evaluations, out[3] overflow flags. */
+cmg_status cmg_voxelize_batch_counters(cmg_batch*, void* stream, uint64_t* out,
+                                       char* errbuf, size_t errbuf_len);
+
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
- * "sdf_voxels", "sdf_backward", "occupancy_columns", "batch_sdf"). Events are recorded on the stream
- * the kernel is launched on. */
+ * "occupancy_columns", "batch_sdf", "batch_sdf_backward"). Events are recorded on the
+ * stream the kernel is launched on; launches recorded into a graph are not timed. */
 void cmg_hip_kernel_timer(const char* name, int enable);
 
 /* Total milliseconds and launch count of the timed launches of `name` since the last

@@ -64,7 +64,7 @@ def _load() -> ctypes.CDLL:
     lib.orc_sdf_grad.argtypes = [P, sz, P, sz, P, P, P, P, P, i32]
     lib.orc_sdf_dist2_to.argtypes = [P, P, P, P, P, i32]
     lib.orc_sdf_fd.restype = d
-    lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, P, P, i32, i32, d, i32]
+    lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, P, P, i32, i32, d, P, i32]
     return lib
 
 
@@ -283,9 +283,12 @@ def sdf_dist2_to(pts, tris, grid: Grid, arg, nthreads=0) -> np.ndarray:
 
 
 def sdf_fd(pts, tris, grid: Grid, occ, gout, vert: int, axis: int, h=1e-5,
-           nthreads=0, base=None) -> float:
+           nthreads=0, base=None, with_ties=False):
     """Central difference of L = sum gout * sdf w.r.t. pts[vert, axis]; base =
-    (best, arg) from sdf_bvh lets many calls share the unperturbed search."""
+    (best, arg) from sdf_bvh lets many calls share the unperturbed search. with_ties:
+    returns (fd, tie_bound), tie_bound = sum over the cells whose nearest triangle
+    differs between the samples -h, 0, +h of |g| times the spread of those triangles'
+    derivatives."""
     pts = _f64(pts).reshape(-1, 3)
     tris = _i32(tris).reshape(-1, 3)
     occ = np.ascontiguousarray(occ, dtype=np.uint8)
@@ -296,9 +299,11 @@ def sdf_fd(pts, tris, grid: Grid, occ, gout, vert: int, axis: int, h=1e-5,
         best = np.ascontiguousarray(base[0], dtype=np.float64)
         arg = np.ascontiguousarray(base[1], dtype=np.int32)
         bb, ba = _p(best), _p(arg)
-    return lib().orc_sdf_fd(_p(pts), pts.shape[0], _p(tris), tris.shape[0],
-                            ctypes.byref(g), _p(occ), _p(gout), bb, ba, vert, axis, h,
-                            nthreads)
+    ties = ctypes.c_double(0.0)
+    fd = lib().orc_sdf_fd(_p(pts), pts.shape[0], _p(tris), tris.shape[0],
+                          ctypes.byref(g), _p(occ), _p(gout), bb, ba, vert, axis, h,
+                          ctypes.byref(ties), nthreads)
+    return (fd, ties.value) if with_ties else fd
 
 
 def voxelize(pts, polygons_or_tris, resolution=64, pad=1, mode="occupancy", nthreads=0):

@@ -117,7 +117,8 @@ void orc_sdf_bvh(const double* pts, const int* tris, size_t ntris, const orc_gri
  * (+1 outside, -1 inside, from occ) held constant, accumulates dL/dxyz into
  * grad[3*npts] (zeroed first). The closest feature is the closest point of the
  * minimising triangle, or of triangle arg[cell] when arg is given (-1: no
- * contribution); d = 0 carries zero gradient. */
+ * contribution); a cell within 2^-40 of the grid extent of the surface (on it up to
+ * rounding, where the unsigned distance has a kink) carries zero gradient. */
 void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const orc_grid* g, const uint8_t* occ, const float* gout,
                   const int32_t* arg, double* grad, int nthreads);
@@ -130,11 +131,14 @@ void orc_sdf_dist2_to(const double* pts, const int* tris, const orc_grid* g,
 /* FP64 central finite difference of the same L with respect to coordinate `axis` of
  * vertex `vert`, step h, signs held at their unperturbed values. base_best/base_arg
  * (optional, from orc_sdf_bvh) are the unperturbed squared distances and minimising
- * triangles, so that many differences can share one search. */
+ * triangles, so that many differences can share one search. tie_bound (optional)
+ * receives, over the cells whose nearest triangle differs between the samples -h, 0
+ * and +h, the sum of |g| times the spread of those triangles' derivatives: how far the
+ * difference quotient and the adjoint may legitimately differ. */
 double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const orc_grid* g, const uint8_t* occ, const float* gout,
                   const double* base_best, const int32_t* base_arg, int vert, int axis,
-                  double h, int nthreads);
+                  double h, double* tie_bound, int nthreads);
 
 #ifdef __cplusplus
 }

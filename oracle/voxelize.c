@@ -610,6 +610,9 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const int32_t* arg, double* grad, int nthreads) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t ncell = (size_t)nx * ny * nz;
+    /* cells within 2^-40 of the grid extent of the surface are on it up to rounding */
+    const int mdim = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
+    const double dzero = ldexp(mdim * g->spacing, -40);
     bvh_t b = bvh_build(pts, tris, ntris);
     memset(grad, 0, sizeof(double) * 3 * npts);
     int nt = NT(nthreads);
@@ -633,7 +636,7 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
             double w[3];
             double d2 = orc_point_tri_dist2(p, pts + 3 * (size_t)tv[0],
                                             pts + 3 * (size_t)tv[1], pts + 3 * (size_t)tv[2], w);
-            if (!(d2 > 0)) continue;
+            if (!(sqrt(d2) > dzero)) continue;
             double q[3];
             for (int a = 0; a < 3; ++a)
                 q[a] = w[0] * pts[3 * (size_t)tv[0] + a] + w[1] * pts[3 * (size_t)tv[1] + a] +
@@ -656,7 +659,7 @@ void orc_sdf_grad(const double* pts, size_t npts, const int* tris, size_t ntris,
 double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const orc_grid* g, const uint8_t* occ, const float* gout,
                   const double* base_best, const int32_t* base_arg, int vert, int axis,
-                  double h, int nthreads) {
+                  double h, double* tie_bound, int nthreads) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t ncell = (size_t)nx * ny * nz;
     bvh_t b = bvh_build(pts, tris, ntris);
@@ -683,8 +686,8 @@ double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
     bvh_node box;
     memcpy(box.lo, lo, sizeof lo);
     memcpy(box.hi, hi, sizeof hi);
-    double total = 0;
-#pragma omp parallel for schedule(dynamic, 256) num_threads(NT(nthreads)) reduction(+ : total)
+    double total = 0, ties = 0;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(NT(nthreads)) reduction(+ : total, ties)
     for (size_t idx = 0; idx < ncell; ++idx) {
         if (gout[idx] == 0.0f) continue;
         int k = (int)(idx / ((size_t)nx * ny));
@@ -703,28 +706,67 @@ double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
         if (ninc == 0) continue;
         if (box_d2(&box, p) > base * (1.0 + 1e-6) && !(t0 >= 0 && inc[t0])) continue;
         double others = bvh_nearest(&b, pts, tris, p, inc, NULL);
+        const double ro = sqrt(others);
+        /* incident distances at +h, -h and 0, and each incident triangle's derivative
+         * -w_vert (p - q)_axis / d at 0 (a non-incident triangle's is 0) */
+        double dist[3][64], der[64];
+        int nc = 0;
+        for (int q = 0; q < ninc && q < 64; ++q) nc = q + 1;
         double dpm[2];
-        for (int s = 0; s < 2; ++s) {
-            double m = others;
+        for (int s = 0; s < 3; ++s) {
             double moved[3];
             for (int a = 0; a < 3; ++a) moved[a] = pts[3 * (size_t)vert + a];
-            moved[axis] += s == 0 ? h : -h;
-            for (int q = 0; q < ninc; ++q) {
+            moved[axis] += s == 0 ? h : (s == 1 ? -h : 0.0);
+            double m = others;
+            for (int q = 0; q < nc; ++q) {
                 const int* tv = tris + 3 * (size_t)inc_list[q];
                 const double* v3[3];
                 for (int v = 0; v < 3; ++v)
                     v3[v] = tv[v] == vert ? moved : pts + 3 * (size_t)tv[v];
-                m = keep_min(m, orc_point_tri_dist2(p, v3[0], v3[1], v3[2], NULL));
+                double w[3];
+                const double d2 = orc_point_tri_dist2(p, v3[0], v3[1], v3[2], w);
+                m = keep_min(m, d2);
+                dist[s][q] = sqrt(d2);
+                if (s == 2) {
+                    double wv = 0.0, qa = 0.0;
+                    for (int v = 0; v < 3; ++v) {
+                        if (tv[v] == vert) wv = w[v];
+                        qa += w[v] * v3[v][axis];
+                    }
+                    der[q] = dist[2][q] > 0.0 ? -wv * (p[axis] - qa) / dist[2][q] : 0.0;
+                }
             }
-            dpm[s] = sqrt(m);
+            if (s < 2) dpm[s] = sqrt(m);
         }
         double sgnv = occ[idx] ? -1.0 : 1.0;
         total += (double)gout[idx] * sgnv * (dpm[0] - dpm[1]);
+        /* Tie allowance: the candidates that are nearest at -h, 0 or +h. When more than
+         * one is (the nearest triangle changes within the step), the difference quotient
+         * mixes their derivatives while the adjoint takes the derivative of one of them:
+         * the two differ by at most |g| times the spread of those derivatives (equal
+         * derivatives, e.g. of triangles sharing the closest point, add nothing). */
+        double lo_d = DBL_MAX, hi_d = -DBL_MAX;
+        for (int s = 0; s < 3; ++s) {
+            double mn = ro;
+            for (int q = 0; q < nc; ++q) mn = dist[s][q] < mn ? dist[s][q] : mn;
+            const double cut = mn * (1.0 + 1e-12);
+            if (ro <= cut) {
+                lo_d = lo_d < 0.0 ? lo_d : 0.0;
+                hi_d = hi_d > 0.0 ? hi_d : 0.0;
+            }
+            for (int q = 0; q < nc; ++q)
+                if (dist[s][q] <= cut) {
+                    if (der[q] < lo_d) lo_d = der[q];
+                    if (der[q] > hi_d) hi_d = der[q];
+                }
+        }
+        if (hi_d > lo_d) ties += fabs((double)gout[idx]) * (hi_d - lo_d);
     }
     free(inc);
     free(inc_list);
     bvh_free(&b);
     (void)npts;
+    if (tie_bound) *tie_bound = ties;
     return total / (2.0 * h);
 }
 

@@ -116,3 +116,13 @@ def frac_insphere(a, b, c, d, e) -> int:
     v = det4(rows)
     return (v > 0) - (v < 0)
 
+
+def box_on_cell_centres():
+    """Box [1/8, 7/8] x [1/8, 5/8] x [1/8, 3/8] plus anchor points (0,0,0) and (1,1,1):
+    on a 4^3 grid over [0, 1]^3 the cell centres (1/8 + k/4) lie exactly on its faces,
+    edges and corners, where the distance is 0."""
+    pts, tris = brick(0.75, 0.5, 0.25)
+    pts = pts + 0.125
+    pts = np.vstack([pts, [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0]]])
+    return pts, tris
+
