@@ -172,7 +172,7 @@ class SdfBatch:
         return sdf, grad
 
     def autotune(self, grad_sdf, sdf=None, grad=None, stream=None) -> dict:
-        """Times the four voxel-pass variants on this batch and keeps the fastest; returns
+        """Times the six voxel-pass variants on this batch and keeps the fastest; returns
         {"vpl": .., "depth": .., "ms": {(vpl, depth): ms}}."""
         import torch
         if sdf is None:
@@ -181,13 +181,13 @@ class SdfBatch:
             grad = torch.empty((self.points.shape[0], 3), dtype=torch.float32,
                                device=self.device)
         vpl, depth = _i(), _i()
-        ms = np.zeros(4, dtype=np.float32)
+        ms = np.zeros(6, dtype=np.float32)
         err = _err()
         _check(lib().cmg_voxelize_batch_autotune(
             self._h, sdf.data_ptr(), grad_sdf.contiguous().data_ptr(), grad.data_ptr(),
             _stream(stream), ctypes.byref(vpl), ctypes.byref(depth), ms.ctypes.data_as(_P),
             err, len(err)), err)
-        keys = [(1, 0), (1, 8), (2, 0), (2, 8)]
+        keys = [(1, 0), (1, 8), (2, 0), (2, 8), (4, 0), (4, 8)]
         self._variant = (vpl.value, depth.value)
         return {"vpl": vpl.value, "depth": depth.value,
                 "ms": {f"{v}x{d}": float(t) for (v, d), t in zip(keys, ms)}}

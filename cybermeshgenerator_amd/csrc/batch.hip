@@ -44,6 +44,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "batch_internal.hpp"
@@ -513,8 +515,7 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
 
 // ---- voxel pass ---------------------------------------------------------------------------
 
-// One voxel leaf per item: 4x4x4 voxels (VPL = 1 voxel per lane) or 4x4x8 (VPL = 2, the
-// second voxel of a lane 4 cells above the first). D = 0 reads a tree leaf's triangle
+// One voxel leaf per item: 4x4x(4 VPL) voxels, VPL per lane, 4 cells apart along z. D = 0 reads a tree leaf's triangle
 // records with scalar loads (SGPR operands); D = 8 stages the leaf's 8 records through LDS
 // as one 1 KiB tile. BWD: the adjoint (and, when C.nearest is set, the nearest triangle
 // of every cell) instead of the distance.
@@ -522,7 +523,7 @@ template <bool BWD, int VPL, int D>
 __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
     __shared__ int s_stack[kStack];
     __shared__ TriRec s_rec[D > 0 ? D : 1];
-    const long long* off = phase_off(C, VPL == 1 ? kPhL0 : kPhL0b);
+    const long long* off = phase_off(C, VPL == 1 ? kPhL0 : (VPL == 2 ? kPhL0b : kPhL0c));
     const long long total = off[C.nm];
     const int lane = threadIdx.x;
     unsigned long long* q = &C.ctr->queue[BWD ? 2 : 1];
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         const int pos = find_pos(off, C.nm, item);
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec& g = M.g;
-        const Level L0 = VPL == 1 ? M.L0 : M.L0b;
+        const Level L0 = VPL == 1 ? M.L0 : (VPL == 2 ? M.L0b : M.L0c);
         const int b = (int)(item - off[pos]);
         const int ix = b % L0.bx, iy = (b / L0.bx) % L0.by, iz = b / (L0.bx * L0.by);
         const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3);
@@ -744,7 +745,9 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             continue;
         }
         // ---- adjoint: d = |p - q| with q the closest point, dd/dv_m = -w_m (p - q) / d
-        for (int v = 0; v < VPL; ++v) {
+        // (one body per voxel of the lane, each with compile-time indices)
+        auto adjoint = [&](auto vc) {
+            constexpr int v = decltype(vc)::value;
             float c[9];
             bool act = false;
             const size_t idx = valid[v] ? ((size_t)k[v] * g.ny + j) * g.nx + i : 0;
@@ -812,7 +815,10 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 }
                 act = act && !match;
             }
-        }
+        };
+        [&]<int... V>(std::integer_sequence<int, V...>) {
+            (adjoint(std::integral_constant<int, V>{}), ...);
+        }(std::make_integer_sequence<int, VPL>{});
     }
     if (lane == 0) {
         if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
@@ -923,15 +929,26 @@ struct BatchEngine::Impl {
     void launch_backward(hipStream_t st);
     void launch_tree(hipStream_t st);
 
+    // Persistent voxel passes get as many one-wave workgroups as fit on the device at
+    // once (the tree walk is latency bound: every resident wave hides some of it).
+    template <class K>
+    unsigned resident_grid(K kernel) {
+        int per_cu = 0;
+        CMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0));
+        return (unsigned)cu_count() * (unsigned)std::max(per_cu, 1);
+    }
+
     template <bool BWD>
     void launch_vox(hipStream_t st) {
 #define CMG_VOX(V, DD)                                                 \
     if (vpl == V && depth == DD) {                                    \
-        vox_kernel<BWD, V, DD><<<grid64, 64, 0, st>>>(ctx);            \
+        static unsigned grid = 0;                                      \
+        if (!grid) grid = resident_grid(vox_kernel<BWD, V, DD>);       \
+        vox_kernel<BWD, V, DD><<<grid, 64, 0, st>>>(ctx);              \
         CMG_HIP_LAUNCH();                                              \
         return;                                                        \
     }
-        CMG_VOX(1, 0) CMG_VOX(1, 8) CMG_VOX(2, 0) CMG_VOX(2, 8)
+        CMG_VOX(1, 0) CMG_VOX(1, 8) CMG_VOX(2, 0) CMG_VOX(2, 8) CMG_VOX(4, 0) CMG_VOX(4, 8)
 #undef CMG_VOX
         throw HipError("batch: unknown voxel-pass variant");
     }
@@ -1040,6 +1057,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.mask_base = words;
         M.L0 = make_level(g, kLeaf);
         M.L0b = make_level(g, kLeaf, 2 * kLeaf);
+        M.L0c = make_level(g, kLeaf, 4 * kLeaf);
         M.eps = distance_eps(g);
         M.dzero = zero_distance(g);
         // tree levels: level l has ceil(ntris / 8^l) nodes, up to the single root
@@ -1092,6 +1110,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             if (ph == kPhTile) n = (long long)M.ntx * M.nty;
             if (ph == kPhL0) n = (long long)M.L0.bx * M.L0.by * M.L0.bz;
             if (ph == kPhL0b) n = (long long)M.L0b.bx * M.L0b.by * M.L0b.bz;
+            if (ph == kPhL0c) n = (long long)M.L0c.bx * M.L0c.by * M.L0c.bz;
             o[p + 1] = o[p] + n;
         }
     }
@@ -1253,7 +1272,7 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
     CMG_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
     int best_v = I.vpl, best_d = I.depth;
-    for (int v : {1, 2})
+    for (int v : {1, 2, 4})
         for (int d : {0, 8}) {
             I.vpl = v;
             I.depth = d;
@@ -1285,8 +1304,8 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
 }
 
 void BatchEngine::set_variant(int vpl, int depth) {
-    if (!((vpl == 1 || vpl == 2) && (depth == 0 || depth == 8)))
-        throw InvalidInput("batch: variant must have vpl in {1,2} and depth in {0,8}");
+    if (!((vpl == 1 || vpl == 2 || vpl == 4) && (depth == 0 || depth == 8)))
+        throw InvalidInput("batch: variant must have vpl in {1,2,4} and depth in {0,8}");
     Impl& I = *p_;
     if (I.vpl != vpl || I.depth != depth) {
         I.vpl = vpl;

@@ -34,14 +34,14 @@ struct MeshDesc {
     long long mask_base;  // first occupancy bit-mask word
     int words, ntx, nty;  // occupancy: mask words per column, column tiles per axis
     int tile_base;        // first column tile in the global tile arrays
-    sdfdev::Level L0, L0b;         // voxel leaves: 4x4x4 and 4x4x8
+    sdfdev::Level L0, L0b, L0c;    // voxel leaves: 4x4x4, 4x4x8 and 4x4x16
     int node_base[kMaxLevels + 1]; // first node of each level (1..nlev) in the node array
     int nnodes[kMaxLevels + 1];    // nodes per level (level 0: triangles)
     float eps;            // FP32 distance error margin for this grid
     double dzero;         // distances at or below this carry no gradient
 };
 
-enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kNumPhase = 3 };
+enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };
 
 struct BatchCounters {
     unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 backward
@@ -55,9 +55,9 @@ struct BatchCounters {
 
 enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4 };
 
-// A voxel-pass variant: voxels per lane (1: 4x4x4 leaves, 2: 4x4x8) and the LDS tile
-// depth of a tree leaf's triangle records (0: read with scalar loads, 8: staged through
-// LDS as one 1 KiB tile).
+// A voxel-pass variant: voxels per lane (1: 4x4x4 leaves, 2: 4x4x8, 4: 4x4x16) and the
+// LDS tile depth of a tree leaf's triangle records (0: read with scalar loads, 8: staged
+// through LDS as one 1 KiB tile).
 struct BatchVariant {
     int vpl, depth;
     float ms;  // forward + backward voxel passes, when measured by autotune

@@ -476,7 +476,7 @@ cmg_status cmg_voxelize_batch_autotune(cmg_batch* b, float* sdf, const float* gr
     return guarded(errbuf, len, [&] {
         auto all = b->engine->autotune(grad_sdf, sdf, grad_xyz, as_stream(stream));
         if (ms)
-            for (size_t i = 0; i < all.size() && i < 4; ++i) ms[i] = all[i].ms;
+            for (size_t i = 0; i < all.size() && i < 6; ++i) ms[i] = all[i].ms;
         cmg::hip::BatchVariant v = b->engine->variant();
         if (vpl) *vpl = v.vpl;
         if (depth) *depth = v.depth;

@@ -36,6 +36,7 @@ struct PredWorkspace {
     uint32_t* wave_64 = nullptr;  // per-wave FP64-decided count
     uint32_t* wave_x = nullptr;   // per-wave count decided by the inline exact stage
     uint32_t* queue = nullptr;    // nwaves segments of seg entries
+    uint32_t* qpref = nullptr;    // exclusive prefix of wave_q (nwaves + 1 entries)
     uint32_t* wide = nullptr;     // wide-capacity queue (n entries)
     unsigned seg = 0, nwaves = 0;
 };

@@ -476,19 +476,54 @@ __global__ __launch_bounds__(256) void filter_kernel(Src src, size_t n,
     }
 }
 
-// Exact pass. Pass 1 (WIDE = 0): block b walks queue segment b (and every gridDim.x-th
-// after it) with small private arrays; lanes that outgrow them are appended to the wide
-// queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide arrays;
-// a lane that still overflows is counted in ctr->failed and left at 0.
+// Exclusive prefix of the per-wave queue counts (one workgroup of 1024), so that the
+// exact pass can give every queued tuple its own lane wherever its wave put it.
+__global__ __launch_bounds__(1024) void queue_scan_kernel(PredWorkspace ws) {
+    __shared__ uint32_t part[1024];
+    const unsigned per = (ws.nwaves + 1023) / 1024;
+    const unsigned b = threadIdx.x * per, e = min(b + per, ws.nwaves);
+    uint32_t sum = 0;
+    for (unsigned w = b; w < e; ++w) sum += ws.wave_q[w];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (unsigned o = 1; o < 1024; o <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t v = threadIdx.x >= o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (unsigned w = b; w < e; ++w) {
+        ws.qpref[w] = run;
+        run += ws.wave_q[w];
+    }
+    if (threadIdx.x == 1023) ws.qpref[ws.nwaves] = part[1023];
+}
+
+// Queued tuple k (0 <= k < qpref[nwaves]) -> its index in the input.
+__device__ __forceinline__ uint32_t queued(const PredWorkspace& ws, uint32_t k) {
+    unsigned lo = 0, hi = ws.nwaves;  // qpref[lo] <= k < qpref[hi]
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (ws.qpref[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    return ws.queue[(size_t)lo * ws.seg + (k - ws.qpref[lo])];
+}
+
+// Exact pass. Pass 1 (WIDE = 0): one lane per queued tuple (through the prefix of the
+// per-wave counts), with small private arrays; lanes that outgrow them are appended to
+// the wide queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide
+// arrays; a lane that still overflows is counted in ctr->failed and left at 0.
 template <int NPTS, bool WIDE, class Src>
 __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__ out,
                                                    PredWorkspace ws) {
     if (!WIDE) {
-        for (unsigned sgi = blockIdx.x; sgi < ws.nwaves; sgi += gridDim.x) {
-            const unsigned count = ws.wave_q[sgi];
-            const uint32_t* q = ws.queue + (size_t)sgi * ws.seg;
-            for (unsigned k = threadIdx.x; k < count; k += blockDim.x) {
-                uint32_t i = q[k];
+        const uint32_t total = ws.qpref[ws.nwaves];
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+            {
+                const uint32_t i = queued(ws, k);
                 double v[NPTS * 3];
                 src.load(i, v);
                 int s = NPTS == 4 ? orient3d_exact_sign<40, false>(v, v + 3, v + 6, v + 9)
@@ -554,11 +589,11 @@ template <class Src>
 __global__ __launch_bounds__(64) void o3_values_exact_kernel(Src src,
                                                              double* __restrict__ out,
                                                              PredWorkspace ws) {
-    for (unsigned sgi = blockIdx.x; sgi < ws.nwaves; sgi += gridDim.x) {
-        const unsigned count = ws.wave_q[sgi];
-        const uint32_t* q = ws.queue + (size_t)sgi * ws.seg;
-        for (unsigned k = threadIdx.x; k < count; k += blockDim.x) {
-            uint32_t i = q[k];
+    const uint32_t total = ws.qpref[ws.nwaves];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+        {
+            const uint32_t i = queued(ws, k);
             double v[12];
             src.load(i, v);
             double top = 0.0;
@@ -609,8 +644,10 @@ __global__ void zero_counters_kernel(PredCounters* c) {
 
 template <int NPTS, class Src>
 void launch_exact(Src src, int8_t* out, const PredWorkspace& ws, hipStream_t stream) {
-    unsigned g = ws.nwaves < (unsigned)cu_count() * 8 ? ws.nwaves : (unsigned)cu_count() * 8;
-    exact_kernel<NPTS, false, Src><<<g ? g : 1, 64, 0, stream>>>(src, out, ws);
+    queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
+    CMG_HIP_LAUNCH();
+    const unsigned g = (unsigned)cu_count() * 8;
+    exact_kernel<NPTS, false, Src><<<g, 64, 0, stream>>>(src, out, ws);
     CMG_HIP_LAUNCH();
     unsigned gw = cu_count() / 4 > 0 ? cu_count() / 4 : 1;
     exact_kernel<NPTS, true, Src><<<gw, 64, 0, stream>>>(src, out, ws);
@@ -633,7 +670,8 @@ PredPlan pred_plan(size_t items, unsigned per_thread) {
 
 size_t pred_workspace_bytes(const PredPlan& p, size_t n) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return 256 + 4 * al((size_t)p.nwaves * 4) + al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
+    return 256 + 4 * al((size_t)p.nwaves * 4) + al(((size_t)p.nwaves + 1) * 4) +
+           al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
 }
 
 PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
@@ -650,6 +688,8 @@ PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
     c += al((size_t)p.nwaves * 4);
     ws.wave_x = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * 4);
+    ws.qpref = reinterpret_cast<uint32_t*>(c);
+    c += al(((size_t)p.nwaves + 1) * 4);
     ws.queue = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * p.seg * 4);
     ws.wide = reinterpret_cast<uint32_t*>(c);
@@ -728,8 +768,10 @@ void launch_o3_values_indexed(const double* xyz, const int* quads, size_t n, dou
     o3_values_kernel<IndexedSource<4>>
         <<<p.grid, p.block, 0, stream>>>(src, n, out, ws, p.iters);
     CMG_HIP_LAUNCH();
-    unsigned g = ws.nwaves < (unsigned)cu_count() * 8 ? ws.nwaves : (unsigned)cu_count() * 8;
-    o3_values_exact_kernel<IndexedSource<4>><<<g ? g : 1, 64, 0, stream>>>(src, out, ws);
+    queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
+    CMG_HIP_LAUNCH();
+    const unsigned g = (unsigned)cu_count() * 8;
+    o3_values_exact_kernel<IndexedSource<4>><<<g, 64, 0, stream>>>(src, out, ws);
     CMG_HIP_LAUNCH();
 }
 

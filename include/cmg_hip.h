@@ -213,16 +213,16 @@ cmg_status cmg_voxelize_batch_status(cmg_batch*, void* stream, unsigned* overflo
                                      uint64_t* pairs, char* errbuf, size_t errbuf_len);
 
 /* Autotuner: times every voxel-pass variant on this batch — voxels per lane (1: 4x4x4
- * voxels per wavefront, 2: 4x4x8) x LDS tile depth of a search leaf's triangle records
- * (0: read with scalar loads, 8: staged through LDS as one tile) — with HIP events on
- * `stream` (forward + backward voxel passes, buffers as in cmg_voxelize_batch_run) and
- * keeps the fastest. ms (optional, 4 entries) receives the timings in the order (1,0)
- * (1,8) (2,0) (2,8). */
+ * voxels per wavefront, 2: 4x4x8, 4: 4x4x16) x LDS tile depth of a search leaf's
+ * triangle records (0: read with scalar loads, 8: staged through LDS as one tile) — with
+ * HIP events on `stream` (forward + backward voxel passes, buffers as in
+ * cmg_voxelize_batch_run) and keeps the fastest. ms (optional, 6 entries) receives the
+ * timings in the order (1,0) (1,8) (2,0) (2,8) (4,0) (4,8). */
 cmg_status cmg_voxelize_batch_autotune(cmg_batch*, float* sdf, const float* grad_sdf,
                                        float* grad_xyz, void* stream, int* vpl, int* depth,
                                        float* ms, char* errbuf, size_t errbuf_len);
 
-/* Selects a voxel-pass variant explicitly (vpl 1 or 2, depth 0 or 8). */
+/* Selects a voxel-pass variant explicitly (vpl 1, 2 or 4, depth 0 or 8). */
 cmg_status cmg_voxelize_batch_set_variant(cmg_batch*, int vpl, int depth);
 
 /* Device pointer to the occupancy bytes of every grid (valid after a forward). */

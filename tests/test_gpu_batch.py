@@ -142,7 +142,7 @@ def test_variants_agree_and_autotune_picks_one(gpu):
                  [d for *_, d in cases])
     gs = torch.from_numpy(W.upstream_grad(b.cells, seed=11)).cuda()
     ref_sdf = ref_grad = None
-    for vpl in (1, 2):
+    for vpl in (1, 2, 4):
         for depth in (0, 8):
             b.set_variant(vpl, depth)
             sdf, grad = b.step(gs, graph=False)
@@ -154,7 +154,7 @@ def test_variants_agree_and_autotune_picks_one(gpu):
             rel = float((grad - ref_grad).norm() / ref_grad.norm())
             assert rel < 1e-5, (vpl, depth, rel)
     tune = b.autotune(gs)
-    assert (tune["vpl"], tune["depth"]) in {(1, 0), (1, 8), (2, 0), (2, 8)}
+    assert (tune["vpl"], tune["depth"]) in {(v, d) for v in (1, 2, 4) for d in (0, 8)}
     assert all(ms > 0 for ms in tune["ms"].values()), tune
     sdf, _ = b.step(gs, graph=True)
     torch.cuda.synchronize()
