@@ -60,7 +60,6 @@ using namespace occdev;
 
 namespace {
 
-constexpr int kStack = 96;  // >= 7 * kMaxLevels + 1 entries of the traversal stack
 
 // ---- work queue -----------------------------------------------------------------------
 
@@ -521,7 +520,6 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
 // of every cell) instead of the distance.
 template <bool BWD, int VPL, int D>
 __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
-    __shared__ int s_stack[kStack];
     __shared__ TriRec s_rec[D > 0 ? D : 1];
     const long long* off = phase_off(C, VPL == 1 ? kPhL0 : (VPL == 2 ? kPhL0b : kPhL0c));
     const long long total = off[C.nm];
@@ -552,15 +550,15 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         const float eps = M.eps;
 
         // Walks the tree for the lanes in act[] with squared bounds b2[] (which on_leaf may
-        // tighten); on_leaf(loc) visits the triangles of level-1 node loc.
+        // tighten); on_leaf(loc) visits the triangles of level-1 node loc. The stack is
+        // wave-uniform and lives in one VGPR, entry e in lane e (at most 1 + 7 (nlev - 1)
+        // <= 64 entries, nlev <= 10).
         auto walk = [&](const bool* act, const float* b2, auto&& on_leaf) {
-            int sp = 0;
-            if (lane == 0) s_stack[0] = M.nlev << 27;
-            sp = 1;
+            int stk = M.nlev << 27;  // entry 0: the root
+            int sp = 1;
             while (sp > 0) {
-                wave_sync();
-                const int ref = uniform_int(s_stack[sp - 1]);
                 --sp;
+                const int ref = __builtin_amdgcn_readlane(stk, sp);
                 const int lev = ref >> 27, loc = ref & 0x7ffffff;
                 ++nvis;
                 {  // the bounds may have tightened since this node was pushed
@@ -609,11 +607,11 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
                     if (((mask >> c) & 1u) && c != near_c) {
-                        if (lane == 0) s_stack[sp] = child | (first + c);
+                        stk = lane == sp ? (child | (first + c)) : stk;
                         ++sp;
                     }
                 if (near_c >= 0) {
-                    if (lane == 0) s_stack[sp] = child | (first + near_c);
+                    stk = lane == sp ? (child | (first + near_c)) : stk;
                     ++sp;
                 }
             }
@@ -1014,8 +1012,9 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         if (pt_start[m + 1] <= pt_start[m]) throw InvalidInput("batch: a mesh has no points");
         if (tri_start[m + 1] <= tri_start[m])
             throw InvalidInput("batch: a mesh has no triangles");
-        if (tri_start[m + 1] - tri_start[m] >= (1LL << 30))
-            throw InvalidInput("batch: a mesh has 2^30 triangles or more");
+        // (at most 10 tree levels: the traversal stack fits one register)
+        if (tri_start[m + 1] - tri_start[m] > (1LL << 30))
+            throw InvalidInput("batch: a mesh has more than 2^30 triangles");
         if (resolution[m] < 1) throw InvalidInput("batch: resolution must be >= 1");
     }
     // grids from device bounding boxes (min/max are exact, so make_grid matches the

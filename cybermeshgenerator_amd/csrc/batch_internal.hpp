@@ -13,8 +13,9 @@
 namespace cmg::hip {
 
 // Levels of the per-mesh triangle tree: level 0 are the triangles (in Morton order of
-// their centroids), a level-l node covers 8^l consecutive triangles. 8^11 > 2^31.
-constexpr int kMaxLevels = 12;
+// their centroids), a level-l node covers 8^l consecutive triangles; a mesh has at most
+// 2^30 = 8^10 triangles, so at most 10 levels above the triangles.
+constexpr int kMaxLevels = 10;
 
 // Bounding cylinder of a tree node (coordinates relative to the grid corner).
 struct __align__(16) NodeRec {
