@@ -154,9 +154,10 @@ def cpu_sdf_sample(W, budget_s=20.0):
                       f"{t_build:.2f} s excluded ({cpu_model()})"}
 
 
-# FP32 operations per lane of the voxel passes' three kinds of work (FMA = 2): a cylinder
-# (or disk) lower bound ~30, a point-triangle distance ~80.
-FLOP_CYL, FLOP_TRI = 30, 80
+# FP32 operations per lane of the voxel passes' three kinds of work (FMA = 2): an
+# oriented-box lower bound (tree node) ~26, a bounding-disk lower bound (triangle) ~30, a
+# point-triangle distance ~80.
+FLOP_BOX, FLOP_DISK, FLOP_TRI = 26, 30, 80
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -164,10 +165,12 @@ def voxel_work(counters, cells, vpl, kernel_ms):
     """Work of the forward + backward voxel passes from the engine's counters (each
     counted once per wavefront of 64 lanes x vpl voxels) and the FP32 rate it implies."""
     lanes = 64 * vpl
-    cyl = (counters["node_tests"] + counters["tri_tests"]) * lanes
+    box = counters["node_tests"] * lanes
+    disk = counters["tri_tests"] * lanes
     tri = counters["tri_evals"] * lanes
-    flops = cyl * FLOP_CYL + tri * FLOP_TRI
-    out = {"cylinder_tests_per_voxel": cyl / (2 * cells),
+    flops = box * FLOP_BOX + disk * FLOP_DISK + tri * FLOP_TRI
+    out = {"node_tests_per_voxel": box / (2 * cells),
+           "triangle_bound_tests_per_voxel": disk / (2 * cells),
            "triangle_distances_per_voxel": tri / (2 * cells),
            "counted_fp32_gflop_per_step": flops / 1e9}
     if kernel_ms > 0:

@@ -9,11 +9,12 @@
 //
 // Nearest-triangle search. At creation every mesh's triangles are put in Morton order of
 // their centroids. A level-l tree node covers 8^l consecutive triangles (an implicit
-// 8-ary tree, no pointers) and carries a bounding cylinder: centre, axis (aligned area-
-// weighted normal), lateral radius, half-height. For a surface patch seen from afar the
-// cylinder is a thin disk, and the distance to it bounds the distance to the patch far
-// more tightly than a sphere or a box would; every triangle likewise has a bounding disk.
-// Each step rebuilds the cylinders bottom-up from the current vertices. A wavefront owns
+// 8-ary tree, no pointers) and carries an oriented bounding box whose first axis is the
+// patch's aligned area-weighted normal and whose other two are its principal in-plane
+// directions. For a surface patch seen from afar the box is a thin plate, and the
+// distance to it bounds the distance to the patch far more tightly than a sphere or an
+// axis-aligned box would; every triangle has a bounding disk. Each step rebuilds the
+// boxes bottom-up from the current vertices. A wavefront owns
 // 64 or 128 neighbouring voxels and walks the tree once for all of them: a node is opened
 // when any voxel's lower bound beats that voxel's best distance so far, the child nearest
 // the leaf centre first (a wave-uniform stack in LDS). The minimum of the FP32 distance
@@ -151,12 +152,20 @@ __device__ __forceinline__ NodeRec load_node(const NodeRec* base, int i) {
     NodeRec r;
     float* d = reinterpret_cast<float*>(&r);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) d[k] = p[k];
+    for (int k = 0; k < 15; ++k) d[k] = p[k];
+    r.pad = 0.0f;
     return r;
 }
 
+// Squared distance from p to the oriented box: a lower bound on the distance to anything
+// inside (the float axes are orthonormal to a few ulps; the distance error margin covers
+// the difference).
 __device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, float pz) {
-    return cyl_lb2(N.cx, N.cy, N.cz, N.nx, N.ny, N.nz, N.r, N.t, px, py, pz);
+    const float vx = px - N.cx, vy = py - N.cy, vz = pz - N.cz;
+    const float dn = fmaxf(fabsf(fma3(vx, N.nx, vy, N.ny, vz, N.nz)) - N.en, 0.0f);
+    const float du = fmaxf(fabsf(fma3(vx, N.ux, vy, N.uy, vz, N.uz)) - N.eu, 0.0f);
+    const float dw = fmaxf(fabsf(fma3(vx, N.wx, vy, N.wy, vz, N.wz)) - N.ew, 0.0f);
+    return fma3(dn, dn, du, du, dw, dw);
 }
 
 // ---- phases ----------------------------------------------------------------------------
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
     }
 }
 
-// ---- tree of bounding cylinders -----------------------------------------------------------
+// ---- tree of oriented bounding boxes ---------------------------------------------------
 
 __device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
     c[0] = a[1] * b[2] - a[2] * b[1];
@@ -385,25 +394,101 @@ __device__ __forceinline__ double dot3(const double* a, const double* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// Rounds a bounding cylinder computed in FP64 to float, outward: the radius and the
-// half-height grow by the rounding of the centre and of the axis (a few ulps of the
-// coordinates) so that the float cylinder still contains the geometry.
-__device__ inline NodeRec round_node(const double* c, const double* n, double r, double t) {
-    const double ext = fmax(fmax(fabs(c[0]), fabs(c[1])), fabs(c[2])) + r + t;
-    const double slack = 8.0 * 0x1p-24 * ext;
-    NodeRec N;
-    N.cx = (float)c[0];
-    N.cy = (float)c[1];
-    N.cz = (float)c[2];
-    N.nx = (float)n[0];
-    N.ny = (float)n[1];
-    N.nz = (float)n[2];
-    N.r = (float)(r * (1.0 + 1e-6) + slack);
-    N.t = (float)(t * (1.0 + 1e-6) + slack);
-    return N;
+__device__ __forceinline__ void normalize3(double* a) {
+    const double l = sqrt(dot3(a, a));
+    for (int k = 0; k < 3; ++k) a[k] /= l;
 }
 
-// Level 1: the cylinder of up to 8 consecutive triangles, from their FP64 vertices.
+// The oriented box of a point set given by points(f) (f is called with every point, twice)
+// and the aligned normal sum N: axis n = N / |N| (or z), u the principal direction of the
+// points projected on the plane of n (closed-form 2x2 eigenvector), w = n x u. The axes
+// are rounded to float first and the extents measured along the float axes, so the float
+// box contains every point; the extents then grow by the rounding of the centre.
+template <class Points>
+__device__ inline NodeRec make_box(const double* N, Points&& points) {
+    double n[3] = {N[0], N[1], N[2]};
+    if (dot3(n, n) > 0.0) {
+        normalize3(n);
+    } else {
+        n[0] = n[1] = 0.0;
+        n[2] = 1.0;
+    }
+    double a[3], b[3];
+    const double tmp[3] = {fabs(n[0]) < 0.9 ? 1.0 : 0.0, fabs(n[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    cross3(n, tmp, a);
+    normalize3(a);
+    cross3(n, a, b);
+    // second moments of the points in the (a, b) plane
+    double cnt = 0.0, mx = 0.0, my = 0.0, sxx = 0.0, syy = 0.0, sxy = 0.0;
+    points([&](const double* P) {
+        const double x = dot3(P, a), y = dot3(P, b);
+        cnt += 1.0;
+        mx += x;
+        my += y;
+        sxx += x * x;
+        syy += y * y;
+        sxy += x * y;
+    });
+    mx /= cnt;
+    my /= cnt;
+    sxx = sxx / cnt - mx * mx;
+    syy = syy / cnt - my * my;
+    sxy = sxy / cnt - mx * my;
+    const double th = 0.5 * atan2(2.0 * sxy, sxx - syy);
+    double u[3], w[3];
+    for (int k = 0; k < 3; ++k) u[k] = cos(th) * a[k] + sin(th) * b[k];
+    normalize3(u);
+    cross3(n, u, w);
+    normalize3(w);
+    float nf[3], uf[3], wf[3];
+    double ax[3][3];
+    for (int k = 0; k < 3; ++k) {
+        nf[k] = (float)n[k];
+        uf[k] = (float)u[k];
+        wf[k] = (float)w[k];
+        ax[0][k] = nf[k];
+        ax[1][k] = uf[k];
+        ax[2][k] = wf[k];
+    }
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    points([&](const double* P) {
+        for (int k = 0; k < 3; ++k) {
+            const double q = dot3(P, ax[k]);
+            lo[k] = fmin(lo[k], q);
+            hi[k] = fmax(hi[k], q);
+        }
+    });
+    double c[3] = {0.0, 0.0, 0.0}, e[3];
+    for (int k = 0; k < 3; ++k) {
+        const double mid = 0.5 * (lo[k] + hi[k]);
+        e[k] = 0.5 * (hi[k] - lo[k]);
+        for (int j = 0; j < 3; ++j) c[j] += mid * ax[k][j];
+    }
+    const double ext = fmax(fmax(fabs(c[0]), fabs(c[1])), fabs(c[2])) + e[0] + e[1] + e[2];
+    const double slack = 8.0 * 0x1p-24 * ext;
+    NodeRec B;
+    B.cx = (float)c[0];
+    B.cy = (float)c[1];
+    B.cz = (float)c[2];
+    B.nx = nf[0];
+    B.ny = nf[1];
+    B.nz = nf[2];
+    B.ux = uf[0];
+    B.uy = uf[1];
+    B.uz = uf[2];
+    B.wx = wf[0];
+    B.wy = wf[1];
+    B.wz = wf[2];
+    B.en = (float)(e[0] * (1.0 + 1e-6) + slack);
+    B.eu = (float)(e[1] * (1.0 + 1e-6) + slack);
+    B.ew = (float)(e[2] * (1.0 + 1e-6) + slack);
+    B.pad = 0.0f;
+    return B;
+}
+
+// Level 1: the box of up to 8 consecutive triangles, from their FP64 vertices (relative
+// to the grid corner); the normal is the sum of the triangles' area vectors, each turned
+// to agree with the first.
 __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
     const int* lo = C.lvl_off + (C.nm + 1);
     const int stride = gridDim.x * blockDim.x;
@@ -414,7 +499,6 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
         const long long s0 = M.tbase + 8LL * loc;
         const int cnt = min(8, M.ntris - 8 * loc);
         double N[3] = {0.0, 0.0, 0.0}, ref[3] = {0.0, 0.0, 0.0};
-        double bl[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bh[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
         bool have_ref = false;
         for (int k = 0; k < cnt; ++k) {
             const TriRec64& T = C.rec64[s0 + k];
@@ -422,8 +506,6 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
             for (int a = 0; a < 3; ++a) {
                 e1[a] = T.b[a] - T.a[a];
                 e2[a] = T.c[a] - T.a[a];
-                bl[a] = fmin(bl[a], fmin(T.a[a], fmin(T.b[a], T.c[a])) - M.g.corner[a]);
-                bh[a] = fmax(bh[a], fmax(T.a[a], fmax(T.b[a], T.c[a])) - M.g.corner[a]);
             }
             cross3(e1, e2, nt);
             if (!have_ref && dot3(nt, nt) > 0.0) {
@@ -433,35 +515,23 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
             const double sg = dot3(nt, ref) < 0.0 ? -1.0 : 1.0;
             for (int a = 0; a < 3; ++a) N[a] += sg * nt[a];
         }
-        double c[3], n[3];
-        for (int a = 0; a < 3; ++a) c[a] = 0.5 * (bl[a] + bh[a]);
-        const double nn = sqrt(dot3(N, N));
-        if (nn > 0.0) {
-            for (int a = 0; a < 3; ++a) n[a] = N[a] / nn;
-        } else {
-            n[0] = n[1] = 0.0;
-            n[2] = 1.0;
-        }
-        double r = 0.0, t = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const TriRec64& T = C.rec64[s0 + k];
-            for (int v = 0; v < 3; ++v) {
-                const double* P = v == 0 ? T.a : (v == 1 ? T.b : T.c);
-                double d[3], w[3];
-                for (int a = 0; a < 3; ++a) d[a] = P[a] - M.g.corner[a] - c[a];
-                cross3(n, d, w);
-                t = fmax(t, fabs(dot3(n, d)));
-                r = fmax(r, sqrt(dot3(w, w)));
+        C.node[M.node_base[1] + loc] = make_box(N, [&](auto&& f) {
+            for (int k = 0; k < cnt; ++k) {
+                const TriRec64& T = C.rec64[s0 + k];
+                for (int v = 0; v < 3; ++v) {
+                    const double* P = v == 0 ? T.a : (v == 1 ? T.b : T.c);
+                    const double q[3] = {P[0] - M.g.corner[0], P[1] - M.g.corner[1],
+                                         P[2] - M.g.corner[2]};
+                    f(q);
+                }
             }
-        }
-        C.node[M.node_base[1] + loc] = round_node(c, n, r, t);
+        });
     }
 }
 
-// Level l >= 2: a cylinder around up to 8 child cylinders. For a child (centre c_k, axis
-// n_k, radius r_k, half-height t_k) and the new axis n through c, with cos = |n.n_k|,
-// sin = |n x n_k|, d = c_k - c: every child point lies within |n.d| + t_k cos + r_k sin
-// of the new mid-plane and within |n x d| + t_k sin + r_k of the new axis.
+// Level l >= 2: the box around the corners of up to 8 child boxes; the normal is the sum
+// of the child normals weighted by their in-plane areas, each turned to agree with the
+// first.
 __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int total) {
     const int* lo = C.lvl_off + (size_t)level * (C.nm + 1);
     const int stride = gridDim.x * blockDim.x;
@@ -472,43 +542,28 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
         const int first = 8 * loc;
         const int cnt = min(8, M.nnodes[level - 1] - first);
         const NodeRec* ch = C.node + M.node_base[level - 1] + first;
-        double N[3] = {0.0, 0.0, 0.0}, ref[3] = {(double)ch[0].nx, (double)ch[0].ny,
-                                                  (double)ch[0].nz};
-        double bl[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bh[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        const double ref[3] = {ch[0].nx, ch[0].ny, ch[0].nz};
+        double N[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < cnt; ++k) {
-            const NodeRec K = ch[k];
-            const double nk[3] = {K.nx, K.ny, K.nz}, ck[3] = {K.cx, K.cy, K.cz};
-            const double w = (double)K.r * K.r + (double)K.t * K.t;
+            const double nk[3] = {ch[k].nx, ch[k].ny, ch[k].nz};
+            const double wgt = (double)ch[k].eu * ch[k].ew + 1e-30;
             const double sg = dot3(nk, ref) < 0.0 ? -1.0 : 1.0;
-            const double ext = (double)K.r + K.t;
-            for (int a = 0; a < 3; ++a) {
-                N[a] += sg * w * nk[a];
-                bl[a] = fmin(bl[a], ck[a] - ext);
-                bh[a] = fmax(bh[a], ck[a] + ext);
+            for (int a = 0; a < 3; ++a) N[a] += sg * wgt * nk[a];
+        }
+        C.node[M.node_base[level] + loc] = make_box(N, [&](auto&& f) {
+            for (int k = 0; k < cnt; ++k) {
+                const NodeRec K = ch[k];
+                for (int corner = 0; corner < 8; ++corner) {
+                    const double sn = (corner & 1) ? K.en : -(double)K.en;
+                    const double su = (corner & 2) ? K.eu : -(double)K.eu;
+                    const double sw = (corner & 4) ? K.ew : -(double)K.ew;
+                    const double q[3] = {K.cx + sn * K.nx + su * K.ux + sw * K.wx,
+                                         K.cy + sn * K.ny + su * K.uy + sw * K.wy,
+                                         K.cz + sn * K.nz + su * K.uz + sw * K.wz};
+                    f(q);
+                }
             }
-        }
-        double c[3], n[3];
-        for (int a = 0; a < 3; ++a) c[a] = 0.5 * (bl[a] + bh[a]);
-        const double nn = sqrt(dot3(N, N));
-        if (nn > 0.0) {
-            for (int a = 0; a < 3; ++a) n[a] = N[a] / nn;
-        } else {
-            n[0] = n[1] = 0.0;
-            n[2] = 1.0;
-        }
-        double r = 0.0, t = 0.0;
-        for (int k = 0; k < cnt; ++k) {
-            const NodeRec K = ch[k];
-            const double nk[3] = {K.nx, K.ny, K.nz};
-            double d[3] = {K.cx - c[0], K.cy - c[1], K.cz - c[2]}, w[3], x[3];
-            cross3(n, nk, x);
-            const double sn = fmin(1.0, sqrt(dot3(x, x)));
-            const double cs = fmin(1.0, fabs(dot3(n, nk)));
-            cross3(n, d, w);
-            t = fmax(t, fabs(dot3(n, d)) + K.t * cs + K.r * sn);
-            r = fmax(r, sqrt(dot3(w, w)) + K.t * sn + K.r);
-        }
-        C.node[M.node_base[level] + loc] = round_node(c, n, r, t);
+        });
     }
 }
 

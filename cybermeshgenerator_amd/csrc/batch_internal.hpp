@@ -17,12 +17,16 @@ namespace cmg::hip {
 // 2^30 = 8^10 triangles, so at most 10 levels above the triangles.
 constexpr int kMaxLevels = 10;
 
-// Bounding cylinder of a tree node (coordinates relative to the grid corner).
+// Oriented bounding box of a tree node (coordinates relative to the grid corner): the
+// first axis is the patch's aligned mean normal, the other two its principal in-plane
+// directions; en/eu/ew are the half-extents along n/u/w.
 struct __align__(16) NodeRec {
-    float cx, cy, cz, r;  // centre, lateral radius
-    float nx, ny, nz, t;  // unit axis, half-height
+    float cx, cy, cz, en;
+    float nx, ny, nz, eu;
+    float ux, uy, uz, ew;
+    float wx, wy, wz, pad;
 };
-static_assert(sizeof(NodeRec) == 32, "node size");
+static_assert(sizeof(NodeRec) == 64, "node size");
 
 // Per-mesh description, uploaded once at creation.
 struct MeshDesc {
