@@ -18,17 +18,19 @@ namespace sdfdev {
 constexpr int kLeaf = 4;          // 4^3 voxels per wavefront
 constexpr float kU = 0x1p-24f;    // FP32 unit roundoff
 
-// FP32 triangle record, coordinates relative to the grid corner (128 bytes).
+// FP32 triangle record, coordinates relative to the grid corner (128 bytes): the vertex
+// a and edge vectors for the distance, the dual basis and unit normal for its interior
+// test, and an oriented bounding box (axes n, u along the longest edge, w = n x u;
+// centre c; half-extents en, eu, ew) for the lower bound.
 struct __align__(16) TriRec {
-    float mx, my, mz, r;       // bounding sphere (centroid, radius rounded up)
+    float cx, cy, cz, en;      // box centre, half-extent along n
     float ax, ay, az, ub;      // vertex a; ub = 1 (or -1: no interior, degenerate)
-    float abx, aby, abz, iab;  // b - a and 1/|b - a|^2 (0 for a zero-length edge)
-    float acx, acy, acz, iac;  // c - a and 1/|c - a|^2
-    float bcx, bcy, bcz, ibc;  // c - b and 1/|c - b|^2
-    float g1x, g1y, g1z, pad0; // dual basis: v = (p - a).g1, w = (p - a).g2
-    float g2x, g2y, g2z, pad1;
-    float nx, ny, nz, t;       // unit normal and half-thickness of the bounding disk
-                               // (degenerate triangle: normal (0,0,1), t = r)
+    float abx, aby, abz, eu;   // b - a; half-extent along u
+    float acx, acy, acz, ew;   // c - a; half-extent along w
+    float ux, uy, uz, nx;      // box axis u; unit normal x
+    float wx, wy, wz, ny;      // box axis w; unit normal y
+    float g1x, g1y, g1z, nz;   // dual basis: v = (p - a).g1, w = (p - a).g2; normal z
+    float g2x, g2y, g2z, pad;  // (degenerate triangle: n = (0,0,1), u = x, w = y)
 };
 static_assert(sizeof(TriRec) == 128, "record size");
 
@@ -71,25 +73,31 @@ __device__ __forceinline__ float clamp01(float x) {
 }
 
 // Squared FP32 distance from p (relative to the corner) to the triangle.
+__device__ __forceinline__ float inv_len2(float x, float y, float z) {
+    const float l = fma3(x, x, y, y, z, z);
+    return l > 0.0f ? 1.0f / l : 0.0f;
+}
+
 __device__ __forceinline__ float tri_d2(const TriRec& T, float px, float py, float pz) {
     const float apx = px - T.ax, apy = py - T.ay, apz = pz - T.az;
+    const float bcx = T.acx - T.abx, bcy = T.acy - T.aby, bcz = T.acz - T.abz;
     // edge ab
-    float t = clamp01(fma3(apx, T.abx, apy, T.aby, apz, T.abz) * T.iab);
+    float t = clamp01(fma3(apx, T.abx, apy, T.aby, apz, T.abz) * inv_len2(T.abx, T.aby, T.abz));
     float ex = __builtin_fmaf(-t, T.abx, apx), ey = __builtin_fmaf(-t, T.aby, apy),
           ez = __builtin_fmaf(-t, T.abz, apz);
     float d = fma3(ex, ex, ey, ey, ez, ez);
     // edge ac
-    t = clamp01(fma3(apx, T.acx, apy, T.acy, apz, T.acz) * T.iac);
+    t = clamp01(fma3(apx, T.acx, apy, T.acy, apz, T.acz) * inv_len2(T.acx, T.acy, T.acz));
     ex = __builtin_fmaf(-t, T.acx, apx);
     ey = __builtin_fmaf(-t, T.acy, apy);
     ez = __builtin_fmaf(-t, T.acz, apz);
     d = fminf(d, fma3(ex, ex, ey, ey, ez, ez));
     // edge bc, from b
     const float bpx = apx - T.abx, bpy = apy - T.aby, bpz = apz - T.abz;
-    t = clamp01(fma3(bpx, T.bcx, bpy, T.bcy, bpz, T.bcz) * T.ibc);
-    ex = __builtin_fmaf(-t, T.bcx, bpx);
-    ey = __builtin_fmaf(-t, T.bcy, bpy);
-    ez = __builtin_fmaf(-t, T.bcz, bpz);
+    t = clamp01(fma3(bpx, bcx, bpy, bcy, bpz, bcz) * inv_len2(bcx, bcy, bcz));
+    ex = __builtin_fmaf(-t, bcx, bpx);
+    ey = __builtin_fmaf(-t, bcy, bpy);
+    ez = __builtin_fmaf(-t, bcz, bpz);
     d = fminf(d, fma3(ex, ex, ey, ey, ez, ez));
     // interior: plane distance when the projection falls inside
     const float v = fma3(apx, T.g1x, apy, T.g1y, apz, T.g1z);
@@ -278,20 +286,6 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     const double nn = sqrt(N[0] * N[0] + N[1] * N[1] + N[2] * N[2]);
     // degenerate (no interior) when the area is negligible against the edge lengths
     const bool degen = !(det > 1e-24 * a11 * a22) || !(nn > 0.0);
-    double M[3];
-    double rad = 0.0;
-    for (int a = 0; a < 3; ++a) M[a] = (P[0][a] + P[1][a] + P[2][a]) / 3.0;
-    for (int v = 0; v < 3; ++v) {
-        double dx = P[v][0] - M[0], dy = P[v][1] - M[1], dz = P[v][2] - M[2];
-        rad = fmax(rad, sqrt(dx * dx + dy * dy + dz * dz));
-    }
-    r.mx = (float)(M[0] - g.corner[0]);
-    r.my = (float)(M[1] - g.corner[1]);
-    r.mz = (float)(M[2] - g.corner[2]);
-    // rounded up: covers the float rounding of the centre and of the radius itself
-    double ext = fmax(fmax(fabs(M[0] - g.corner[0]), fabs(M[1] - g.corner[1])),
-                      fabs(M[2] - g.corner[2]));
-    r.r = (float)(rad * (1.0 + 1e-6) + 4.0 * 0x1p-24 * ext);
     r.ax = (float)A[0];
     r.ay = (float)A[1];
     r.az = (float)A[2];
@@ -299,54 +293,82 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     r.abx = (float)AB[0];
     r.aby = (float)AB[1];
     r.abz = (float)AB[2];
-    r.iab = a11 > 0.0 ? (float)(1.0 / a11) : 0.0f;
     r.acx = (float)AC[0];
     r.acy = (float)AC[1];
     r.acz = (float)AC[2];
-    r.iac = a22 > 0.0 ? (float)(1.0 / a22) : 0.0f;
-    r.bcx = (float)BC[0];
-    r.bcy = (float)BC[1];
-    r.bcz = (float)BC[2];
-    r.ibc = a33 > 0.0 ? (float)(1.0 / a33) : 0.0f;
-    // bounding disk: centre m, radius r, unit normal; its half-thickness covers the float
-    // rounding of the vertices and of the normal (a degenerate triangle gets a cylinder
-    // around its bounding sphere instead)
+    // box axes: the unit normal, the longest edge's direction and their cross product;
+    // the coordinate axes for a degenerate triangle
+    double n[3] = {0.0, 0.0, 1.0}, u[3] = {1.0, 0.0, 0.0}, w[3] = {0.0, 1.0, 0.0};
     if (degen) {
         r.g1x = r.g1y = r.g1z = r.g2x = r.g2y = r.g2z = 0.0f;
-        r.nx = r.ny = 0.0f;
-        r.nz = 1.0f;
-        r.t = r.r;
     } else {
         for (int a = 0; a < 3; ++a) {
             (&r.g1x)[a] = (float)((a22 * AB[a] - a12 * AC[a]) / det);
             (&r.g2x)[a] = (float)((a11 * AC[a] - a12 * AB[a]) / det);
-            (&r.nx)[a] = (float)(N[a] / nn);
+            n[a] = N[a] / nn;
         }
-        r.t = (float)(8.0 * 0x1p-24 * (ext + rad));
+        const double* E = a11 >= a22 ? (a11 >= a33 ? AB : BC) : (a22 >= a33 ? AC : BC);
+        const double el = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+        for (int a = 0; a < 3; ++a) u[a] = E[a] / el;
+        w[0] = n[1] * u[2] - n[2] * u[1];
+        w[1] = n[2] * u[0] - n[0] * u[2];
+        w[2] = n[0] * u[1] - n[1] * u[0];
     }
-    r.pad0 = r.pad1 = 0.0f;
+    // extents measured along the float axes (so the float box holds the vertices), grown
+    // by the rounding of the centre and of the vertices themselves
+    float nf[3], uf[3], wf[3];
+    for (int a = 0; a < 3; ++a) {
+        nf[a] = (float)n[a];
+        uf[a] = (float)u[a];
+        wf[a] = (float)w[a];
+    }
+    const float* axf[3] = {nf, uf, wf};
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double ext = 0.0;
+    for (int v = 0; v < 3; ++v) {
+        double q[3];
+        for (int a = 0; a < 3; ++a) {
+            q[a] = P[v][a] - g.corner[a];
+            ext = fmax(ext, fabs(q[a]));
+        }
+        for (int k = 0; k < 3; ++k) {
+            const double d = q[0] * axf[k][0] + q[1] * axf[k][1] + q[2] * axf[k][2];
+            lo[k] = fmin(lo[k], d);
+            hi[k] = fmax(hi[k], d);
+        }
+    }
+    double c[3] = {0.0, 0.0, 0.0}, e[3];
+    for (int k = 0; k < 3; ++k) {
+        const double mid = 0.5 * (lo[k] + hi[k]);
+        e[k] = 0.5 * (hi[k] - lo[k]) * (1.0 + 1e-6) + 8.0 * 0x1p-24 * ext;
+        for (int a = 0; a < 3; ++a) c[a] += mid * axf[k][a];
+    }
+    r.cx = (float)c[0];
+    r.cy = (float)c[1];
+    r.cz = (float)c[2];
+    r.en = (float)e[0];
+    r.eu = (float)e[1];
+    r.ew = (float)e[2];
+    r.ux = uf[0];
+    r.uy = uf[1];
+    r.uz = uf[2];
+    r.wx = wf[0];
+    r.wy = wf[1];
+    r.wz = wf[2];
+    r.nx = nf[0];
+    r.ny = nf[1];
+    r.nz = nf[2];
+    r.pad = 0.0f;
 }
 
-// Squared distance from p to the solid cylinder with centre c, unit axis n, radius r and
-// half-height t (a disk when t is 0): a lower bound on the distance to anything inside.
-// The lateral distance comes from |n x v| (no cancellation).
-__device__ __forceinline__ float cyl_lb2(float cx, float cy, float cz, float nx, float ny,
-                                         float nz, float r, float t, float px, float py,
-                                         float pz) {
-    const float vx = px - cx, vy = py - cy, vz = pz - cz;
-    const float h = fma3(vx, nx, vy, ny, vz, nz);
-    const float wx = __builtin_fmaf(vy, nz, -vz * ny);
-    const float wy = __builtin_fmaf(vz, nx, -vx * nz);
-    const float wz = __builtin_fmaf(vx, ny, -vy * nx);
-    const float rho = sqrtf(fma3(wx, wx, wy, wy, wz, wz));
-    const float dh = fmaxf(fabsf(h) - t, 0.0f), dr = fmaxf(rho - r, 0.0f);
-    return __builtin_fmaf(dh, dh, dr * dr);
-}
-
+// Squared distance from p to the triangle's oriented box: a lower bound on its distance.
 __device__ __forceinline__ float tri_lb2(const TriRec& T, float px, float py, float pz) {
-    return cyl_lb2(T.mx, T.my, T.mz, T.nx, T.ny, T.nz, T.r, T.t, px, py, pz);
+    const float vx = px - T.cx, vy = py - T.cy, vz = pz - T.cz;
+    const float dn = fmaxf(fabsf(fma3(vx, T.nx, vy, T.ny, vz, T.nz)) - T.en, 0.0f);
+    const float du = fmaxf(fabsf(fma3(vx, T.ux, vy, T.uy, vz, T.uz)) - T.eu, 0.0f);
+    const float dw = fmaxf(fabsf(fma3(vx, T.wx, vy, T.wy, vz, T.wz)) - T.ew, 0.0f);
+    return fma3(dn, dn, du, du, dw, dw);
 }
-
 // Distances at or below this are zero up to the FP64 rounding of the coordinates: such
 // a cell lies on the surface, where the unsigned distance has a kink, and carries no
 // gradient (2^-40 of the grid extent).
