@@ -8,7 +8,9 @@ resident in HBM as 12 float64 planes. `value` is whole-job signs/s over the K ti
 steps. The JSON line also carries:
   roofline      the filter kernel (the dominant one) against the 8 TB/s HBM peak,
                 97 algorithmic bytes per tuple (96 B of FP64 coordinates in, 1 B out),
-                timed with HIP events on its own stream
+                timed with HIP events on its own stream; `traffic` is the HBM bytes per
+                launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a 1 GiB
+                copy), read from the summary committed under profiles/
   cpu_baseline  the CPU oracle (a restatement of the reference's scalar predicates)
                 over the same batch on one host core
   extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, and the SDF
@@ -108,6 +110,21 @@ def bench_predicates(cm, W, steps, warmup):
     res["insphere"] = {"n": n5, "ms": ms5, "filter_ms": k5 / max(c5, 1),
                        "stats": cm.pred_stats(n5, 5, workspace=ws5, stream=stream)}
     return res
+
+
+def measured_traffic(n):
+    """HBM bytes per launch of the filter kernel from the rocprofv3 PMC summary committed
+    under profiles/ (FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy), when it covers
+    this workload; {} otherwise."""
+    path = os.path.join(ROOT, "profiles", "orient3d_2e24_filter_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    if t.get("algorithmic_bytes_per_launch") != ORIENT3D_BYTES * n:
+        return {}
+    return t
 
 
 def cpu_baseline_predicates(soa_np, gpu_signs):
@@ -302,6 +319,7 @@ def main():
     n = pr["n"]
     value = n / (pr["ms"] / 1e3)
     achieved = ORIENT3D_BYTES * n / (pr["filter_ms"] / 1e3) / 1e9
+    traffic = measured_traffic(n)
     st = pr["stats"]
     extra = {
         "orient3d": {"tuples": n, "ms_per_batch": pr["ms"], "filter_kernel_ms": pr["filter_ms"],
@@ -354,7 +372,9 @@ def main():
                                "in HBM, int8 signs", "tuples": n, "mode": "exact",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic.get("traffic_bytes_per_launch"),
+                     "traffic_source": traffic.get("source"),
                      "kernel": "filter_soa_kernel<orient3d, exact>",
                      "bytes_per_launch": ORIENT3D_BYTES * n},
         "cpu_baseline": ({k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")}
