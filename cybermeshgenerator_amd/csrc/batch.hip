@@ -13,11 +13,11 @@
 // patch's aligned area-weighted normal and whose other two are its principal in-plane
 // directions. For a surface patch seen from afar the box is a thin plate, and the
 // distance to it bounds the distance to the patch far more tightly than a sphere or an
-// axis-aligned box would; every triangle has a bounding disk. Each step rebuilds the
-// boxes bottom-up from the current vertices. A wavefront owns
-// 64 or 128 neighbouring voxels and walks the tree once for all of them: a node is opened
-// when any voxel's lower bound beats that voxel's best distance so far, the child nearest
-// the leaf centre first (a wave-uniform stack in LDS). The minimum of the FP32 distance
+// axis-aligned box would; every triangle has its own oriented box. Each step rebuilds the
+// boxes bottom-up from the current vertices. A wavefront owns 64 or more neighbouring
+// voxels and walks the tree once for all of them: a node is opened when any voxel's lower
+// bound beats that voxel's best distance so far, the child nearest the voxel block's
+// centre first (a wave-uniform stack in one register). The minimum of the FP32 distance
 // over the triangles that survive is the minimum over all triangles (every rejection is a
 // rigorous lower bound), so results do not depend on the tree. Voxels within one cell of
 // the surface walk again with the bound fixed at the FP32 minimum plus twice the FP32
@@ -603,6 +603,9 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         }
         const float px = (i + 0.5f) * h, py = (j + 0.5f) * h;
         const float eps = M.eps;
+        // centre of the wavefront's voxel block (relative to the grid corner)
+        const float wcx = (ix * kLeaf + 2) * h, wcy = (iy * kLeaf + 2) * h,
+                    wcz = (iz * L0.sz + 0.5f * L0.sz) * h;
 
         // Walks the tree for the lanes in act[] with squared bounds b2[] (which on_leaf may
         // tighten); on_leaf(loc) visits the triangles of level-1 node loc. The stack is
@@ -639,18 +642,16 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                         ++nvis;
                         const NodeRec K = load_node(C.node, M.node_base[lev - 1] + first + c);
                         bool need = false;
-                        float lb0 = 0.0f;
 #pragma unroll
-                        for (int v = 0; v < VPL; ++v) {
-                            const float lb = node_lb2(K, px, py, pz[v]);
-                            need |= act[v] && lb <= b2[v];
-                            if (v == 0) lb0 = lb;
-                        }
+                        for (int v = 0; v < VPL; ++v)
+                            need |= act[v] && node_lb2(K, px, py, pz[v]) <= b2[v];
                         if (__ballot(need) != 0) {
                             mask |= 1u << c;
-                            // order key: the bound seen from a voxel near the leaf centre
-                            const float key = __int_as_float(
-                                __builtin_amdgcn_readlane(__float_as_int(lb0), 21));
+                            // order key: distance from the voxel block's centre to the box
+                            // centre (discriminates better than the lower bounds, which
+                            // are 0 for every box that overlaps the block)
+                            const float kx = K.cx - wcx, ky = K.cy - wcy, kz = K.cz - wcz;
+                            const float key = fma3(kx, kx, ky, ky, kz, kz);
                             if (key < near_k) {
                                 near_k = key;
                                 near_c = c;
