@@ -227,14 +227,16 @@ def bench_sdf(cm, W, steps, warmup):
     ms_b = time_steps(bstep, reps, max(1, warmup), torch.cuda.synchronize)
     st = batch.counters()
     # dominant kernel: the forward voxel pass
-    cm.kernel_timer("batch_sdf", True)
-    cm.kernel_timer("batch_sdf_backward", True)
+    timers = ("batch_sdf", "batch_sdf_backward", "batch_occupancy")
+    for t in timers:
+        cm.kernel_timer(t, True)
     batch.step(gs, sdf=sdf, grad=grad, graph=False)
     torch.cuda.synchronize()
     fwd_ms, _ = cm.kernel_time("batch_sdf")
     bwd_ms, _ = cm.kernel_time("batch_sdf_backward")
-    cm.kernel_timer("batch_sdf", False)
-    cm.kernel_timer("batch_sdf_backward", False)
+    occ_ms, _ = cm.kernel_time("batch_occupancy")
+    for t in timers:
+        cm.kernel_timer(t, False)
     # plain: one engine per mesh, one mesh at a time
     log("[bench] planning 64 single-mesh engines")
     singles = [SdfBatch([m], [r], [p]) for m, r, p in zip(meshes, res, pads)]
@@ -263,6 +265,7 @@ def bench_sdf(cm, W, steps, warmup):
            "speedup_batched_vs_plain": ms_p / ms_b,
            "batched_equals_plain": same, "plan_s": plan_s,
            "fwd_voxel_kernel_ms": fwd_ms, "bwd_voxel_kernel_ms": bwd_ms,
+           "occupancy_kernel_ms": occ_ms,
            "variant": batch.variant_info(), "autotune_ms": tune["ms"],
            "overflow": st["overflow"]}
     out.update(voxel_work(st, cells, batch.vpl(), fwd_ms + bwd_ms))

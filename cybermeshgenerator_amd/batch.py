@@ -213,14 +213,16 @@ class SdfBatch:
         return {"overflow": int(ov.value), "pairs": int(pairs.value)}
 
     def counters(self, stream=None) -> dict:
-        """Work counters of the last voxel passes (per wavefront): node cylinder tests,
-        triangle disk tests, FP32 triangle distances; and the overflow flags."""
-        out = np.zeros(4, dtype=np.uint64)
+        """Work counters of the last step: per wavefront of the voxel passes, node box
+        tests, triangle box tests and FP32 triangle distances; the overflow flags; and
+        the occupancy pass's column-tile list entries."""
+        out = np.zeros(5, dtype=np.uint64)
         err = _err()
         _check(lib().cmg_voxelize_batch_counters(self._h, _stream(stream),
                                                  out.ctypes.data_as(_P), err, len(err)), err)
         return {"node_tests": int(out[0]), "tri_tests": int(out[1]),
-                "tri_evals": int(out[2]), "overflow": int(out[3])}
+                "tri_evals": int(out[2]), "overflow": int(out[3]),
+                "tile_entries": int(out[4])}
 
     def nearest(self, stream=None):
         """After a forward: per cell (device int32, sdf layout) the mesh-local index of

@@ -30,7 +30,8 @@
 //                range and tile counts
 //   tile alloc   per tile: list offset from the tile pool
 //   tile fill    per triangle: append to the lists of the tiles its range touches
-//   occupancy    per (mesh, 8x8 column tile): exact column parity, as voxelize.hip
+//   occupancy    per (mesh, 8x8 column tile): exact column parity (below); tiles where
+//                a naive determinant is exactly 0 are redone with exact predicates
 //   tree         level 1 from the triangles, then each level from the one below
 //   voxels       per (mesh, voxel leaf): FP32 nearest, FP64 near the surface, signed
 // A backward step reuses the forward's occupancy and tree and runs the adjoint voxel
@@ -123,6 +124,7 @@ struct Ctx {  // kernel arguments shared by every phase
     int* tile_pool;
     long long tile_cap;
     unsigned long long* masks;
+    int* redo;                  // tiles (global items) for the exact occupancy pass
     uint8_t* occ;
     float* sdf;
     const float* gsdf;
@@ -274,17 +276,62 @@ __global__ __launch_bounds__(256) void tile_fill_kernel(Ctx C) {
     }
 }
 
-// One 8x8 column tile per workgroup item; same algorithm as voxelize.hip's kernel.
+// Occupancy (voxelize.cpp:78-195). The xy bucket of a column is reproduced exactly:
+// triangle t belongs to column (i, j) iff i0 <= i <= i1 and j0 <= j <= j1, with the
+// reference's clamp(floor((x - corner) / spacing)) ranges over the three vertices. One
+// 64-lane workgroup per 8x8 column tile, one lane per column; the tile's triangle list
+// (every triangle whose xy range touches the tile) is streamed through LDS in chunks of
+// 64 triangles as structure-of-arrays FP64, and a lane runs the reference's crossing
+// test (compat orient3d in the reference's operation order) on its own bucket.
+//
+// Heights are not stored or sorted: the reference counts, per cell, the crossing heights
+// strictly above the cell centre zc_k and keeps the parity (voxelize.cpp:185-191). A
+// crossing at height h flips exactly the cells with zc_k < h, i.e. cells 0 .. K(h)-1, so
+// each lane keeps one bit per cell boundary ("flip below K") in a per-column bitmask and
+// the final parity of cell k is the XOR of the bits above k.
+//
+// Grazing (an orient3d exactly 0) restarts the column with the next kNudge offset
+// (voxelize.cpp:148-165). If the last nudge grazes too, the reference keeps the
+// crossings found before the first grazing triangle of the bucket, which lists triangles
+// in ascending index order; pass 6 records the smallest grazing index and a final pass
+// replays that nudge counting only triangles below it.
+
+// Sign of the naive FP64 orient3d determinant (predicates.cpp:2201-2222), 0 when it is
+// exactly 0 — then only the exact sign can tell, and the tile is redone by the exact pass.
+__device__ __forceinline__ int o3_naive_sign(const double* pa, const double* pb,
+                                             const double* pc, const double* pd) {
+    double adx = pa[0] - pd[0], ady = pa[1] - pd[1], adz = pa[2] - pd[2];
+    double bdx = pb[0] - pd[0], bdy = pb[1] - pd[1], bdz = pb[2] - pd[2];
+    double cdx = pc[0] - pd[0], cdy = pc[1] - pd[1], cdz = pc[2] - pd[2];
+    double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+    double cdxady = cdx * ady, adxcdy = adx * cdy;
+    double adxbdy = adx * bdy, bdxady = bdx * ady;
+    double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+    return det > 0.0 ? 1 : (det < 0.0 ? -1 : 0);
+}
+
+// One 8x8 column tile per workgroup item. EXACT = false: every predicate is the naive
+// FP64 sign; a tile in which any of them is exactly 0 (a possible graze, which only the
+// exact sign can settle) writes nothing and is queued for the EXACT pass, which runs the
+// compat predicate (naive sign, exact sign on 0) with the nudge and replay passes. Kept
+// as two kernels so that the expansion arithmetic's registers do not limit the
+// occupancy of the common one.
+template <bool EXACT>
 __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
     __shared__ double sv[9][kChunk];
     __shared__ int4 sr[kChunk];
     __shared__ int st[kChunk];
     const long long* off = phase_off(C, kPhTile);
-    const long long total = off[C.nm];
+    const long long total = EXACT ? 0 : off[C.nm];
     const int lane = threadIdx.x;
     while (true) {
-        const long long item = next_item(&C.ctr->queue[0]);
-        if (item >= total) break;
+        long long item = next_item(&C.ctr->queue[EXACT ? 3 : 0]);
+        if (EXACT) {
+            if (item >= (long long)C.ctr->redo_n) break;
+            item = C.redo[item];
+        } else if (item >= total) {
+            break;
+        }
         const int pos = find_pos(off, C.nm, item);
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec g = M.g;
@@ -304,8 +351,15 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
         const double cx = g.corner[0] + (ci + 0.5) * g.spacing;
         const double cy = g.corner[1] + (cj + 0.5) * g.spacing;
         bool active = valid && ok;
+        bool unsure = false;  // EXACT = false: a naive determinant was exactly 0
         int cutoff = INT_MAX;
-        for (int pass = 0; pass < 8; ++pass) {
+        auto o3 = [&](const double* a, const double* b, const double* c, const double* d) {
+            if (EXACT) return o3_compat(a, b, c, d);
+            const int sgn = o3_naive_sign(a, b, c, d);
+            unsure |= sgn == 0;
+            return sgn;
+        };
+        for (int pass = 0; pass < (EXACT ? 8 : 1); ++pass) {
             if (!__syncthreads_or(active)) break;
             double x = cx, y = cy;
             if (pass > 0) {
@@ -346,17 +400,17 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
                     const double A[3] = {sv[0][q], sv[1][q], sv[2][q]};
                     const double B[3] = {sv[3][q], sv[4][q], sv[5][q]};
                     const double Cc[3] = {sv[6][q], sv[7][q], sv[8][q]};
-                    const int sa = o3_compat(P, A, B, Cc);
-                    const int sb = o3_compat(Q, A, B, Cc);
+                    const int sa = o3(P, A, B, Cc);
+                    const int sb = o3(Q, A, B, Cc);
                     if (sa == 0 || sb == 0) {
                         grazed = true;
                         first_graze = min(first_graze, t);
                         continue;
                     }
                     if (sa == sb) continue;
-                    const int s1 = o3_compat(P, Q, A, B);
-                    const int s2 = o3_compat(P, Q, B, Cc);
-                    const int s3 = o3_compat(P, Q, Cc, A);
+                    const int s1 = o3(P, Q, A, B);
+                    const int s2 = o3(P, Q, B, Cc);
+                    const int s3 = o3(P, Q, Cc, A);
                     if (s1 == 0 || s2 == 0 || s3 == 0) {
                         grazed = true;
                         first_graze = min(first_graze, t);
@@ -377,6 +431,10 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
                     active = false;
                 }
             }
+        }
+        if (!EXACT && __syncthreads_or(unsure)) {
+            if (lane == 0) C.redo[atomicAdd(&C.ctr->redo_n, 1ull)] = (int)item;
+            continue;
         }
         if (valid && ok) write_column(mask, M.words, g, ci, cj, C.occ + M.cell_base);
     }
@@ -982,6 +1040,8 @@ struct BatchEngine::Impl {
     void launch_forward(hipStream_t st);
     void launch_backward(hipStream_t st);
     void launch_tree(hipStream_t st);
+    void launch_occupancy(hipStream_t st);
+    void launch_occupancy_phases(hipStream_t st);
 
     // Persistent voxel passes get as many one-wave workgroups as fit on the device at
     // once (the tree walk is latency bound: every resident wave hides some of it).
@@ -1020,7 +1080,20 @@ void BatchEngine::Impl::launch_tree(hipStream_t st) {
     }
 }
 
-void BatchEngine::Impl::launch_forward(hipStream_t st) {
+void BatchEngine::Impl::launch_occupancy(hipStream_t st) {
+    static unsigned g_fast = 0, g_exact = 0;
+    if (!g_fast) g_fast = resident_grid(occ_kernel<false>);
+    if (!g_exact) g_exact = resident_grid(occ_kernel<true>);
+    {
+        TimedLaunch timed("batch_occupancy", st);
+        occ_kernel<false><<<g_fast, 64, 0, st>>>(ctx);
+        CMG_HIP_LAUNCH();
+    }
+    occ_kernel<true><<<g_exact, 64, 0, st>>>(ctx);
+    CMG_HIP_LAUNCH();
+}
+
+void BatchEngine::Impl::launch_occupancy_phases(hipStream_t st) {
     Ctx& C = ctx;
     reset_kernel<<<grid256, 256, 0, st>>>(C);
     CMG_HIP_LAUNCH();
@@ -1030,8 +1103,11 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     CMG_HIP_LAUNCH();
     tile_fill_kernel<<<grid256, 256, 0, st>>>(C);
     CMG_HIP_LAUNCH();
-    occ_kernel<<<grid64, 64, 0, st>>>(C);
-    CMG_HIP_LAUNCH();
+    launch_occupancy(st);
+}
+
+void BatchEngine::Impl::launch_forward(hipStream_t st) {
+    launch_occupancy_phases(st);
     launch_tree(st);
     {
         TimedLaunch timed("batch_sdf", st);
@@ -1201,6 +1277,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.tile_off = I.dalloc<int>(tiles);
     C.tile_cursor = I.dalloc<int>(tiles);
     C.masks = I.dalloc<unsigned long long>(words);
+    C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
@@ -1256,6 +1333,8 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     Impl& I = *p_;
     if (d_xyz != I.xyz) throw InvalidInput("batch: the points must stay at the address used at creation");
     const bool fwd = flags & kBatchForward, bwd = flags & kBatchBackward;
+    const bool occ_only = (flags & kBatchOccupancy) && !fwd;
+    if (occ_only && bwd) throw InvalidInput("batch: backward needs a forward");
     if (fwd && !d_sdf) throw InvalidInput("batch: forward needs an sdf buffer");
     if (bwd && (!d_gsdf || !d_grad)) throw InvalidInput("batch: backward needs grad buffers");
     I.ctx.sdf = d_sdf;
@@ -1263,6 +1342,7 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     I.ctx.grad = d_grad;
     I.ctx.nearest = nullptr;
     if (!(flags & kBatchGraph)) {
+        if (occ_only) I.launch_occupancy_phases(st);
         if (fwd) I.launch_forward(st);
         if (bwd) I.launch_backward(st);
         return;
@@ -1271,7 +1351,7 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     // may be the legacy default stream, which cannot be captured) and launched on st;
     // it depends only on the buffer addresses and the pass selection.
     const void* key[4] = {d_xyz, d_sdf, d_gsdf, d_grad};
-    const int f = flags & (kBatchForward | kBatchBackward);
+    const int f = flags & (kBatchForward | kBatchBackward | kBatchOccupancy);
     if (!I.exec || I.graph_flags != f || std::memcmp(key, I.graph_key, sizeof key) != 0) {
         if (I.exec) {
             CMG_HIP(hipGraphExecDestroy(I.exec));
@@ -1285,6 +1365,7 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
         hipStream_t cs = I.capture;
         CMG_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
         try {
+            if (occ_only) I.launch_occupancy_phases(cs);
             if (fwd) I.launch_forward(cs);
             if (bwd) I.launch_backward(cs);
         } catch (...) {

@@ -49,8 +49,10 @@ struct MeshDesc {
 enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };
 
 struct BatchCounters {
-    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 backward
+    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 backward,
+                                   // 3 tiles redone with exact predicates
     unsigned long long tile_used;  // tile-list pool cursor
+    unsigned long long redo_n;     // tiles queued for the exact occupancy pass
     // work of the voxel passes, counted per wavefront: cylinder tests of tree nodes,
     // bounding-disk tests of triangles, FP32 point-triangle distances
     unsigned long long node_tests, tri_tests, tri_evals;
@@ -58,7 +60,7 @@ struct BatchCounters {
     unsigned pad;
 };
 
-enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4 };
+enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4, kBatchOccupancy = 8 };
 
 // A voxel-pass variant: voxels per lane (1: 4x4x4 leaves, 2: 4x4x8, 4: 4x4x16) and the
 // LDS tile depth of a tree leaf's triangle records (0: read with scalar loads, 8: staged
@@ -89,8 +91,9 @@ class BatchEngine {
 
     // One step on stream st: kBatchForward writes the SDF of every mesh to d_sdf
     // (total_cells floats); kBatchBackward (after a forward) writes dL/dxyz to d_grad
-    // (3 floats per point) for the upstream d_gsdf. kBatchGraph captures the sequence
-    // into a hipGraph on first use (per buffer set) and replays it.
+    // (3 floats per point) for the upstream d_gsdf; kBatchOccupancy alone computes only
+    // the occupancy (occupancy()). kBatchGraph captures the sequence into a hipGraph on
+    // first use (per buffer set) and replays it.
     void run(int flags, const double* d_xyz, float* d_sdf, const float* d_gsdf, float* d_grad,
              hipStream_t st);
 

@@ -507,6 +507,7 @@ cmg_status cmg_voxelize_batch_counters(cmg_batch* b, void* stream, uint64_t* out
         out[1] = c.tri_tests;
         out[2] = c.tri_evals;
         out[3] = c.overflow;
+        out[4] = c.tile_used;
     });
 }
 

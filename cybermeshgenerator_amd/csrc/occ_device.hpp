@@ -1,6 +1,5 @@
-// occ_device.hpp — device-side pieces of the occupancy path shared by the single-mesh
-// launcher (voxelize.hip) and the batched engine (batch.hip): bucket ranges, the compat
-// orient3d, plane heights and the cell count below a crossing, each following
+// occ_device.hpp — device-side pieces of the occupancy path (batch.hip): bucket ranges,
+// the compat orient3d, plane heights and the cell count below a crossing, each following
 // src/voxelize/voxelize.cpp operation for operation. Every TU that includes it is
 // compiled with -ffp-contract=off.
 #pragma once

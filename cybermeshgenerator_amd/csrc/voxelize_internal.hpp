@@ -58,11 +58,4 @@ struct OccPlan {
 };
 OccPlan occupancy_plan(const GridSpec& g);
 
-// Occupancy of one mesh into d_occ (device, nx*ny*nz bytes). Synchronises `st` once to
-// size the tile lists. h_stats (optional, host, 2 entries): columns nudged, columns
-// where every nudge grazed.
-void launch_occupancy(const double* d_xyz, const int* d_tris, int ntris, const GridSpec& g,
-                      uint8_t* d_occ, DeviceArena& arena, hipStream_t st,
-                      unsigned* h_stats);
-
 }  // namespace cmg::hip

@@ -234,15 +234,16 @@ const unsigned char* cmg_voxelize_batch_occupancy(const cmg_batch*);
 cmg_status cmg_voxelize_batch_nearest(cmg_batch*, int32_t* tri, void* stream,
                                       char* errbuf, size_t errbuf_len);
 
-/* Synchronises `stream` and reads the work counters of the last voxel passes, per
- * wavefront: out[0] cylinder tests of search-tree nodes, out[1] bounding-disk tests of
- * triangles, out[2] FP32 point-triangle distance evaluations, out[3] overflow flags. */
+/* Synchronises `stream` and reads the work counters of the last step: per wavefront of
+ * the voxel passes, out[0] bounding-box tests of search-tree nodes, out[1] bounding-box
+ * tests of triangles, out[2] FP32 point-triangle distance evaluations; out[3] overflow
+ * flags; out[4] column-tile list entries of the occupancy pass. */
 cmg_status cmg_voxelize_batch_counters(cmg_batch*, void* stream, uint64_t* out,
                                        char* errbuf, size_t errbuf_len);
 
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
- * "occupancy_columns", "batch_sdf", "batch_sdf_backward"). Events are recorded on the
+ * "batch_occupancy", "batch_sdf", "batch_sdf_backward"). Events are recorded on the
  * stream the kernel is launched on; launches recorded into a graph are not timed. */
 void cmg_hip_kernel_timer(const char* name, int enable);
 
