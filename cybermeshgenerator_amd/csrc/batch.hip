@@ -296,6 +296,30 @@ __global__ __launch_bounds__(256) void tile_fill_kernel(Ctx C) {
 // in ascending index order; pass 6 records the smallest grazing index and a final pass
 // replays that nudge counting only triangles below it.
 
+// write_column for masks updated by other lanes' atomics: the words are read with atomic
+// loads, which bypass the non-coherent L1.
+__device__ inline void write_column_coherent(const unsigned long long* mask, int words,
+                                             const GridSpec& g, int i, int j, uint8_t* occ) {
+    unsigned long long carry = 0;
+    for (int w = words - 1; w >= 0; --w) {
+        const unsigned long long x = __hip_atomic_load(mask + w, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+        unsigned long long s = x;
+        s ^= s >> 1;
+        s ^= s >> 2;
+        s ^= s >> 4;
+        s ^= s >> 8;
+        s ^= s >> 16;
+        s ^= s >> 32;  // bit b: XOR of bits b..63
+        const unsigned long long above = (s >> 1) ^ (carry ? ~0ull : 0ull);
+        carry ^= s & 1ull;
+        const int k0 = w * 64;
+        const int k1 = min(g.nz, k0 + 64);
+        for (int k = k0; k < k1; ++k)
+            occ[((size_t)k * g.ny + j) * g.nx + i] = (uint8_t)((above >> (k - k0)) & 1ull);
+    }
+}
+
 // Sign of the naive FP64 orient3d determinant (predicates.cpp:2201-2222), 0 when it is
 // exactly 0 — then only the exact sign can tell, and the tile is redone by the exact pass.
 __device__ __forceinline__ int o3_naive_sign(const double* pa, const double* pb,
@@ -308,6 +332,121 @@ __device__ __forceinline__ int o3_naive_sign(const double* pa, const double* pb,
     double adxbdy = adx * bdy, bdxady = bdx * ady;
     double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
     return det > 0.0 ? 1 : (det < 0.0 ? -1 : 0);
+}
+
+// Fast occupancy pass: one 8x8 column tile per workgroup item, but the lanes work on
+// (column, triangle) pairs instead of columns, so that the FP64 predicates run on full
+// wavefronts (a triangle usually covers only a few of a tile's 64 columns). Each LDS
+// chunk of 64 triangles is expanded into the pairs of its triangles' column ranges
+// clipped to the tile (a wave prefix sum gives every pair a lane); a crossing flips its
+// column's "below K" bit with an atomic XOR. Every predicate is the naive FP64 sign: a
+// tile in which any of them is exactly 0 writes nothing and is queued for the exact pass
+// (occ_kernel), which settles zeros with the exact sign, nudges and replays as the
+// reference does.
+__global__ __launch_bounds__(64) void occ_pairs_kernel(Ctx C) {
+    __shared__ double sv[9][kChunk];
+    __shared__ int s_i0[kChunk], s_j0[kChunk], s_ni[kChunk];
+    __shared__ int s_off[kChunk + 1];
+    const long long* off = phase_off(C, kPhTile);
+    const long long total = off[C.nm];
+    const int lane = threadIdx.x;
+    while (true) {
+        const long long item = next_item(&C.ctr->queue[0]);
+        if (item >= total) break;
+        const int pos = find_pos(off, C.nm, item);
+        const MeshDesc& M = C.mesh[C.order[pos]];
+        const GridSpec g = M.g;
+        const int tile = (int)(item - off[pos]);
+        const int tx0 = (tile % M.ntx) * kTile, ty0 = (tile / M.ntx) * kTile;
+        const int ci = tx0 + (lane % kTile), cj = ty0 + (lane / kTile);
+        const bool valid = ci < g.nx && cj < g.ny;
+        const int gt = M.tile_base + tile;
+        const long long beg = C.tile_off[gt];
+        const long long end = beg + C.tile_count[gt];
+        const bool ok = end <= C.tile_cap;  // overflowed lists are reported, not read
+        const double* xyz = C.xyz + 3 * M.vbase;
+        unsigned long long* const masks = C.masks + M.mask_base;
+        if (valid)
+            for (int w = 0; w < M.words; ++w) masks[((size_t)cj * g.nx + ci) * M.words + w] = 0ull;
+        __threadfence();  // the clears land before any lane's crossings
+        const double zmin = g.corner[2] - g.spacing;
+        const double zmax = g.corner[2] + (g.nz + 1) * g.spacing;
+        bool unsure = false;
+        for (long long c0 = beg; ok && c0 < end; c0 += kChunk) {
+            const int cnt = (int)min((long long)kChunk, end - c0);
+            int npair = 0;
+            __syncthreads();
+            if (lane < cnt) {
+                const int t = C.tile_pool[c0 + lane];
+                const int4 r = C.xyr[t];  // a list triangle's range meets the tile
+                const int i0 = max(r.x, tx0), i1 = min(r.y, tx0 + kTile - 1);
+                const int j0 = max(r.z, ty0), j1 = min(r.w, ty0 + kTile - 1);
+                s_i0[lane] = i0;
+                s_j0[lane] = j0;
+                s_ni[lane] = i1 - i0 + 1;
+                npair = (i1 >= i0 && j1 >= j0) ? (i1 - i0 + 1) * (j1 - j0 + 1) : 0;
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const double* p = xyz + 3 * (size_t)C.tris[3 * (size_t)t + v];
+                    sv[3 * v][lane] = p[0];
+                    sv[3 * v + 1][lane] = p[1];
+                    sv[3 * v + 2][lane] = p[2];
+                }
+            }
+            int incl = npair;  // inclusive prefix over the lanes
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            s_off[lane + 1] = incl;
+            if (lane == 0) s_off[0] = 0;
+            __syncthreads();
+            const int npairs = s_off[cnt];
+            for (int p = lane; p < npairs; p += 64) {
+                int lo = 0, hi = cnt;  // s_off[lo] <= p < s_off[hi]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= p) lo = mid;
+                    else hi = mid;
+                }
+                const int q = lo, r = p - s_off[q];
+                const int i = s_i0[q] + r % s_ni[q], j = s_j0[q] + r / s_ni[q];
+                const double x = g.corner[0] + (i + 0.5) * g.spacing;  // voxelize.cpp:180
+                const double y = g.corner[1] + (j + 0.5) * g.spacing;  // voxelize.cpp:181
+                const double A[3] = {sv[0][q], sv[1][q], sv[2][q]};
+                const double B[3] = {sv[3][q], sv[4][q], sv[5][q]};
+                const double Cc[3] = {sv[6][q], sv[7][q], sv[8][q]};
+                const double P[3] = {x, y, zmin};
+                const double Q[3] = {x, y, zmax};
+                const int sa = o3_naive_sign(P, A, B, Cc);
+                const int sb = o3_naive_sign(Q, A, B, Cc);
+                if (sa == 0 || sb == 0) {
+                    unsure = true;
+                    continue;
+                }
+                if (sa == sb) continue;
+                const int s1 = o3_naive_sign(P, Q, A, B);
+                const int s2 = o3_naive_sign(P, Q, B, Cc);
+                const int s3 = o3_naive_sign(P, Q, Cc, A);
+                if (s1 == 0 || s2 == 0 || s3 == 0) {
+                    unsure = true;
+                    continue;
+                }
+                if (s1 != s2 || s2 != s3) continue;
+                const int K = cells_below(g, plane_z(A, B, Cc, x, y));
+                atomicXor(masks + ((size_t)j * g.nx + i) * M.words + (K >> 6), 1ull << (K & 63));
+            }
+        }
+        if (__syncthreads_or(unsure)) {
+            if (lane == 0) C.redo[atomicAdd(&C.ctr->redo_n, 1ull)] = (int)item;
+            continue;
+        }
+        __threadfence();  // every crossing has landed
+        if (valid && ok) {
+            const unsigned long long* cm = masks + ((size_t)cj * g.nx + ci) * M.words;
+            write_column_coherent(cm, M.words, g, ci, cj, C.occ + M.cell_base);
+        }
+    }
 }
 
 // One 8x8 column tile per workgroup item. EXACT = false: every predicate is the naive
@@ -1082,11 +1221,11 @@ void BatchEngine::Impl::launch_tree(hipStream_t st) {
 
 void BatchEngine::Impl::launch_occupancy(hipStream_t st) {
     static unsigned g_fast = 0, g_exact = 0;
-    if (!g_fast) g_fast = resident_grid(occ_kernel<false>);
+    if (!g_fast) g_fast = resident_grid(occ_pairs_kernel);
     if (!g_exact) g_exact = resident_grid(occ_kernel<true>);
     {
         TimedLaunch timed("batch_occupancy", st);
-        occ_kernel<false><<<g_fast, 64, 0, st>>>(ctx);
+        occ_pairs_kernel<<<g_fast, 64, 0, st>>>(ctx);
         CMG_HIP_LAUNCH();
     }
     occ_kernel<true><<<g_exact, 64, 0, st>>>(ctx);
