@@ -808,15 +808,20 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         // tighten); on_leaf(loc) visits the triangles of level-1 node loc. The stack is
         // wave-uniform and lives in one VGPR, entry e in lane e (at most 1 + 7 (nlev - 1)
         // <= 64 entries, nlev <= 10).
+        // `ver` counts the leaf visits that tightened some lane's bound; a node pushed at
+        // the current version passed its test with the current bounds and is not re-tested.
+        int ver = 0;
         auto walk = [&](const bool* act, const float* b2, auto&& on_leaf) {
             int stk = M.nlev << 27;  // entry 0: the root
+            int stv = -1;            // version at push, entry e in lane e
             int sp = 1;
             while (sp > 0) {
                 --sp;
                 const int ref = __builtin_amdgcn_readlane(stk, sp);
+                const int pushed = __builtin_amdgcn_readlane(stv, sp);
                 const int lev = ref >> 27, loc = ref & 0x7ffffff;
-                ++nvis;
-                {  // the bounds may have tightened since this node was pushed
+                if (pushed != ver) {  // the bounds have tightened since this node was pushed
+                    ++nvis;
                     const NodeRec N = load_node(C.node, M.node_base[lev] + loc);
                     bool need = false;
 #pragma unroll
@@ -861,10 +866,12 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 for (int c = 0; c < 8; ++c)
                     if (((mask >> c) & 1u) && c != near_c) {
                         stk = lane == sp ? (child | (first + c)) : stk;
+                        stv = lane == sp ? ver : stv;
                         ++sp;
                     }
                 if (near_c >= 0) {
                     stk = lane == sp ? (child | (first + near_c)) : stk;
+                    stv = lane == sp ? ver : stv;
                     ++sp;
                 }
             }
@@ -905,12 +912,14 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     need |= valid[v] && tri_lb2(T, px, py, pz[v]) <= b2[v];
                 if (__ballot(need) == 0) continue;
                 ++neval;
+                bool tighter = false;
 #pragma unroll
                 for (int v = 0; v < VPL; ++v) {
                     // equal distances go to the lower slot, so the choice does not
                     // depend on the order the tree is walked in
                     const float d2 = tri_d2(T, px, py, pz[v]);
                     const int s = (int)(s0 + c);
+                    tighter |= d2 < best[v];
                     if (d2 < best[v] || (d2 == best[v] && s < arg[v])) {
                         best[v] = d2;
                         arg[v] = s;
@@ -918,6 +927,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + eps;
                     b2[v] = bb * bb;
                 }
+                if (__ballot(tighter) != 0) ++ver;
             }
         });
 
@@ -942,6 +952,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             p64[v][2] = g.corner[2] + (k[v] + 0.5) * g.spacing;
         }
         if (__ballot(any_near) != 0) {
+            ++ver;  // new bounds: nothing on the stack was tested against them
             float b2n[VPL];
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
