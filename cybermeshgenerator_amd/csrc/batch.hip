@@ -126,6 +126,7 @@ struct Ctx {  // kernel arguments shared by every phase
     unsigned long long* masks;
     int* redo;                  // tiles (global items) for the exact occupancy pass
     uint8_t* occ;
+    float* best;                // per cell: the forward's FP32 squared distance
     float* sdf;
     const float* gsdf;
     float* grad;
@@ -891,14 +892,24 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             return s0;
         };
 
-        // ---- FP32 nearest triangle
+        // ---- FP32 nearest triangle. The backward starts from the forward's final bound
+        // (the same triangles, the same FP32 distances): whatever it prunes has a
+        // distance above the forward's minimum, so it finds the same triangle with a
+        // shorter walk.
         float best[VPL], b2[VPL];
         int arg[VPL];
+        float* const cell_best = C.best + M.cell_base;
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             best[v] = FLT_MAX;
             b2[v] = FLT_MAX;
             arg[v] = -1;
+            if (BWD && valid[v]) {
+                const float bb =
+                    sqrtf(cell_best[((size_t)k[v] * g.ny + j) * g.nx + i]) * (1.0f + 4.0f * kU) +
+                    eps;
+                b2[v] = bb * bb;
+            }
         }
         walk(valid, b2, [&](int loc) {
             int cnt;
@@ -919,13 +930,13 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     // depend on the order the tree is walked in
                     const float d2 = tri_d2(T, px, py, pz[v]);
                     const int s = (int)(s0 + c);
-                    tighter |= d2 < best[v];
                     if (d2 < best[v] || (d2 == best[v] && s < arg[v])) {
                         best[v] = d2;
                         arg[v] = s;
                     }
                     const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + eps;
-                    b2[v] = bb * bb;
+                    tighter |= bb * bb < b2[v];
+                    b2[v] = fminf(b2[v], bb * bb);
                 }
                 if (__ballot(tighter) != 0) ++ver;
             }
@@ -996,6 +1007,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             for (int v = 0; v < VPL; ++v) {
                 if (!valid[v]) continue;
                 const size_t idx = ((size_t)k[v] * g.ny + j) * g.nx + i;
+                cell_best[idx] = best[v];
                 if (near[v]) {
                     const double d = sqrt(best64[v]);
                     out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
@@ -1186,6 +1198,7 @@ struct BatchEngine::Impl {
     }
 
     int vpl = 1, depth = 0;  // voxel-pass variant (voxels per lane, LDS tile depth)
+    bool have_forward = false;  // a forward was issued: tree, occupancy and bounds exist
 
     void launch_forward(hipStream_t st);
     void launch_backward(hipStream_t st);
@@ -1257,6 +1270,7 @@ void BatchEngine::Impl::launch_occupancy_phases(hipStream_t st) {
 }
 
 void BatchEngine::Impl::launch_forward(hipStream_t st) {
+    have_forward = true;
     launch_occupancy_phases(st);
     launch_tree(st);
     {
@@ -1429,6 +1443,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.masks = I.dalloc<unsigned long long>(words);
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
+    C.best = I.dalloc<float>(cells);
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
     I.grid256 = (unsigned)cu_count() * 4;
@@ -1484,9 +1499,14 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
     if (d_xyz != I.xyz) throw InvalidInput("batch: the points must stay at the address used at creation");
     const bool fwd = flags & kBatchForward, bwd = flags & kBatchBackward;
     const bool occ_only = (flags & kBatchOccupancy) && !fwd;
-    if (occ_only && bwd) throw InvalidInput("batch: backward needs a forward");
+    if (bwd && !fwd && (occ_only || !I.have_forward))
+        throw InvalidInput("batch: backward needs a forward");
     if (fwd && !d_sdf) throw InvalidInput("batch: forward needs an sdf buffer");
     if (bwd && (!d_gsdf || !d_grad)) throw InvalidInput("batch: backward needs grad buffers");
+    // an occupancy-only pass re-reads the points but leaves the tree and the bounds of
+    // the last forward as they were: a backward needs a new forward after it
+    if (fwd) I.have_forward = true;
+    else if (occ_only) I.have_forward = false;
     I.ctx.sdf = d_sdf;
     I.ctx.gsdf = d_gsdf;
     I.ctx.grad = d_grad;
@@ -1534,6 +1554,7 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
 
 void BatchEngine::nearest(int* d_tri, hipStream_t st) {
     Impl& I = *p_;
+    if (!I.have_forward) throw InvalidInput("batch: nearest needs a forward");
     Ctx saved = I.ctx;
     I.ctx.gsdf = nullptr;
     I.ctx.grad = nullptr;

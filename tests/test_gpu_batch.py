@@ -175,3 +175,21 @@ def test_batch_nearest_matches_single_mesh_query(gpu):
         gr = b.grids[m]
         want = gpu.sdf_nearest(gpu.plc_from_mesh(p, t), r, d)
         assert np.array_equal(near[gr.offset:gr.offset + gr.ncell], want), m
+
+
+def test_backward_and_nearest_need_a_forward(gpu):
+    """The backward pass starts from the forward's per-cell bounds and tree: before a
+    forward, and after an occupancy-only pass, it refuses to run."""
+    import torch
+    from cybermeshgenerator_amd.batch import SdfBatch
+    p, t, r, d = _cases()[0]
+    b = SdfBatch([(p, t)], [r], [d])
+    g = torch.ones(b.cells, dtype=torch.float32, device="cuda")
+    with pytest.raises(Exception, match="needs a forward"):
+        b.backward(g, graph=False)
+    with pytest.raises(Exception, match="needs a forward"):
+        b.nearest()
+    b.forward(graph=False)
+    b.backward(g, graph=False)
+    b.nearest()
+    torch.cuda.synchronize()
