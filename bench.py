@@ -197,10 +197,11 @@ def voxel_work(counters, cells, vpl, kernel_ms):
     return out
 
 
-def bench_sdf(cm, W, steps, warmup):
+def bench_sdf(cm, W, steps, warmup, variant=None, full=True):
     """C5: 64 uneven meshes, SDF forward+backward per step, batched (one graph of
     persistent kernels) vs plain (one mesh at a time, synchronised per mesh, same kernels).
-    Also C3/C4 single-mesh forward throughput."""
+    Also C3/C4 single-mesh forward throughput. variant=(vpl, depth) skips the autotuner;
+    full=False times the batched path only (for profiling its kernels)."""
     import torch
     from cybermeshgenerator_amd.batch import SdfBatch
     log("[bench] generating C5 meshes")
@@ -217,8 +218,12 @@ def bench_sdf(cm, W, steps, warmup):
     sdf = torch.empty(cells, dtype=torch.float32, device="cuda")
     grad = torch.empty((batch.points.shape[0], 3), dtype=torch.float32, device="cuda")
 
-    log("[bench] autotuning the voxel passes")
-    tune = batch.autotune(gs, sdf=sdf, grad=grad)
+    if variant is None:
+        log("[bench] autotuning the voxel passes")
+        tune = batch.autotune(gs, sdf=sdf, grad=grad)
+    else:
+        batch.set_variant(*variant)
+        tune = {"vpl": variant[0], "depth": variant[1], "ms": None}
 
     def bstep():
         batch.step(gs, sdf=sdf, grad=grad, graph=True)
@@ -237,6 +242,14 @@ def bench_sdf(cm, W, steps, warmup):
     occ_ms, _ = cm.kernel_time("batch_occupancy")
     for t in timers:
         cm.kernel_timer(t, False)
+    if not full:
+        out = {"meshes": len(cases), "cells": cells, "triangles": tris, "batched_ms": ms_b,
+               "batched_voxels_per_s": cells / (ms_b / 1e3), "fwd_voxel_kernel_ms": fwd_ms,
+               "bwd_voxel_kernel_ms": bwd_ms, "occupancy_kernel_ms": occ_ms,
+               "variant": batch.variant_info()}
+        out.update(voxel_work(st, cells, batch.vpl(), fwd_ms + bwd_ms))
+        batch.close()
+        return out
     # plain: one engine per mesh, one mesh at a time
     log("[bench] planning 64 single-mesh engines")
     singles = [SdfBatch([m], [r], [p]) for m, r, p in zip(meshes, res, pads)]
@@ -294,6 +307,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
+    ap.add_argument("--sdf-only", action="store_true",
+                    help="time only the batched 64-mesh SDF step (no predicates, plain "
+                         "path or single meshes) and print its figures: for profiling")
+    ap.add_argument("--variant", default=None,
+                    help="voxel-pass variant VPLxDEPTH (e.g. 1x8) instead of the autotuner")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="also run one device-to-device copy of 2^30 bytes (a kernel of "
                          "known traffic, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -317,6 +335,13 @@ def main():
         torch.cuda.synchronize()
         log("[bench] calibration copy: 1073741824 bytes read, 1073741824 bytes written")
         del src, dst
+
+    variant = tuple(int(x) for x in args.variant.split("x")) if args.variant else None
+    if args.sdf_only:
+        print(json.dumps({"workload": "C5 batched SDF forward+backward, 64 meshes",
+                          "sdf": bench_sdf(cm, W, args.steps, args.warmup, variant,
+                                           full=False)}), flush=True)
+        return
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
     n = pr["n"]
@@ -344,7 +369,7 @@ def main():
 
     if not args.no_sdf:
         try:
-            extra["sdf"] = bench_sdf(cm, W, args.steps, args.warmup)
+            extra["sdf"] = bench_sdf(cm, W, args.steps, args.warmup, variant)
             if not args.no_cpu:
                 log("[bench] CPU reference SDF search sample (1 thread)")
                 extra["sdf"]["cpu_baseline"] = cpu_sdf_sample(W)

@@ -117,6 +117,7 @@ struct Ctx {  // kernel arguments shared by every phase
     TriRec* rec;                // per tree slot
     TriRec64* rec64;
     NodeRec* node;
+    float4* nctr;               // per node: box centre (x, y, z, 0)
     int4* xyr;                  // per triangle
     int* tile_count;
     int* tile_off;
@@ -155,8 +156,7 @@ __device__ __forceinline__ NodeRec load_node(const NodeRec* base, int i) {
     NodeRec r;
     float* d = reinterpret_cast<float*>(&r);
 #pragma unroll
-    for (int k = 0; k < 15; ++k) d[k] = p[k];
-    r.pad = 0.0f;
+    for (int k = 0; k < 16; ++k) d[k] = p[k];
     return r;
 }
 
@@ -164,12 +164,14 @@ __device__ __forceinline__ NodeRec load_node(const NodeRec* base, int i) {
 // inside (the float axes are orthonormal to a few ulps; the distance error margin covers
 // the difference).
 __device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, float pz) {
-    const float vx = px - N.cx, vy = py - N.cy, vz = pz - N.cz;
-    const float dn = fmaxf(fabsf(fma3(vx, N.nx, vy, N.ny, vz, N.nz)) - N.en, 0.0f);
-    const float du = fmaxf(fabsf(fma3(vx, N.ux, vy, N.uy, vz, N.uz)) - N.eu, 0.0f);
-    const float dw = fmaxf(fabsf(fma3(vx, N.wx, vy, N.wy, vz, N.wz)) - N.ew, 0.0f);
-    return fma3(dn, dn, du, du, dw, dw);
+    const float tn = fmaxf(fabsf(proj(N.nx, N.ny, N.nz, N.dn, px, py, pz)) - N.en, 0.0f);
+    const float tu = fmaxf(fabsf(proj(N.ux, N.uy, N.uz, N.du, px, py, pz)) - N.eu, 0.0f);
+    const float tw = fmaxf(fabsf(proj(N.wx, N.wy, N.wz, N.dw, px, py, pz)) - N.ew, 0.0f);
+    return fma3(tn, tn, tu, tu, tw, tw);
 }
+
+// true when the predicate holds on any lane of the wavefront
+__device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
 // ---- phases ----------------------------------------------------------------------------
 
@@ -601,9 +603,11 @@ __device__ __forceinline__ void normalize3(double* a) {
 // and the aligned normal sum N: axis n = N / |N| (or z), u the principal direction of the
 // points projected on the plane of n (closed-form 2x2 eigenvector), w = n x u. The axes
 // are rounded to float first and the extents measured along the float axes, so the float
-// box contains every point; the extents then grow by the rounding of the centre.
+// box contains every point; the extents then grow by the rounding of the centre's
+// coordinates and by the rounding error of the test's projection fma(n, p, -dn) for any
+// point p with |p|_1 <= pmax. The centre goes to ctr.
 template <class Points>
-__device__ inline NodeRec make_box(const double* N, Points&& points) {
+__device__ inline NodeRec make_box(const double* N, double pmax, float4& ctr, Points&& points) {
     double n[3] = {N[0], N[1], N[2]};
     if (dot3(n, n) > 0.0) {
         normalize3(n);
@@ -656,18 +660,17 @@ __device__ inline NodeRec make_box(const double* N, Points&& points) {
             hi[k] = fmax(hi[k], q);
         }
     });
-    double c[3] = {0.0, 0.0, 0.0}, e[3];
+    double c[3] = {0.0, 0.0, 0.0}, d[3], e[3];
     for (int k = 0; k < 3; ++k) {
-        const double mid = 0.5 * (lo[k] + hi[k]);
+        d[k] = 0.5 * (lo[k] + hi[k]);
         e[k] = 0.5 * (hi[k] - lo[k]);
-        for (int j = 0; j < 3; ++j) c[j] += mid * ax[k][j];
+        for (int j = 0; j < 3; ++j) c[j] += d[k] * ax[k][j];
     }
-    const double ext = fmax(fmax(fabs(c[0]), fabs(c[1])), fabs(c[2])) + e[0] + e[1] + e[2];
-    const double slack = 8.0 * 0x1p-24 * ext;
+    ctr = make_float4((float)c[0], (float)c[1], (float)c[2], 0.0f);
     NodeRec B;
-    B.cx = (float)c[0];
-    B.cy = (float)c[1];
-    B.cz = (float)c[2];
+    B.dn = (float)d[0];
+    B.du = (float)d[1];
+    B.dw = (float)d[2];
     B.nx = nf[0];
     B.ny = nf[1];
     B.nz = nf[2];
@@ -677,10 +680,10 @@ __device__ inline NodeRec make_box(const double* N, Points&& points) {
     B.wx = wf[0];
     B.wy = wf[1];
     B.wz = wf[2];
-    B.en = (float)(e[0] * (1.0 + 1e-6) + slack);
-    B.eu = (float)(e[1] * (1.0 + 1e-6) + slack);
-    B.ew = (float)(e[2] * (1.0 + 1e-6) + slack);
-    B.pad = 0.0f;
+    B.en = (float)(e[0] * (1.0 + 1e-6) + 8.0 * 0x1p-24 * (pmax + fabs(d[0])));
+    B.eu = (float)(e[1] * (1.0 + 1e-6) + 8.0 * 0x1p-24 * (pmax + fabs(d[1])));
+    B.ew = (float)(e[2] * (1.0 + 1e-6) + 8.0 * 0x1p-24 * (pmax + fabs(d[2])));
+    B.link = 0u;
     return B;
 }
 
@@ -713,7 +716,8 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
             const double sg = dot3(nt, ref) < 0.0 ? -1.0 : 1.0;
             for (int a = 0; a < 3; ++a) N[a] += sg * nt[a];
         }
-        C.node[M.node_base[1] + loc] = make_box(N, [&](auto&& f) {
+        float4 ctr;
+        NodeRec B = make_box(N, 3.0 * grid_extent(M.g), ctr, [&](auto&& f) {
             for (int k = 0; k < cnt; ++k) {
                 const TriRec64& T = C.rec64[s0 + k];
                 for (int v = 0; v < 3; ++v) {
@@ -724,6 +728,9 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
                 }
             }
         });
+        B.link = kLinkLeaf | ((unsigned)loc << 4) | (unsigned)cnt;
+        C.node[M.node_base[1] + loc] = B;
+        C.nctr[M.node_base[1] + loc] = ctr;
     }
 }
 
@@ -739,7 +746,8 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
         const int loc = gi - lo[m];
         const int first = 8 * loc;
         const int cnt = min(8, M.nnodes[level - 1] - first);
-        const NodeRec* ch = C.node + M.node_base[level - 1] + first;
+        const int cfirst = M.node_base[level - 1] + first;
+        const NodeRec* ch = C.node + cfirst;
         const double ref[3] = {ch[0].nx, ch[0].ny, ch[0].nz};
         double N[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < cnt; ++k) {
@@ -748,20 +756,25 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
             const double sg = dot3(nk, ref) < 0.0 ? -1.0 : 1.0;
             for (int a = 0; a < 3; ++a) N[a] += sg * wgt * nk[a];
         }
-        C.node[M.node_base[level] + loc] = make_box(N, [&](auto&& f) {
+        float4 ctr;
+        NodeRec B = make_box(N, 3.0 * grid_extent(M.g), ctr, [&](auto&& f) {
             for (int k = 0; k < cnt; ++k) {
                 const NodeRec K = ch[k];
+                const float4 KC = C.nctr[cfirst + k];
                 for (int corner = 0; corner < 8; ++corner) {
                     const double sn = (corner & 1) ? K.en : -(double)K.en;
                     const double su = (corner & 2) ? K.eu : -(double)K.eu;
                     const double sw = (corner & 4) ? K.ew : -(double)K.ew;
-                    const double q[3] = {K.cx + sn * K.nx + su * K.ux + sw * K.wx,
-                                         K.cy + sn * K.ny + su * K.uy + sw * K.wy,
-                                         K.cz + sn * K.nz + su * K.uz + sw * K.wz};
+                    const double q[3] = {KC.x + sn * K.nx + su * K.ux + sw * K.wx,
+                                         KC.y + sn * K.ny + su * K.uy + sw * K.wy,
+                                         KC.z + sn * K.nz + su * K.uz + sw * K.wz};
                     f(q);
                 }
             }
         });
+        B.link = ((unsigned)cfirst << 4) | (unsigned)cnt;
+        C.node[M.node_base[level] + loc] = B;
+        C.nctr[M.node_base[level] + loc] = ctr;
     }
 }
 
@@ -805,56 +818,68 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         const float wcx = (ix * kLeaf + 2) * h, wcy = (iy * kLeaf + 2) * h,
                     wcz = (iz * L0.sz + 0.5f * L0.sz) * h;
 
-        // Walks the tree for the lanes in act[] with squared bounds b2[] (which on_leaf may
-        // tighten); on_leaf(loc) visits the triangles of level-1 node loc. The stack is
-        // wave-uniform and lives in one VGPR, entry e in lane e (at most 1 + 7 (nlev - 1)
-        // <= 64 entries, nlev <= 10).
+        // Walks the tree with the squared bounds b2[] (which on_leaf may tighten; lanes
+        // that take no part hold a negative bound); on_leaf(loc, cnt) visits the cnt
+        // triangles of tree leaf loc. The stack is wave-uniform and lives in three VGPRs,
+        // entry e in lane e (at most 1 + 7 (nlev - 1) <= 64 entries, nlev <= 10): the
+        // node's global index, its child link and the version it was pushed at.
         // `ver` counts the leaf visits that tightened some lane's bound; a node pushed at
         // the current version passed its test with the current bounds and is not re-tested.
         int ver = 0;
-        auto walk = [&](const bool* act, const float* b2, auto&& on_leaf) {
-            int stk = M.nlev << 27;  // entry 0: the root
-            int stv = -1;            // version at push, entry e in lane e
+        const int root = M.node_base[M.nlev];
+        const long long tbase = M.tbase;
+        auto walk = [&](const float* b2, auto&& on_leaf) {
+            int stk = root;  // entry 0: the root, never tested before
+            int stl = 0;
+            int stv = -1;
             int sp = 1;
             while (sp > 0) {
                 --sp;
-                const int ref = __builtin_amdgcn_readlane(stk, sp);
-                const int pushed = __builtin_amdgcn_readlane(stv, sp);
-                const int lev = ref >> 27, loc = ref & 0x7ffffff;
-                if (pushed != ver) {  // the bounds have tightened since this node was pushed
+                const int node = __builtin_amdgcn_readlane(stk, sp);
+                unsigned link;
+                if (__builtin_amdgcn_readlane(stv, sp) != ver) {
+                    // the bounds have tightened since this node was pushed
                     ++nvis;
-                    const NodeRec N = load_node(C.node, M.node_base[lev] + loc);
+                    const NodeRec N = load_node(C.node, node);
                     bool need = false;
 #pragma unroll
-                    for (int v = 0; v < VPL; ++v)
-                        need |= act[v] && node_lb2(N, px, py, pz[v]) <= b2[v];
-                    if (__ballot(need) == 0) continue;
+                    for (int v = 0; v < VPL; ++v) need |= node_lb2(N, px, py, pz[v]) <= b2[v];
+                    if (!wave_any(need)) continue;
+                    link = N.link;
+                } else {
+                    link = (unsigned)__builtin_amdgcn_readlane(stl, sp);
                 }
-                if (lev == 1) {
-                    on_leaf(loc);
+                const int cnt = (int)(link & 15u), first = (int)((link >> 4) & 0x7ffffffu);
+                if (link & kLinkLeaf) {
+                    on_leaf(first, cnt);
                     continue;
                 }
-                const int first = 8 * loc;
-                const int cnt = min(8, M.nnodes[lev - 1] - first);
-                unsigned mask = 0;
+                // order key of child `lane`: squared distance from the voxel block's centre
+                // to the child's box centre (discriminates better than the lower bounds,
+                // which are 0 for every box that overlaps the block); non-negative, so its
+                // bits order as unsigned integers
+                float ckey = 0.0f;
+                if (lane < cnt) {
+                    const float4 cc = C.nctr[first + lane];
+                    const float kx = cc.x - wcx, ky = cc.y - wcy, kz = cc.z - wcz;
+                    ckey = fma3(kx, kx, ky, ky, kz, kz);
+                }
+                unsigned mask = 0, near_k = ~0u, clink[8];
                 int near_c = -1;
-                float near_k = FLT_MAX;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     if (c < cnt) {
                         ++nvis;
-                        const NodeRec K = load_node(C.node, M.node_base[lev - 1] + first + c);
+                        const NodeRec K = load_node(C.node, first + c);
+                        clink[c] = K.link;
                         bool need = false;
 #pragma unroll
                         for (int v = 0; v < VPL; ++v)
-                            need |= act[v] && node_lb2(K, px, py, pz[v]) <= b2[v];
-                        if (__ballot(need) != 0) {
+                            need |= node_lb2(K, px, py, pz[v]) <= b2[v];
+                        if (wave_any(need)) {
                             mask |= 1u << c;
-                            // order key: distance from the voxel block's centre to the box
-                            // centre (discriminates better than the lower bounds, which
-                            // are 0 for every box that overlaps the block)
-                            const float kx = K.cx - wcx, ky = K.cy - wcy, kz = K.cz - wcz;
-                            const float key = fma3(kx, kx, ky, ky, kz, kz);
+                            const unsigned key = (unsigned)__builtin_amdgcn_readlane(
+                                __builtin_bit_cast(int, ckey), c);
                             if (key < near_k) {
                                 near_k = key;
                                 near_c = c;
@@ -862,31 +887,34 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                         }
                     }
                 }
-                const int child = (lev - 1) << 27;
+                // the nearest child last, so it is visited first
 #pragma unroll
                 for (int c = 0; c < 8; ++c)
                     if (((mask >> c) & 1u) && c != near_c) {
-                        stk = lane == sp ? (child | (first + c)) : stk;
-                        stv = lane == sp ? ver : stv;
+                        const bool mine = lane == sp;
+                        stk = mine ? first + c : stk;
+                        stl = mine ? (int)clink[c] : stl;
+                        stv = mine ? ver : stv;
                         ++sp;
                     }
                 if (near_c >= 0) {
-                    stk = lane == sp ? (child | (first + near_c)) : stk;
-                    stv = lane == sp ? ver : stv;
+                    const bool mine = lane == sp;
+                    stk = mine ? first + near_c : stk;
+                    stl = mine ? (int)clink[near_c] : stl;
+                    stv = mine ? ver : stv;
                     ++sp;
                 }
             }
         };
 
         // triangles of one tree leaf, through LDS (D > 0) or scalar loads
-        auto leaf_rec = [&](int loc, int& cnt) -> long long {
-            const long long s0 = M.tbase + 8LL * loc;
-            cnt = min(8, M.ntris - 8 * loc);
+        auto leaf_rec = [&](int loc, int cnt) -> long long {
+            const long long s0 = tbase + 8LL * loc;
             if (D > 0) {
                 wave_sync();
-                if (lane < cnt * 8)
-                    reinterpret_cast<float4*>(s_rec)[lane] =
-                        reinterpret_cast<const float4*>(C.rec + s0)[lane];
+                const float4* src = reinterpret_cast<const float4*>(C.rec + s0);
+                float4* dst = reinterpret_cast<float4*>(s_rec);
+                for (int q = lane; q < cnt * kTriRecF4; q += 64) dst[q] = src[q];
                 wave_sync();
             }
             return s0;
@@ -902,17 +930,15 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             best[v] = FLT_MAX;
-            b2[v] = FLT_MAX;
+            b2[v] = valid[v] ? FLT_MAX : -1.0f;
             arg[v] = -1;
             if (BWD && valid[v]) {
                 const float bb =
-                    sqrtf(cell_best[((size_t)k[v] * g.ny + j) * g.nx + i]) * (1.0f + 4.0f * kU) +
-                    eps;
+                    bound_root(cell_best[((size_t)k[v] * g.ny + j) * g.nx + i]) + eps;
                 b2[v] = bb * bb;
             }
         }
-        walk(valid, b2, [&](int loc) {
-            int cnt;
+        walk(b2, [&](int loc, int cnt) {
             const long long s0 = leaf_rec(loc, cnt);
             for (int c = 0; c < cnt; ++c) {
                 const TriRec T = D > 0 ? s_rec[c] : load_uniform(C.rec, (int)(s0 + c));
@@ -920,8 +946,8 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 bool need = false;
 #pragma unroll
                 for (int v = 0; v < VPL; ++v)
-                    need |= valid[v] && tri_lb2(T, px, py, pz[v]) <= b2[v];
-                if (__ballot(need) == 0) continue;
+                    need |= tri_lb2(T, px, py, pz[v]) <= b2[v];
+                if (!wave_any(need)) continue;
                 ++neval;
                 bool tighter = false;
 #pragma unroll
@@ -934,11 +960,12 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                         best[v] = d2;
                         arg[v] = s;
                     }
-                    const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + eps;
-                    tighter |= bb * bb < b2[v];
-                    b2[v] = fminf(b2[v], bb * bb);
+                    const float bb = bound_root(best[v]) + eps;
+                    const bool lower = bb * bb < b2[v];
+                    tighter |= lower;
+                    b2[v] = lower ? bb * bb : b2[v];
                 }
-                if (__ballot(tighter) != 0) ++ver;
+                if (wave_any(tighter)) ++ver;
             }
         });
 
@@ -962,16 +989,15 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             p64[v][1] = g.corner[1] + (j + 0.5) * g.spacing;
             p64[v][2] = g.corner[2] + (k[v] + 0.5) * g.spacing;
         }
-        if (__ballot(any_near) != 0) {
+        if (wave_any(any_near)) {
             ++ver;  // new bounds: nothing on the stack was tested against them
             float b2n[VPL];
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
-                const float bb = sqrtf(best[v]) * (1.0f + 4.0f * kU) + 2.0f * eps;
-                b2n[v] = bb * bb;
+                const float bb = bound_root(best[v]) + 2.0f * eps;
+                b2n[v] = near[v] ? bb * bb : -1.0f;
             }
-            walk(near, b2n, [&](int loc) {
-                int cnt;
+            walk(b2n, [&](int loc, int cnt) {
                 const long long s0 = leaf_rec(loc, cnt);
                 for (int c = 0; c < cnt; ++c) {
                     const TriRec T = D > 0 ? s_rec[c] : load_uniform(C.rec, (int)(s0 + c));
@@ -979,10 +1005,10 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     bool anyn = false;
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) {
-                        need[v] = near[v] && tri_lb2(T, px, py, pz[v]) <= b2n[v];
+                        need[v] = tri_lb2(T, px, py, pz[v]) <= b2n[v];
                         anyn |= need[v];
                     }
-                    if (__ballot(anyn) == 0) continue;
+                    if (!wave_any(anyn)) continue;
                     const int s = (int)(s0 + c);
                     const TriRec64 T64 = load_uniform(C.rec64, s);
 #pragma unroll
@@ -1387,7 +1413,8 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         lo[nm] = (int)rel;
         node_total += rel;
     }
-    if (node_total >= (1LL << 31)) throw InvalidInput("batch: tree too large");
+    if (node_total >= kMaxNodes)
+        throw InvalidInput("batch: more than 2^27 tree nodes (about 10^9 triangles) in one batch");
     I.total_cells = cells;
     I.total_tiles = tiles;
     I.total_cols_words = words;
@@ -1436,6 +1463,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.rec = I.dalloc<TriRec>(I.total_tris);
     C.rec64 = I.dalloc<TriRec64>(I.total_tris);
     C.node = I.dalloc<NodeRec>(node_total);
+    C.nctr = I.dalloc<float4>(node_total);
     C.xyr = I.dalloc<int4>(I.total_tris);
     C.tile_count = I.dalloc<int>(tiles);
     C.tile_off = I.dalloc<int>(tiles);

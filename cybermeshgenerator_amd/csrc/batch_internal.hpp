@@ -19,13 +19,19 @@ constexpr int kMaxLevels = 10;
 
 // Oriented bounding box of a tree node (coordinates relative to the grid corner): the
 // first axis is the patch's aligned mean normal, the other two its principal in-plane
-// directions; en/eu/ew are the half-extents along n/u/w.
+// directions; dn/du/dw are the box centre's coordinates along n/u/w and en/eu/ew the
+// half-extents. link = (first << 4) | count of the children: on level 1 (bit 31 set) the
+// node's tree leaf index in its mesh and its triangle count, above it the global index of
+// the first child node. The box centre itself is kept apart (Ctx::nctr).
 struct __align__(16) NodeRec {
-    float cx, cy, cz, en;
-    float nx, ny, nz, eu;
-    float ux, uy, uz, ew;
-    float wx, wy, wz, pad;
+    float nx, ny, nz, dn;
+    float ux, uy, uz, du;
+    float wx, wy, wz, dw;
+    float en, eu, ew;
+    unsigned link;
 };
+constexpr unsigned kLinkLeaf = 1u << 31;
+constexpr long long kMaxNodes = 1LL << 27;  // global node indices and links hold 27 bits
 static_assert(sizeof(NodeRec) == 64, "node size");
 
 // Per-mesh description, uploaded once at creation.

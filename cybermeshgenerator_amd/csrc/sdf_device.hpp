@@ -23,16 +23,18 @@ constexpr float kU = 0x1p-24f;    // FP32 unit roundoff
 // test, and an oriented bounding box (axes n, u along the longest edge, w = n x u;
 // centre c; half-extents en, eu, ew) for the lower bound.
 struct __align__(16) TriRec {
-    float cx, cy, cz, en;      // box centre, half-extent along n
+    float dn, du, dw, en;      // box centre along n, u, w; half-extent along n
     float ax, ay, az, ub;      // vertex a; ub = 1 (or -1: no interior, degenerate)
     float abx, aby, abz, eu;   // b - a; half-extent along u
     float acx, acy, acz, ew;   // c - a; half-extent along w
     float ux, uy, uz, nx;      // box axis u; unit normal x
     float wx, wy, wz, ny;      // box axis w; unit normal y
     float g1x, g1y, g1z, nz;   // dual basis: v = (p - a).g1, w = (p - a).g2; normal z
-    float g2x, g2y, g2z, pad;  // (degenerate triangle: n = (0,0,1), u = x, w = y)
+    float g2x, g2y, g2z, iab;  // (degenerate triangle: n = (0,0,1), u = x, w = y)
+    float iac, ibc, pad0, pad1; // iab, iac, ibc: 1 / squared edge length (0 for no length)
 };
-static_assert(sizeof(TriRec) == 128, "record size");
+static_assert(sizeof(TriRec) == 144, "record size");
+constexpr int kTriRecF4 = (int)(sizeof(TriRec) / 16);  // float4s per record
 
 struct __align__(8) TriRec64 {  // FP64 vertices for the near-surface re-evaluation
     double a[3], b[3], c[3];
@@ -68,33 +70,47 @@ __device__ __forceinline__ float fma3(float a, float b, float c, float d, float 
     return __builtin_fmaf(a, b, __builtin_fmaf(c, d, e * f));
 }
 
+// a . p - d with three roundings: the signed distance of p from the plane a . x = d
+__device__ __forceinline__ float proj(float ax, float ay, float az, float d, float px, float py,
+                                      float pz) {
+    return __builtin_fmaf(ax, px, __builtin_fmaf(ay, py, __builtin_fmaf(az, pz, -d)));
+}
+
+// Largest grid dimension times the spacing: every cell centre p (relative to the grid
+// corner) has 0 <= p_i <= grid_extent.
+__host__ __device__ inline double grid_extent(const GridSpec& g) {
+    const int m = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+    return m * g.spacing;
+}
+
 __device__ __forceinline__ float clamp01(float x) {
     return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f);
 }
 
-// Squared FP32 distance from p (relative to the corner) to the triangle.
-__device__ __forceinline__ float inv_len2(float x, float y, float z) {
-    const float l = fma3(x, x, y, y, z, z);
-    return l > 0.0f ? 1.0f / l : 0.0f;
+// At least sqrt(x) (1 + 4u), from the hardware square root (within 1 ulp) and one product:
+// the scale of the FP32 search bounds.
+__device__ __forceinline__ float bound_root(float x) {
+    return __builtin_amdgcn_sqrtf(x) * (1.0f + 8.0f * kU);
 }
 
+// Squared FP32 distance from p (relative to the corner) to the triangle.
 __device__ __forceinline__ float tri_d2(const TriRec& T, float px, float py, float pz) {
     const float apx = px - T.ax, apy = py - T.ay, apz = pz - T.az;
     const float bcx = T.acx - T.abx, bcy = T.acy - T.aby, bcz = T.acz - T.abz;
     // edge ab
-    float t = clamp01(fma3(apx, T.abx, apy, T.aby, apz, T.abz) * inv_len2(T.abx, T.aby, T.abz));
+    float t = clamp01(fma3(apx, T.abx, apy, T.aby, apz, T.abz) * T.iab);
     float ex = __builtin_fmaf(-t, T.abx, apx), ey = __builtin_fmaf(-t, T.aby, apy),
           ez = __builtin_fmaf(-t, T.abz, apz);
     float d = fma3(ex, ex, ey, ey, ez, ez);
     // edge ac
-    t = clamp01(fma3(apx, T.acx, apy, T.acy, apz, T.acz) * inv_len2(T.acx, T.acy, T.acz));
+    t = clamp01(fma3(apx, T.acx, apy, T.acy, apz, T.acz) * T.iac);
     ex = __builtin_fmaf(-t, T.acx, apx);
     ey = __builtin_fmaf(-t, T.acy, apy);
     ez = __builtin_fmaf(-t, T.acz, apz);
     d = fminf(d, fma3(ex, ex, ey, ey, ez, ez));
     // edge bc, from b
     const float bpx = apx - T.abx, bpy = apy - T.aby, bpz = apz - T.abz;
-    t = clamp01(fma3(bpx, bcx, bpy, bcy, bpz, bcz) * inv_len2(bcx, bcy, bcz));
+    t = clamp01(fma3(bpx, bcx, bpy, bcy, bpz, bcz) * T.ibc);
     ex = __builtin_fmaf(-t, bcx, bpx);
     ey = __builtin_fmaf(-t, bcy, bpy);
     ez = __builtin_fmaf(-t, bcz, bpz);
@@ -247,7 +263,7 @@ __device__ __forceinline__ TriRec load_uniform(const TriRec* base, int t) {
     TriRec r;
     float* d = reinterpret_cast<float*>(&r);
 #pragma unroll
-    for (int i = 0; i < 32; ++i) d[i] = p[i];
+    for (int i = 0; i < 36; ++i) d[i] = p[i];
     return r;
 }
 
@@ -296,6 +312,20 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     r.acx = (float)AC[0];
     r.acy = (float)AC[1];
     r.acz = (float)AC[2];
+    // 1 / |e|^2 of the float edges (e = ab, ac and bc = ac - ab as the distance forms
+    // them), rounded once; 0 when the edge has no length
+    {
+        const double e[3][3] = {{r.abx, r.aby, r.abz},
+                                {r.acx, r.acy, r.acz},
+                                {(double)(r.acx - r.abx), (double)(r.acy - r.aby),
+                                 (double)(r.acz - r.abz)}};
+        float* inv[3] = {&r.iab, &r.iac, &r.ibc};
+        for (int q = 0; q < 3; ++q) {
+            const double l = e[q][0] * e[q][0] + e[q][1] * e[q][1] + e[q][2] * e[q][2];
+            *inv[q] = l > 0.0 ? (float)fmin(1.0 / l, (double)FLT_MAX) : 0.0f;
+        }
+    }
+    r.pad0 = r.pad1 = 0.0f;
     // box axes: the unit normal, the longest edge's direction and their cross product;
     // the coordinate axes for a degenerate triangle
     double n[3] = {0.0, 0.0, 1.0}, u[3] = {1.0, 0.0, 0.0}, w[3] = {0.0, 1.0, 0.0};
@@ -315,7 +345,8 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
         w[2] = n[0] * u[1] - n[1] * u[0];
     }
     // extents measured along the float axes (so the float box holds the vertices), grown
-    // by the rounding of the centre and of the vertices themselves
+    // by the rounding of the vertices and of the centre and by the rounding error of the
+    // test's projection fma(n, p, -dn) for any point p of the grid (|p|_1 <= pmax)
     float nf[3], uf[3], wf[3];
     for (int a = 0; a < 3; ++a) {
         nf[a] = (float)n[a];
@@ -337,15 +368,16 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
             hi[k] = fmax(hi[k], d);
         }
     }
-    double c[3] = {0.0, 0.0, 0.0}, e[3];
+    const double pmax = 3.0 * grid_extent(g);
+    double d[3], e[3];
     for (int k = 0; k < 3; ++k) {
-        const double mid = 0.5 * (lo[k] + hi[k]);
-        e[k] = 0.5 * (hi[k] - lo[k]) * (1.0 + 1e-6) + 8.0 * 0x1p-24 * ext;
-        for (int a = 0; a < 3; ++a) c[a] += mid * axf[k][a];
+        d[k] = 0.5 * (lo[k] + hi[k]);
+        e[k] = 0.5 * (hi[k] - lo[k]) * (1.0 + 1e-6) +
+               8.0 * 0x1p-24 * (ext + pmax + fabs(d[k]));
     }
-    r.cx = (float)c[0];
-    r.cy = (float)c[1];
-    r.cz = (float)c[2];
+    r.dn = (float)d[0];
+    r.du = (float)d[1];
+    r.dw = (float)d[2];
     r.en = (float)e[0];
     r.eu = (float)e[1];
     r.ew = (float)e[2];
@@ -358,16 +390,14 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     r.nx = nf[0];
     r.ny = nf[1];
     r.nz = nf[2];
-    r.pad = 0.0f;
 }
 
 // Squared distance from p to the triangle's oriented box: a lower bound on its distance.
 __device__ __forceinline__ float tri_lb2(const TriRec& T, float px, float py, float pz) {
-    const float vx = px - T.cx, vy = py - T.cy, vz = pz - T.cz;
-    const float dn = fmaxf(fabsf(fma3(vx, T.nx, vy, T.ny, vz, T.nz)) - T.en, 0.0f);
-    const float du = fmaxf(fabsf(fma3(vx, T.ux, vy, T.uy, vz, T.uz)) - T.eu, 0.0f);
-    const float dw = fmaxf(fabsf(fma3(vx, T.wx, vy, T.wy, vz, T.wz)) - T.ew, 0.0f);
-    return fma3(dn, dn, du, du, dw, dw);
+    const float tn = fmaxf(fabsf(proj(T.nx, T.ny, T.nz, T.dn, px, py, pz)) - T.en, 0.0f);
+    const float tu = fmaxf(fabsf(proj(T.ux, T.uy, T.uz, T.du, px, py, pz)) - T.eu, 0.0f);
+    const float tw = fmaxf(fabsf(proj(T.wx, T.wy, T.wz, T.dw, px, py, pz)) - T.ew, 0.0f);
+    return fma3(tn, tn, tu, tu, tw, tw);
 }
 // Distances at or below this are zero up to the FP64 rounding of the coordinates: such
 // a cell lies on the surface, where the unsigned distance has a kink, and carries no
