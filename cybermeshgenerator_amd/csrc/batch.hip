@@ -117,7 +117,7 @@ struct Ctx {  // kernel arguments shared by every phase
     TriRec* rec;                // per tree slot
     TriRec64* rec64;
     NodeRec* node;
-    float4* nctr;               // per node: box centre (x, y, z, 0)
+    float4* nctr;               // per node: box centre (x, y, z) and, as bits, its link
     int4* xyr;                  // per triangle
     int* tile_count;
     int* tile_off;
@@ -729,6 +729,7 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
             }
         });
         B.link = kLinkLeaf | ((unsigned)loc << 4) | (unsigned)cnt;
+        ctr.w = __uint_as_float(B.link);
         C.node[M.node_base[1] + loc] = B;
         C.nctr[M.node_base[1] + loc] = ctr;
     }
@@ -773,6 +774,7 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
             }
         });
         B.link = ((unsigned)cfirst << 4) | (unsigned)cnt;
+        ctr.w = __uint_as_float(B.link);
         C.node[M.node_base[level] + loc] = B;
         C.nctr[M.node_base[level] + loc] = ctr;
     }
@@ -859,19 +861,20 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 // which are 0 for every box that overlaps the block); non-negative, so its
                 // bits order as unsigned integers
                 float ckey = 0.0f;
+                int clink = 0;  // link of child `lane`
                 if (lane < cnt) {
                     const float4 cc = C.nctr[first + lane];
                     const float kx = cc.x - wcx, ky = cc.y - wcy, kz = cc.z - wcz;
                     ckey = fma3(kx, kx, ky, ky, kz, kz);
+                    clink = __float_as_int(cc.w);
                 }
-                unsigned mask = 0, near_k = ~0u, clink[8];
+                unsigned mask = 0, near_k = ~0u;
                 int near_c = -1;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     if (c < cnt) {
                         ++nvis;
                         const NodeRec K = load_node(C.node, first + c);
-                        clink[c] = K.link;
                         bool need = false;
 #pragma unroll
                         for (int v = 0; v < VPL; ++v)
@@ -893,14 +896,14 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     if (((mask >> c) & 1u) && c != near_c) {
                         const bool mine = lane == sp;
                         stk = mine ? first + c : stk;
-                        stl = mine ? (int)clink[c] : stl;
+                        stl = mine ? __builtin_amdgcn_readlane(clink, c) : stl;
                         stv = mine ? ver : stv;
                         ++sp;
                     }
                 if (near_c >= 0) {
                     const bool mine = lane == sp;
                     stk = mine ? first + near_c : stk;
-                    stl = mine ? (int)clink[near_c] : stl;
+                    stl = mine ? __builtin_amdgcn_readlane(clink, near_c) : stl;
                     stv = mine ? ver : stv;
                     ++sp;
                 }
