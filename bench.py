@@ -365,7 +365,10 @@ def main():
         "filter_kernel_ms": ins["filter_ms"],
         "filter_hbm_gbs": INSPHERE_BYTES * n5 / (ins["filter_ms"] / 1e3) / 1e9,
         "stage_fraction": {"fp32": s5["fp32"] / n5, "fp64": s5["fp64"] / n5,
-                           "exact": s5["exact"] / n5}}
+                           "exact": s5["exact"] / n5,
+                           "exact_fixed_point": s5["exact_inline"] / n5,
+                           "expansion": (s5["exact"] - s5["exact_inline"]) / n5},
+        "wide_lanes": s5["wide"], "failed_lanes": s5["failed"]}
 
     if not args.no_sdf:
         try:

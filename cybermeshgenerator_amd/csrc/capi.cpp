@@ -400,7 +400,7 @@ cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
         out->fp32 = c.fp32;
         out->fp64 = c.fp64;
         out->exact = c.exact + c.inline_;
-        out->exact_inline = c.inline_;
+        out->exact_inline = c.inline_ + c.fixed;
         out->wide = c.wide_n;
         out->failed = c.failed;
     });

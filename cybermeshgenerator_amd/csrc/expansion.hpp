@@ -22,7 +22,7 @@ namespace cmg::hip {
 
 constexpr int kOverflow = 2;  // sign slot value meaning "capacity exceeded"
 
-__device__ __forceinline__ void two_sum(double a, double b, double& x, double& y) {
+__host__ __device__ __forceinline__ void two_sum(double a, double b, double& x, double& y) {
     double s = a + b;
     double bv = s - a;
     double av = s - bv;
@@ -30,13 +30,13 @@ __device__ __forceinline__ void two_sum(double a, double b, double& x, double& y
     y = (a - av) + (b - bv);
 }
 
-__device__ __forceinline__ void fast_two_sum(double a, double b, double& x, double& y) {
+__host__ __device__ __forceinline__ void fast_two_sum(double a, double b, double& x, double& y) {
     double s = a + b;
     x = s;
     y = b - (s - a);
 }
 
-__device__ __forceinline__ void two_prod(double a, double b, double& x, double& y) {
+__host__ __device__ __forceinline__ void two_prod(double a, double b, double& x, double& y) {
     double p = a * b;
     x = p;
     y = __builtin_fma(a, b, -p);
@@ -297,7 +297,7 @@ struct DW {
     double hi, lo;
 };
 
-__device__ __forceinline__ DW dw_mul(DW x, DW y) {
+__host__ __device__ __forceinline__ DW dw_mul(DW x, DW y) {
     double ch, cl1;
     two_prod(x.hi, y.hi, ch, cl1);
     const double tl0 = x.lo * y.lo;
@@ -308,7 +308,7 @@ __device__ __forceinline__ DW dw_mul(DW x, DW y) {
     return z;
 }
 
-__device__ __forceinline__ DW dw_add(DW x, DW y) {
+__host__ __device__ __forceinline__ DW dw_add(DW x, DW y) {
     double sh, sl, th, tl;
     two_sum(x.hi, y.hi, sh, sl);
     two_sum(x.lo, y.lo, th, tl);
@@ -321,7 +321,7 @@ __device__ __forceinline__ DW dw_add(DW x, DW y) {
     return z;
 }
 
-__device__ __forceinline__ DW dw_neg(DW x) { return DW{-x.hi, -x.lo}; }
+__host__ __device__ __forceinline__ DW dw_neg(DW x) { return DW{-x.hi, -x.lo}; }
 
 // orient3d sign from exact double-word differences evaluated in double-word arithmetic.
 // Each of the 15 operations adds at most ~5u^2 of the magnitudes it combines, all bounded
@@ -378,12 +378,433 @@ __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
     return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
 }
 
+// ---- fixed-point exact insphere ---------------------------------------------------
+//
+// Each difference p - pe is exactly x + y (Two-Sum). When all twelve are integer
+// multiples of one power of two 2^s with magnitudes below 2^(s+62), the determinant is an
+// integer polynomial in 62-bit integers D = (x + y) / 2^s and is evaluated exactly in two's
+// complement multi-limb integers held in registers: lifts (unsigned, < 2^126), xy minors
+// (signed 128-bit), cofactors (192-bit), lifted products and their sum (320-bit).
+
+typedef unsigned __int128 u128;
+
+struct I192 { unsigned long long w[3]; };  // two's complement
+struct I320 { unsigned long long w[5]; };
+
+// w = -w when neg (two's complement, without branches)
+template <int N>
+__host__ __device__ __forceinline__ void limbs_neg(unsigned long long (&w)[N], bool neg = true) {
+    const unsigned long long m = 0ull - (unsigned long long)neg;
+    unsigned long long c = (unsigned long long)neg;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const u128 t = (u128)(w[i] ^ m) + c;
+        w[i] = (unsigned long long)t;
+        c = (unsigned long long)(t >> 64);
+    }
+}
+
+// a * b for |a|, |b| < 2^63 as a signed 128-bit integer (magnitudes, then the sign)
+__host__ __device__ __forceinline__ __int128 smul(long long a, long long b) {
+    const unsigned long long ua = a < 0 ? 0ull - (unsigned long long)a : (unsigned long long)a;
+    const unsigned long long ub = b < 0 ? 0ull - (unsigned long long)b : (unsigned long long)b;
+    const u128 p = (u128)ua * ub;
+    const u128 m = (u128)0 - (u128)((a < 0) != (b < 0));
+    return (__int128)((p ^ m) - m);
+}
+
+template <int N>
+__host__ __device__ __forceinline__ void limbs_add(unsigned long long (&a)[N],
+                                          const unsigned long long (&b)[N]) {
+    unsigned long long c = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const u128 t = (u128)a[i] + b[i] + c;
+        a[i] = (unsigned long long)t;
+        c = (unsigned long long)(t >> 64);
+    }
+}
+
+// z * m (z < 2^62 in magnitude, m a signed 128-bit minor) as a 192-bit integer
+__host__ __device__ __forceinline__ I192 mul_z_minor(long long z, __int128 m) {
+    const bool neg = (z < 0) != (m < 0);
+    const unsigned long long az = z < 0 ? (unsigned long long)(-z) : (unsigned long long)z;
+    const u128 am = m < 0 ? (u128)(-m) : (u128)m;
+    const u128 p0 = (u128)az * (unsigned long long)am;
+    const u128 p1 = (u128)az * (unsigned long long)(am >> 64);
+    I192 r;
+    r.w[0] = (unsigned long long)p0;
+    const u128 mid = (p0 >> 64) + (unsigned long long)p1;
+    r.w[1] = (unsigned long long)mid;
+    r.w[2] = (unsigned long long)(p1 >> 64) + (unsigned long long)(mid >> 64);
+    limbs_neg(r.w, neg);
+    return r;
+}
+
+// lift * c (lift < 2^126 unsigned, c a signed 192-bit cofactor) as a 320-bit integer
+__host__ __device__ __forceinline__ I320 mul_lift_cof(u128 lift, I192 c) {
+    const bool neg = (long long)c.w[2] < 0;
+    limbs_neg(c.w, neg);
+    const unsigned long long l[2] = {(unsigned long long)lift, (unsigned long long)(lift >> 64)};
+    I320 r{{0, 0, 0, 0, 0}};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        unsigned long long carry = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const u128 t = (u128)l[i] * c.w[j] + r.w[i + j] + carry;
+            r.w[i + j] = (unsigned long long)t;
+            carry = (unsigned long long)(t >> 64);
+        }
+        r.w[i + 3] += carry;  // i + 3 <= 4; no carry out: |product| < 2^316
+    }
+    limbs_neg(r.w, neg);
+    return r;
+}
+
+// Exponent range of a nonzero finite normal double: |x| = mant * 2^e, lowest set bit at
+// 2^low, |x| < 2^high. False for subnormals, infinities and NaNs.
+__host__ __device__ __forceinline__ bool bit_range(double x, int& low, int& high) {
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
+    const int ef = (int)((bits >> 52) & 0x7ff);
+    if (ef == 0 || ef == 0x7ff) return false;
+    const unsigned long long mant = (bits & 0xfffffffffffffull) | (1ull << 52);
+    const int e = ef - 1075;
+    low = e + __builtin_ctzll(mant);
+    high = e + 53;
+    return true;
+}
+
+// x / 2^s as an integer, for a double x whose lowest set bit is at or above 2^s
+__host__ __device__ __forceinline__ long long scaled_int(double x, int s) {
+    if (x == 0.0) return 0;
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
+    const unsigned long long mant = (bits & 0xfffffffffffffull) | (1ull << 52);
+    const int sh = (int)((bits >> 52) & 0x7ff) - 1075 - s;
+    const long long q = (long long)(sh >= 0 ? mant << sh : mant >> -sh);
+    return (bits >> 63) ? -q : q;
+}
+
+// Exact insphere sign in fixed point; false when a difference is subnormal or the
+// differences span more than 62 bits (the caller then uses expansion arithmetic).
+__host__ __device__ inline bool insphere_sign_fixed(const double* pa, const double* pb, const double* pc,
+                                           const double* pd, const double* pe, int& sign) {
+    const double* P[4] = {pa, pb, pc, pd};
+    double dx[12], dy[12];
+    int lo = 1 << 20, hi = -(1 << 20);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(P[v][a], -pe[a], x, y);
+            dx[3 * v + a] = x;
+            dy[3 * v + a] = y;
+            const double xy[2] = {x, y};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (xy[q] == 0.0) continue;
+                int l, h;
+                if (!bit_range(xy[q], l, h)) return false;
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+        }
+    if (hi < lo) {  // every difference is zero
+        sign = 0;
+        return true;
+    }
+    if (hi - lo > 62) return false;
+    // Stay where every product of five differences and its rounding-error terms is a
+    // normal double (components within [2^-150, 2^150]): there the expansion arithmetic
+    // is exact too, so both stages give the same sign; outside, leave the lane to it.
+    if (hi > 150 || lo < -150) return false;
+    long long D[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) D[i] = scaled_int(dx[i], lo) + scaled_int(dy[i], lo);
+    // D[3v + a]: vertex v (a, b, c, d) minus e, axis a
+    u128 lift[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const long long x = D[3 * v], y = D[3 * v + 1], z = D[3 * v + 2];
+        const unsigned long long ax = x < 0 ? -x : x, ay = y < 0 ? -y : y, az = z < 0 ? -z : z;
+        lift[v] = (u128)ax * ax + (u128)ay * ay + (u128)az * az;
+    }
+    auto minor = [&](int u, int v) -> __int128 {  // uex * vey - vex * uey
+        return smul(D[3 * u], D[3 * v + 1]) - smul(D[3 * v], D[3 * u + 1]);
+    };
+    constexpr int A = 0, B = 1, C = 2, Dv = 3;
+    const __int128 ab = minor(A, B), bc = minor(B, C), cd = minor(C, Dv), da = minor(Dv, A),
+                   ac = minor(A, C), bd = minor(B, Dv);
+    auto z = [&](int v) { return D[3 * v + 2]; };
+    auto cof = [&](int v0, __int128 m0, int v1, __int128 m1, int v2, __int128 m2) {
+        I192 r = mul_z_minor(z(v0), m0);
+        const I192 t1 = mul_z_minor(z(v1), m1), t2 = mul_z_minor(z(v2), m2);
+        limbs_add(r.w, t1.w);
+        limbs_add(r.w, t2.w);
+        return r;
+    };
+    // abc = aez*bc - bez*ac + cez*ab      bcd = bez*cd - cez*bd + dez*bc
+    // cda = cez*da + dez*ac + aez*cd      dab = dez*ab + aez*bd + bez*da
+    const I192 abc = cof(A, bc, B, -ac, C, ab);
+    const I192 bcd = cof(B, cd, C, -bd, Dv, bc);
+    const I192 cda = cof(C, da, Dv, ac, A, cd);
+    const I192 dab = cof(Dv, ab, A, bd, B, da);
+    // det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd)
+    I320 det = mul_lift_cof(lift[Dv], abc);
+    I320 t = mul_lift_cof(lift[C], dab);
+    limbs_neg(t.w);
+    limbs_add(det.w, t.w);
+    t = mul_lift_cof(lift[B], cda);
+    limbs_add(det.w, t.w);
+    t = mul_lift_cof(lift[A], bcd);
+    limbs_neg(t.w);
+    limbs_add(det.w, t.w);
+    if ((det.w[0] | det.w[1] | det.w[2] | det.w[3] | det.w[4]) == 0) {
+        sign = 0;
+    } else {
+        sign = (long long)det.w[4] < 0 ? -1 : 1;
+    }
+    return true;
+}
+
+// ---- wide fixed point: the same evaluation for differences spanning up to 126 bits ----
+
+// r = a * b (unsigned, little-endian 64-bit limbs)
+template <int NA, int NB>
+__host__ __device__ __forceinline__ void limbs_mul(const unsigned long long (&a)[NA],
+                                                   const unsigned long long (&b)[NB],
+                                                   unsigned long long (&r)[NA + NB]) {
+#pragma unroll
+    for (int i = 0; i < NA + NB; ++i) r[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        unsigned long long carry = 0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const u128 t = (u128)a[i] * b[j] + r[i + j] + carry;
+            r[i + j] = (unsigned long long)t;
+            carry = (unsigned long long)(t >> 64);
+        }
+        r[i + NB] = carry;
+    }
+}
+
+// w = |w| of a two's complement value; returns whether it was negative
+template <int N>
+__host__ __device__ __forceinline__ bool limbs_abs(unsigned long long (&w)[N]) {
+    const bool neg = (long long)w[N - 1] < 0;
+    limbs_neg(w, neg);
+    return neg;
+}
+
+// signed a * b, both two's complement, into NA + NB limbs (the magnitudes must leave the
+// top bit of the product clear; a and b must not be the same array)
+template <int NA, int NB>
+__host__ __device__ __forceinline__ void limbs_smul(unsigned long long (&a)[NA],
+                                                    unsigned long long (&b)[NB],
+                                                    unsigned long long (&r)[NA + NB]) {
+    const bool na = limbs_abs(a), nb = limbs_abs(b);
+    limbs_mul(a, b, r);
+    limbs_neg(r, na != nb);
+    limbs_neg(a, na);  // restore the operands
+    limbs_neg(b, nb);
+}
+
+// x / 2^s as a 128-bit integer, for a double x whose lowest set bit is at or above 2^s and
+// whose magnitude is below 2^(s+126)
+__host__ __device__ __forceinline__ __int128 scaled_int128(double x, int s) {
+    if (x == 0.0) return 0;
+    const unsigned long long bits = __builtin_bit_cast(unsigned long long, x);
+    const u128 mant = (bits & 0xfffffffffffffull) | (1ull << 52);
+    const int sh = (int)((bits >> 52) & 0x7ff) - 1075 - s;
+    const u128 q = sh >= 0 ? mant << sh : mant >> -sh;
+    return (bits >> 63) ? -(__int128)q : (__int128)q;
+}
+
+// Exact insphere sign in fixed point with 126-bit differences: lifts and minors in 256
+// bits, cofactors in 384, lifted products and their sum in 640. False when a component
+// is outside [2^-150, 2^150] or the differences span more than 126 bits.
+__host__ __device__ inline bool insphere_sign_fixed_wide(const double* pa, const double* pb,
+                                                         const double* pc, const double* pd,
+                                                         const double* pe, int& sign) {
+    const double* P[4] = {pa, pb, pc, pd};
+    double dx[12], dy[12];
+    int lo = 1 << 20, hi = -(1 << 20);
+    for (int v = 0; v < 4; ++v)
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(P[v][a], -pe[a], x, y);
+            dx[3 * v + a] = x;
+            dy[3 * v + a] = y;
+            const double xy[2] = {x, y};
+            for (int q = 0; q < 2; ++q) {
+                if (xy[q] == 0.0) continue;
+                int l, h;
+                if (!bit_range(xy[q], l, h)) return false;
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+        }
+    if (hi < lo) {
+        sign = 0;
+        return true;
+    }
+    if (hi - lo > 126 || hi > 150 || lo < -150) return false;
+    unsigned long long D[12][2];
+    for (int i = 0; i < 12; ++i) {
+        const __int128 q = scaled_int128(dx[i], lo) + scaled_int128(dy[i], lo);
+        D[i][0] = (unsigned long long)q;
+        D[i][1] = (unsigned long long)((u128)q >> 64);
+    }
+    typedef unsigned long long L4[4];
+    typedef unsigned long long L6[6];
+    typedef unsigned long long L10[10];
+    L4 lift[4];
+    for (int v = 0; v < 4; ++v) {
+        for (int i = 0; i < 4; ++i) lift[v][i] = 0;
+        for (int a = 0; a < 3; ++a) {
+            unsigned long long x[2] = {D[3 * v + a][0], D[3 * v + a][1]};
+            limbs_abs(x);
+            L4 sq;
+            limbs_mul(x, x, sq);
+            limbs_add(lift[v], sq);
+        }
+    }
+    auto minor = [&](int u, int v, L4& m) {  // uex * vey - vex * uey
+        L4 t;
+        limbs_smul(D[3 * u], D[3 * v + 1], m);
+        limbs_smul(D[3 * v], D[3 * u + 1], t);
+        limbs_neg(t);
+        limbs_add(m, t);
+    };
+    constexpr int A = 0, B = 1, C = 2, Dv = 3;
+    L4 ab, bc, cd, da, ac, bd;
+    minor(A, B, ab);
+    minor(B, C, bc);
+    minor(C, Dv, cd);
+    minor(Dv, A, da);
+    minor(A, C, ac);
+    minor(B, Dv, bd);
+    auto cof = [&](L6& r, int v0, L4& m0, bool n0, int v1, L4& m1, bool n1, int v2, L4& m2,
+                   bool n2) {
+        L6 t;
+        limbs_smul(D[3 * v0 + 2], m0, r);
+        limbs_neg(r, n0);
+        limbs_smul(D[3 * v1 + 2], m1, t);
+        limbs_neg(t, n1);
+        limbs_add(r, t);
+        limbs_smul(D[3 * v2 + 2], m2, t);
+        limbs_neg(t, n2);
+        limbs_add(r, t);
+    };
+    // abc = aez*bc - bez*ac + cez*ab      bcd = bez*cd - cez*bd + dez*bc
+    // cda = cez*da + dez*ac + aez*cd      dab = dez*ab + aez*bd + bez*da
+    L6 abc, bcd, cda, dab;
+    cof(abc, A, bc, false, B, ac, true, C, ab, false);
+    cof(bcd, B, cd, false, C, bd, true, Dv, bc, false);
+    cof(cda, C, da, false, Dv, ac, false, A, cd, false);
+    cof(dab, Dv, ab, false, A, bd, false, B, da, false);
+    // det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd)
+    L10 det, t;
+    limbs_smul(lift[Dv], abc, det);
+    limbs_smul(lift[C], dab, t);
+    limbs_neg(t);
+    limbs_add(det, t);
+    limbs_smul(lift[B], cda, t);
+    limbs_add(det, t);
+    limbs_smul(lift[A], bcd, t);
+    limbs_neg(t);
+    limbs_add(det, t);
+    unsigned long long any = 0;
+    for (int i = 0; i < 10; ++i) any |= det[i];
+    sign = any == 0 ? 0 : ((long long)det[9] < 0 ? -1 : 1);
+    return true;
+}
+
+// insphere sign from the exact double-word differences p - pe, evaluated in double-word
+// arithmetic (dw_mul relative error <= 4u^2, dw_add <= 3u^2 + 13u^3, u = 2^-53). Following
+// the error through minors (<= 7.1u^2 of their permanents), cofactors (<= 17.4u^2), lifts
+// (<= 10.2u^2), lifted products (<= 31.6u^2) and the final sums gives
+// |det_dw - det| <= 41u^2 P < 2^-100 P, P the permanent (every product taken in absolute
+// value); the test uses 2^-96 P. Components must lie in [2^-150, 2^150] so that no product
+// or error term leaves the normal range. False when |det_dw| is inside the bound (in
+// practice only exact zeros, which the fixed-point stage or the expansions resolve).
+__host__ __device__ inline bool insphere_sign_dw(const double* pa, const double* pb,
+                                                 const double* pc, const double* pd,
+                                                 const double* pe, int& sign) {
+    const double* P[4] = {pa, pb, pc, pd};
+    DW d[12];
+    double h[12];  // |hi| of each difference
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            DW x;
+            two_sum(P[v][a], -pe[a], x.hi, x.lo);
+            const double ax = x.hi < 0.0 ? -x.hi : x.hi;
+            if (ax != 0.0 && !(ax >= 0x1p-150 && ax <= 0x1p150)) return false;
+            const double al = x.lo < 0.0 ? -x.lo : x.lo;
+            if (al != 0.0 && !(al >= 0x1p-150)) return false;
+            d[3 * v + a] = x;
+            h[3 * v + a] = ax;
+        }
+    auto X = [&](int v) { return d[3 * v]; };
+    auto Y = [&](int v) { return d[3 * v + 1]; };
+    auto Z = [&](int v) { return d[3 * v + 2]; };
+    auto minor = [&](int u, int v) { return dw_add(dw_mul(X(u), Y(v)), dw_neg(dw_mul(X(v), Y(u)))); };
+    auto mperm = [&](int u, int v) { return h[3 * u] * h[3 * v + 1] + h[3 * v] * h[3 * u + 1]; };
+    constexpr int A = 0, B = 1, C = 2, Dv = 3;
+    const DW ab = minor(A, B), bc = minor(B, C), cd = minor(C, Dv), da = minor(Dv, A),
+             ac = minor(A, C), bd = minor(B, Dv);
+    const double pab = mperm(A, B), pbc = mperm(B, C), pcd = mperm(C, Dv), pda = mperm(Dv, A),
+                 pac = mperm(A, C), pbd = mperm(B, Dv);
+    auto zm = [&](int v, DW m) { return dw_mul(Z(v), m); };
+    // abc = aez*bc - bez*ac + cez*ab      bcd = bez*cd - cez*bd + dez*bc
+    // cda = cez*da + dez*ac + aez*cd      dab = dez*ab + aez*bd + bez*da
+    const DW abc = dw_add(dw_add(zm(A, bc), dw_neg(zm(B, ac))), zm(C, ab));
+    const DW bcd = dw_add(dw_add(zm(B, cd), dw_neg(zm(C, bd))), zm(Dv, bc));
+    const DW cda = dw_add(dw_add(zm(C, da), zm(Dv, ac)), zm(A, cd));
+    const DW dab = dw_add(dw_add(zm(Dv, ab), zm(A, bd)), zm(B, da));
+    const double zh[4] = {h[2], h[5], h[8], h[11]};
+    const double qabc = zh[A] * pbc + zh[B] * pac + zh[C] * pab;
+    const double qbcd = zh[B] * pcd + zh[C] * pbd + zh[Dv] * pbc;
+    const double qcda = zh[C] * pda + zh[Dv] * pac + zh[A] * pcd;
+    const double qdab = zh[Dv] * pab + zh[A] * pbd + zh[B] * pda;
+    DW lift[4];
+    double lh[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        lift[v] = dw_add(dw_add(dw_mul(X(v), X(v)), dw_mul(Y(v), Y(v))), dw_mul(Z(v), Z(v)));
+        lh[v] = h[3 * v] * h[3 * v] + h[3 * v + 1] * h[3 * v + 1] + h[3 * v + 2] * h[3 * v + 2];
+    }
+    // det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd)
+    const DW det = dw_add(dw_add(dw_mul(lift[Dv], abc), dw_neg(dw_mul(lift[C], dab))),
+                          dw_add(dw_mul(lift[B], cda), dw_neg(dw_mul(lift[A], bcd))));
+    const double perm = lh[Dv] * qabc + lh[C] * qdab + lh[B] * qcda + lh[A] * qbcd;
+    // the hi parts understate the double-word magnitudes by at most a factor 1 + 2u per
+    // difference; (1 + 2^-40) covers that and the rounding of perm
+    const double bound = 0x1p-96 * perm * (1.0 + 0x1p-40);
+    const double v = det.hi + det.lo;
+    if (v > bound) {
+        sign = 1;
+        return true;
+    }
+    if (-v > bound) {
+        sign = -1;
+        return true;
+    }
+    return false;
+}
+
 // Exact sign of insphere(pa, pb, pc, pd, pe); kOverflow if a capacity is too small.
 // CCAP bounds a 3x3 cofactor, LCAP a lifted product.
 template <int CCAP, int LCAP>
 __device__ inline int insphere_exact_sign(const double* pa, const double* pb,
                                           const double* pc, const double* pd,
                                           const double* pe) {
+    int fixed;  // differences within 126 bits: exact in registers
+    if (insphere_sign_fixed_wide(pa, pb, pc, pd, pe, fixed)) return fixed;
     double ex[4][2], ey[4][2], ez[4][2];
     int nx[4], ny[4], nz[4];
     const double* P[4] = {pa, pb, pc, pd};

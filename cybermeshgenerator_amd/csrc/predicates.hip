@@ -511,6 +511,47 @@ __device__ __forceinline__ uint32_t queued(const PredWorkspace& ws, uint32_t k) 
     return ws.queue[(size_t)lo * ws.seg + (k - ws.qpref[lo])];
 }
 
+// Register-only exact stage (insphere): one lane per queued tuple; the tuples whose
+// differences fit 62 bits get their sign in fixed point, the others whose double-word
+// determinant clears its error bound from that; what is left (exact zeros with widely
+// spread differences) is appended to the rest list (one atomic per wavefront) for the
+// expansion pass.
+template <class Src>
+__global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict__ out,
+                                                    PredWorkspace ws) {
+    const uint32_t total = ws.qpref[ws.nwaves];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t rounds = (total + stride - 1) / stride;  // the same in every lane
+    unsigned long long done = 0;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x + r * stride;
+        bool left = false;
+        uint32_t i = 0;
+        if (k < total) {
+            i = queued(ws, k);
+            double v[15];
+            src.load(i, v);
+            int s;
+            if (insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
+                insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s)) {
+                out[i] = (int8_t)s;
+                ++done;
+            } else {
+                left = true;
+            }
+        }
+        const unsigned long long m = __ballot(left);
+        if (m) {
+            uint32_t base = 0;
+            if (lane_id() == 0) base = atomicAdd(&ws.ctr->rest_n, (unsigned)__popcll(m));
+            base = __shfl(base, 0);
+            if (left) ws.rest[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = i;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) done += __shfl_xor(done, o);
+    if (lane_id() == 0 && done) atomicAdd(&ws.ctr->fixed, done);
+}
+
 // Exact pass. Pass 1 (WIDE = 0): one lane per queued tuple (through the prefix of the
 // per-wave counts), with small private arrays; lanes that outgrow them are appended to
 // the wide queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide
@@ -519,11 +560,12 @@ template <int NPTS, bool WIDE, class Src>
 __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__ out,
                                                    PredWorkspace ws) {
     if (!WIDE) {
-        const uint32_t total = ws.qpref[ws.nwaves];
+        // insphere: the fixed-point stage ran first and left its undecided lanes in `rest`
+        const uint32_t total = NPTS == 5 ? ws.ctr->rest_n : ws.qpref[ws.nwaves];
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+            const uint32_t i = NPTS == 5 ? ws.rest[k] : queued(ws, k);
             {
-                const uint32_t i = queued(ws, k);
                 double v[NPTS * 3];
                 src.load(i, v);
                 int s = NPTS == 4 ? orient3d_exact_sign<40, false>(v, v + 3, v + 6, v + 9)
@@ -646,6 +688,10 @@ template <int NPTS, class Src>
 void launch_exact(Src src, int8_t* out, const PredWorkspace& ws, hipStream_t stream) {
     queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
     CMG_HIP_LAUNCH();
+    if constexpr (NPTS == 5) {
+        fixed_kernel<Src><<<(unsigned)cu_count() * 8, 256, 0, stream>>>(src, out, ws);
+        CMG_HIP_LAUNCH();
+    }
     const unsigned g = (unsigned)cu_count() * 8;
     exact_kernel<NPTS, false, Src><<<g, 64, 0, stream>>>(src, out, ws);
     CMG_HIP_LAUNCH();
@@ -671,7 +717,7 @@ PredPlan pred_plan(size_t items, unsigned per_thread) {
 size_t pred_workspace_bytes(const PredPlan& p, size_t n) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     return 256 + 4 * al((size_t)p.nwaves * 4) + al(((size_t)p.nwaves + 1) * 4) +
-           al((size_t)p.nwaves * p.seg * 4) + al(n * 4);
+           al((size_t)p.nwaves * p.seg * 4) + 2 * al(n * 4);
 }
 
 PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
@@ -693,6 +739,8 @@ PredWorkspace pred_workspace(void* base, const PredPlan& p, size_t n) {
     ws.queue = reinterpret_cast<uint32_t*>(c);
     c += al((size_t)p.nwaves * p.seg * 4);
     ws.wide = reinterpret_cast<uint32_t*>(c);
+    c += al(n * 4);
+    ws.rest = reinterpret_cast<uint32_t*>(c);
     ws.seg = p.seg;
     ws.nwaves = p.nwaves;
     return ws;

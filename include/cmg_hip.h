@@ -138,9 +138,11 @@ typedef struct cmg_pred_stats {
     uint64_t exact;        /* left undecided by the filters, resolved exactly */
     uint64_t wide;         /* of those, re-run with wide expansion capacity */
     uint64_t failed;       /* exceeded even the wide capacity (sign left 0) */
-    uint64_t exact_inline; /* of `exact`, decided inside the streaming kernel (exact
-                              difference sums or double-word bound); the rest went
-                              through expansion arithmetic */
+    uint64_t exact_inline; /* of `exact`, decided by register-only exact stages: inside
+                              the streaming kernel (orient3d: exact difference sums or a
+                              double-word bound) or by fixed-point integer evaluation in
+                              the exact pass (insphere); the rest went through expansion
+                              arithmetic */
 } cmg_pred_stats;
 
 /* Workspace bytes a batched-sign call over n tuples needs (npts 4 = orient3d,

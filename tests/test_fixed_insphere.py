@@ -1,0 +1,147 @@
+"""The register-only insphere stages of the GPU exact passes (expansion.hpp: fixed point with
+62-bit differences, a double-word determinant with a rigorous error bound, fixed point with
+126-bit differences) built for the host and checked against the oracle's exact insphere:
+every tuple a stage decides must get the oracle's sign, and together they must decide the
+exactly cospherical and 1-ulp workload tuples."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from cybermeshgenerator_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "native", "fixed_insphere")
+
+
+def _run(tuples, tmp_path):
+    if not os.path.exists(BIN):
+        pytest.skip("tests/native/fixed_insphere not built (run the package build)")
+    t = np.ascontiguousarray(tuples, dtype=np.float64).reshape(-1, 15)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    t.tofile(src)
+    subprocess.run([BIN, str(src), str(dst)], check=True)
+    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 3, 2).astype(np.int64)
+    ok, sign = out[:, :, 0] > 0, out[:, :, 1]
+    # the GPU order: narrow fixed point (1), then the double-word bound (2)
+    stage = np.where(ok[:, 0], 1, np.where(ok[:, 1], 2, 0))
+    first = np.where(ok[:, 0], sign[:, 0], sign[:, 1])
+    _run.last = (ok, sign)
+    return stage, first
+
+
+def _oracle(tuples):
+    soa = np.ascontiguousarray(np.asarray(tuples, dtype=np.float64).reshape(-1, 15).T)
+    return oracle.insphere_batch(soa, oracle.EXACT).astype(np.int64)
+
+
+def _check(tuples, tmp_path):
+    stage, sign = _run(tuples, tmp_path)
+    ok, each = _run.last
+    want = _oracle(tuples)
+    for k in range(3):  # every stage on its own agrees wherever it decides
+        assert np.array_equal(each[ok[:, k], k], want[ok[:, k]]), k
+    return stage, want
+
+
+def test_workload_degenerate_tuples(tmp_path):
+    soa, kind = W.tuples_insphere(20000, seed=77)
+    tuples = soa.T.copy()
+    stage, want = _check(tuples, tmp_path)
+    assert (stage[kind == 1] == 1).all()        # exactly cospherical: fixed point, all 0
+    assert (want[kind == 1] == 0).all()
+    assert (stage[kind == 2] > 0).all()         # one coordinate 1 ulp off
+    assert (stage[kind == 2] == 1).mean() > 0.9
+
+
+def test_integer_grids_scaled_by_powers_of_two(tmp_path):
+    rng = np.random.default_rng(5)
+    vecs = W._norm_vectors(81)
+    rows = []
+    for bits in (3, 20, 40, 55, 61):
+        for scale in (-140, -60, 0, 37, 85):
+            # random integer tuples (nonzero determinants, both signs)
+            q = rng.integers(-(2 ** bits), 2 ** bits, size=(40, 15)).astype(np.float64)
+            rows.append(np.ldexp(q, scale))
+            # exactly cospherical: integer centre + distinct vectors of one norm times m
+            c = rng.integers(-(2 ** min(bits, 40)), 2 ** min(bits, 40), size=(40, 1, 3))
+            m = rng.integers(1, 2 ** max(1, min(bits, 48) - 5), size=(40, 1, 1))
+            pick = np.argsort(rng.random((40, len(vecs))), axis=1)[:, :5]
+            pts = (c + vecs[pick] * m).astype(np.float64)
+            rows.append(np.ldexp(pts.reshape(40, 15), scale))
+    tuples = np.concatenate(rows)
+    stage, want = _check(tuples, tmp_path)
+    dec = stage == 1
+    assert dec.mean() > 0.95
+    assert (want[dec] == 0).any() and (want[dec] > 0).any() and (want[dec] < 0).any()
+
+
+def test_spread_limit(tmp_path):
+    # differences spanning 62 bits are decided, 63 bits are left to the expansions
+    base = np.array([1.0, 0.25, 0.5, -0.5, 0.125, 0.75, 0.375, -0.25, 0.0, 0.0, 0.0, 0.0,
+                     0.0, 0.0, 0.0])
+    t62 = base.copy()
+    t62[8] = 2.0 ** -61
+    t63 = base.copy()
+    t63[8] = 2.0 ** -62
+    stage, sign = _run(np.stack([t62, t63]), tmp_path)
+    assert stage[0] == 1 and stage[1] != 1
+    assert np.array_equal(sign[stage > 0], _oracle(np.stack([t62, t63]))[stage > 0])
+
+
+def test_uniform_tuples_decided_ones_agree(tmp_path):
+    rng = np.random.default_rng(9)
+    tuples = rng.uniform(-1, 1, size=(5000, 15))
+    # a quarter on a coarse grid (exact differences)
+    tuples[:1250] = np.round(tuples[:1250] * 2 ** 20) / 2 ** 20
+    stage, _ = _check(tuples, tmp_path)
+    assert (stage[:1250] == 1).all()
+
+
+def test_extreme_magnitudes_left_to_expansions(tmp_path):
+    # outside [2^-150, 2^150] the stage declines (products of five differences would
+    # leave the normal range, where the expansion arithmetic defines the reference sign)
+    rng = np.random.default_rng(3)
+    q = rng.integers(-1000, 1000, size=(8, 15)).astype(np.float64)
+    stage, _ = _run(np.concatenate([np.ldexp(q, -1000), np.ldexp(q, 900)]), tmp_path)
+    assert not stage.any()
+
+
+def test_double_word_stage_on_widely_spread_tuples(tmp_path):
+    # tuples of mixed magnitudes (differences often span more than 62 bits) and the
+    # workload's 1-ulp tuples whose moved coordinate is small (so its ulp is tiny and the
+    # differences span more than 62 bits): what the fixed-point stage declines, the
+    # double-word bound decides, with the oracle's sign
+    rng = np.random.default_rng(12)
+    uni = rng.uniform(-1, 1, size=(4000, 15)) * 2.0 ** rng.integers(-40, 1, size=(4000, 15))
+    soa, kind = W.tuples_insphere(200000, seed=31)
+    ulp = soa.T[kind == 2]
+    stage, want = _check(np.concatenate([uni, ulp]), tmp_path)
+    n = len(uni)
+    assert (stage[:n] > 0).all() and (stage[:n] == 2).mean() > 0.2
+    wide = stage[n:] != 1
+    assert wide.sum() > 0                      # a few in 10^4
+    assert (stage[n:][wide] == 2).all()
+
+
+def test_wide_fixed_point_stage(tmp_path):
+    # differences spanning 63..126 bits: the narrow stage declines, the wide one decides
+    # (exactly cospherical integer tuples scaled apart by 2^k and then shifted by a tiny
+    # common offset on one vertex pair would not stay cospherical, so this uses random
+    # integer tuples of 64..120-bit spread, including ones with repeated points: det 0)
+    rng = np.random.default_rng(21)
+    rows = []
+    for spread in (64, 80, 100, 120):
+        big = rng.integers(-(2 ** 40), 2 ** 40, size=(60, 15)).astype(np.float64)
+        big = np.ldexp(big, spread - 41)          # top bits near 2^spread
+        small = rng.integers(1, 2 ** 20, size=(60, 15)).astype(np.float64)
+        t = big + np.where(rng.random((60, 15)) < 0.3, small, 0.0)
+        t[:10, 3:6] = t[:10, 0:3]                 # a repeated point: det exactly 0
+        rows.append(t)
+    tuples = np.concatenate(rows)
+    stage, want = _check(tuples, tmp_path)
+    ok, _ = _run.last
+    assert ok[:, 2].mean() > 0.95
+    assert (want[ok[:, 2]] == 0).any()
