@@ -22,7 +22,7 @@ struct PredCounters {
     unsigned int failed;        // lanes whose expansions outgrew even the wide capacity
     unsigned long long inline_; // lanes decided exactly inside the streaming kernel
     unsigned long long fixed;   // queued lanes decided by the fixed-point exact stage
-    unsigned int rest_n;        // queued lanes the fixed-point stage left (the rest list)
+    unsigned int rest_n;        // queued lanes the register-only stages left (rest list)
     unsigned int pad;
 };
 static_assert(sizeof(PredCounters) <= 256, "counter block");
@@ -42,7 +42,7 @@ struct PredWorkspace {
     uint32_t* queue = nullptr;    // nwaves segments of seg entries
     uint32_t* qpref = nullptr;    // exclusive prefix of wave_q (nwaves + 1 entries)
     uint32_t* wide = nullptr;     // wide-capacity queue (n entries)
-    uint32_t* rest = nullptr;     // lanes left after the fixed-point stage (n entries)
+    uint32_t* rest = nullptr;     // lanes left after the register-only stages (n entries)
     unsigned seg = 0, nwaves = 0;
 };
 

@@ -511,11 +511,9 @@ __device__ __forceinline__ uint32_t queued(const PredWorkspace& ws, uint32_t k) 
     return ws.queue[(size_t)lo * ws.seg + (k - ws.qpref[lo])];
 }
 
-// Register-only exact stage (insphere): one lane per queued tuple; the tuples whose
-// differences fit 62 bits get their sign in fixed point, the others whose double-word
-// determinant clears its error bound from that; what is left (exact zeros with widely
-// spread differences) is appended to the rest list (one atomic per wavefront) for the
-// expansion pass.
+// Register-only exact stages (insphere), one lane per queued tuple: fixed point with
+// 62-bit differences, else the double-word determinant when it clears its error bound;
+// what is left goes to the rest list (one atomic per wavefront) for the exact pass.
 template <class Src>
 __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict__ out,
                                                     PredWorkspace ws) {
@@ -560,7 +558,7 @@ template <int NPTS, bool WIDE, class Src>
 __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__ out,
                                                    PredWorkspace ws) {
     if (!WIDE) {
-        // insphere: the fixed-point stage ran first and left its undecided lanes in `rest`
+        // insphere: the register-only stages ran first and left their lanes in `rest`
         const uint32_t total = NPTS == 5 ? ws.ctr->rest_n : ws.qpref[ws.nwaves];
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {

@@ -1,6 +1,6 @@
 """The register-only insphere stages of the GPU exact passes (expansion.hpp: fixed point with
-62-bit differences, a double-word determinant with a rigorous error bound, fixed point with
-126-bit differences) built for the host and checked against the oracle's exact insphere:
+62- and 126-bit differences, a double-word determinant with a rigorous error bound) built
+for the host and checked against the oracle's exact insphere:
 every tuple a stage decides must get the oracle's sign, and together they must decide the
 exactly cospherical and 1-ulp workload tuples."""
 import os
