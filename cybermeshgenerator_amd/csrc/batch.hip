@@ -164,9 +164,10 @@ __device__ __forceinline__ NodeRec load_node(const NodeRec* base, int i) {
 // inside (the float axes are orthonormal to a few ulps; the distance error margin covers
 // the difference).
 __device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, float pz) {
-    const float tn = fmaxf(fabsf(proj(N.nx, N.ny, N.nz, N.dn, px, py, pz)) - N.en, 0.0f);
-    const float tu = fmaxf(fabsf(proj(N.ux, N.uy, N.uz, N.du, px, py, pz)) - N.eu, 0.0f);
-    const float tw = fmaxf(fabsf(proj(N.wx, N.wy, N.wz, N.dw, px, py, pz)) - N.ew, 0.0f);
+    // search units: |projection| < 1, so clamping to [0, 1] is max(., 0)
+    const float tn = clamp01(fabsf(proj(N.nx, N.ny, N.nz, N.dn, px, py, pz)) - N.en);
+    const float tu = clamp01(fabsf(proj(N.ux, N.uy, N.uz, N.du, px, py, pz)) - N.eu);
+    const float tw = clamp01(fabsf(proj(N.wx, N.wy, N.wz, N.dw, px, py, pz)) - N.ew);
     return fma3(tn, tn, tu, tu, tw, tw);
 }
 
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
             for (int a = 0; a < 3; ++a) P[v][a] = xyz[3 * (size_t)C.tris[3 * (size_t)t + v] + a];
         TriRec r;
         TriRec64 r64;
-        prep_triangle(P, M.g, r, r64);
+        prep_triangle(P, M.g, M.sc, r, r64);
         C.rec[s] = r;
         C.rec64[s] = r64;
         const int4 x = xy_range(P, M.g);
@@ -717,13 +718,14 @@ __global__ __launch_bounds__(256) void node_leaf_kernel(Ctx C, int total) {
             for (int a = 0; a < 3; ++a) N[a] += sg * nt[a];
         }
         float4 ctr;
-        NodeRec B = make_box(N, 3.0 * grid_extent(M.g), ctr, [&](auto&& f) {
+        NodeRec B = make_box(N, 3.0 * grid_extent(M.g) * M.sc, ctr, [&](auto&& f) {
             for (int k = 0; k < cnt; ++k) {
                 const TriRec64& T = C.rec64[s0 + k];
                 for (int v = 0; v < 3; ++v) {
                     const double* P = v == 0 ? T.a : (v == 1 ? T.b : T.c);
-                    const double q[3] = {P[0] - M.g.corner[0], P[1] - M.g.corner[1],
-                                         P[2] - M.g.corner[2]};
+                    const double q[3] = {(P[0] - M.g.corner[0]) * M.sc,
+                                         (P[1] - M.g.corner[1]) * M.sc,
+                                         (P[2] - M.g.corner[2]) * M.sc};
                     f(q);
                 }
             }
@@ -758,7 +760,7 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
             for (int a = 0; a < 3; ++a) N[a] += sg * wgt * nk[a];
         }
         float4 ctr;
-        NodeRec B = make_box(N, 3.0 * grid_extent(M.g), ctr, [&](auto&& f) {
+        NodeRec B = make_box(N, 3.0 * grid_extent(M.g) * M.sc, ctr, [&](auto&& f) {
             for (int k = 0; k < cnt; ++k) {
                 const NodeRec K = ch[k];
                 const float4 KC = C.nctr[cfirst + k];
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         int k[VPL];
         bool valid[VPL];
         float pz[VPL];
-        const float h = (float)g.spacing;
+        const float h = (float)g.spacing * M.scf;  // the cell size in search units (exact)
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             k[v] = iz * L0.sz + (lane >> 4) + 4 * v;
@@ -1041,7 +1043,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     const double d = sqrt(best64[v]);
                     out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
                 } else {
-                    const float d32 = sqrtf(best[v]);
+                    const float d32 = sqrtf(best[v]) * M.iscf;  // back to world units (exact)
                     out[idx] = occ[idx] ? -d32 : d32;
                 }
             }
@@ -1382,7 +1384,10 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.L0 = make_level(g, kLeaf);
         M.L0b = make_level(g, kLeaf, 2 * kLeaf);
         M.L0c = make_level(g, kLeaf, 4 * kLeaf);
-        M.eps = distance_eps(g);
+        M.sc = search_scale(g);
+        M.scf = (float)M.sc;
+        M.iscf = (float)(1.0 / M.sc);
+        M.eps = distance_eps(g) * M.scf;
         M.dzero = zero_distance(g);
         // tree levels: level l has ceil(ntris / 8^l) nodes, up to the single root
         M.nnodes[0] = M.ntris;

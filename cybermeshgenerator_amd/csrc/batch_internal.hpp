@@ -48,8 +48,10 @@ struct MeshDesc {
     sdfdev::Level L0, L0b, L0c;    // voxel leaves: 4x4x4, 4x4x8 and 4x4x16
     int node_base[kMaxLevels + 1]; // first node of each level (1..nlev) in the node array
     int nnodes[kMaxLevels + 1];    // nodes per level (level 0: triangles)
-    float eps;            // FP32 distance error margin for this grid
-    double dzero;         // distances at or below this carry no gradient
+    float eps;            // FP32 distance error margin for this grid, in search units
+    double dzero;         // distances at or below this carry no gradient (world units)
+    double sc;            // search units: (x - corner) * sc, a power of two
+    float scf, iscf;      // sc and 1 / sc as floats (exact)
 };
 
 enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };

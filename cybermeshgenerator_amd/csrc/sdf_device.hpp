@@ -278,8 +278,10 @@ __device__ __forceinline__ TriRec64 load_uniform(const TriRec64* base, int t) {
 
 
 // Fills the FP32 and FP64 records of one triangle (vertices P[3][3], world coordinates).
-__device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g, TriRec& r,
-                                     TriRec64& r64) {
+// The FP32 record is in the mesh's search units: coordinates relative to the grid corner
+// times sc, a power of two (exact).
+__device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g, double sc,
+                                     TriRec& r, TriRec64& r64) {
     for (int a = 0; a < 3; ++a) {
         r64.a[a] = P[0][a];
         r64.b[a] = P[1][a];
@@ -287,10 +289,10 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     }
     double A[3], AB[3], AC[3], BC[3];
     for (int a = 0; a < 3; ++a) {
-        A[a] = P[0][a] - g.corner[a];
-        AB[a] = P[1][a] - P[0][a];
-        AC[a] = P[2][a] - P[0][a];
-        BC[a] = P[2][a] - P[1][a];
+        A[a] = (P[0][a] - g.corner[a]) * sc;
+        AB[a] = (P[1][a] - P[0][a]) * sc;
+        AC[a] = (P[2][a] - P[0][a]) * sc;
+        BC[a] = (P[2][a] - P[1][a]) * sc;
     }
     const double a11 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
     const double a22 = AC[0] * AC[0] + AC[1] * AC[1] + AC[2] * AC[2];
@@ -359,7 +361,7 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
     for (int v = 0; v < 3; ++v) {
         double q[3];
         for (int a = 0; a < 3; ++a) {
-            q[a] = P[v][a] - g.corner[a];
+            q[a] = (P[v][a] - g.corner[a]) * sc;
             ext = fmax(ext, fabs(q[a]));
         }
         for (int k = 0; k < 3; ++k) {
@@ -368,7 +370,7 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
             hi[k] = fmax(hi[k], d);
         }
     }
-    const double pmax = 3.0 * grid_extent(g);
+    const double pmax = 3.0 * grid_extent(g) * sc;
     double d[3], e[3];
     for (int k = 0; k < 3; ++k) {
         d[k] = 0.5 * (lo[k] + hi[k]);
@@ -393,11 +395,21 @@ __device__ inline void prep_triangle(const double (&P)[3][3], const GridSpec& g,
 }
 
 // Squared distance from p to the triangle's oriented box: a lower bound on its distance.
+// In search units every |projection| is below 1, so clamping to [0, 1] is max(., 0) and
+// folds into the subtraction (one instruction per axis).
 __device__ __forceinline__ float tri_lb2(const TriRec& T, float px, float py, float pz) {
-    const float tn = fmaxf(fabsf(proj(T.nx, T.ny, T.nz, T.dn, px, py, pz)) - T.en, 0.0f);
-    const float tu = fmaxf(fabsf(proj(T.ux, T.uy, T.uz, T.du, px, py, pz)) - T.eu, 0.0f);
-    const float tw = fmaxf(fabsf(proj(T.wx, T.wy, T.wz, T.dw, px, py, pz)) - T.ew, 0.0f);
+    const float tn = clamp01(fabsf(proj(T.nx, T.ny, T.nz, T.dn, px, py, pz)) - T.en);
+    const float tu = clamp01(fabsf(proj(T.ux, T.uy, T.uz, T.du, px, py, pz)) - T.eu);
+    const float tw = clamp01(fabsf(proj(T.wx, T.wy, T.wz, T.dw, px, py, pz)) - T.ew);
     return fma3(tn, tn, tu, tu, tw, tw);
+}
+
+// Search units of a grid: 2^-e with 2^(e-2) >= grid extent, so every grid point, box
+// centre and projection has magnitude below 1/2 * sqrt(3) * 2 < 1.
+inline double search_scale(const GridSpec& g) {
+    int e = 0;
+    std::frexp(grid_extent(g), &e);  // grid extent < 2^e
+    return std::ldexp(1.0, -(e + 2));
 }
 // Distances at or below this are zero up to the FP64 rounding of the coordinates: such
 // a cell lies on the surface, where the unsigned distance has a kink, and carries no
