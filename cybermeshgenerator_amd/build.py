@@ -78,11 +78,11 @@ def _build_native_tests() -> None:
              os.path.join(tdir, "smoke_c_abi"), *link],
             ["g++", "-std=c++20", "-O1", *inc, os.path.join(tdir, "test_cpp_api.cpp"), "-o",
              os.path.join(tdir, "test_cpp_api"), *link]]
-    # host build of the fixed-point exact insphere stage (device-and-host source)
+    # host build of the register-only exact stages (device-and-host source)
     jobs.append([HIPCC, "-x", "hip", "-O2", "-std=c++20", "-ffp-contract=off",
                  "--offload-arch=gfx950", "-I" + CSRC, *inc,
-                 os.path.join(tdir, "fixed_insphere.cpp"), "-o",
-                 os.path.join(tdir, "fixed_insphere")])
+                 os.path.join(tdir, "register_stages.cpp"), "-o",
+                 os.path.join(tdir, "register_stages")])
     for cmd in jobs:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -228,7 +228,7 @@ __device__ inline bool orient3d_exact(const double* pa, const double* pb, const 
 // the sum of the other magnitudes (with a margin for rounding that sum), and the sum is 0
 // once every entry is 0. Returns false if neither happened within MAXPASS passes.
 template <int N, int MAXPASS>
-__device__ inline bool sum_sign(double (&v)[N], int& sign) {
+__host__ __device__ inline bool sum_sign(double (&v)[N], int& sign) {
 #pragma unroll 1
     for (int pass = 0; pass < MAXPASS; ++pass) {
 #pragma unroll
@@ -257,7 +257,7 @@ __device__ inline bool sum_sign(double (&v)[N], int& sign) {
 // orient3d sign when every difference a - d etc. is exact in FP64: the determinant is the
 // exact sum of six triple products of those differences, each split into four doubles
 // with FMA (x*y = p + e exactly, then p*z and e*z exactly). False if undecided.
-__device__ inline bool orient3d_sign_exact_diffs(const double* pa, const double* pb,
+__host__ __device__ inline bool orient3d_sign_exact_diffs(const double* pa, const double* pb,
                                                  const double* pc, const double* pd,
                                                  int& sign) {
     double d[9];
@@ -328,7 +328,7 @@ __host__ __device__ __forceinline__ DW dw_neg(DW x) { return DW{-x.hi, -x.lo}; }
 // by the permanent P, so |det_dw - det| <= 2^-96 P with a wide margin; lanes inside that
 // band (in practice only exact zeros) are left to the expansion arithmetic. Inputs must
 // keep every product normal (same range check as the exact-difference path).
-__device__ inline bool orient3d_sign_dw(const double* pa, const double* pb, const double* pc,
+__host__ __device__ inline bool orient3d_sign_dw(const double* pa, const double* pb, const double* pc,
                                         const double* pd, int& sign) {
     DW d[9];
     double perm_in[9];
@@ -365,18 +365,6 @@ __device__ inline bool orient3d_sign_dw(const double* pa, const double* pb, cons
     return false;
 }
 
-// Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small. FAST = false
-// skips the register-only stages (for lanes that already failed them).
-template <int CAP, bool FAST = true>
-__device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
-                                          const double* pc, const double* pd) {
-    int fast;
-    if (FAST && orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
-    if (FAST && orient3d_sign_dw(pa, pb, pc, pd, fast)) return fast;
-    double top;
-    if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
-    return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
-}
 
 // ---- fixed-point exact insphere ---------------------------------------------------
 //
@@ -568,6 +556,51 @@ __host__ __device__ inline bool insphere_sign_fixed(const double* pa, const doub
     return true;
 }
 
+// Exact orient3d sign in fixed point: each difference p - pd is exactly x + y (Two-Sum);
+// when all nine are integer multiples of one power of two with magnitudes within 62
+// bits, det = adz*(bdx*cdy - cdx*bdy) + bdz*(cdx*ady - adx*cdy) + cdz*(adx*bdy - bdx*ady)
+// is evaluated exactly (minors 128-bit, products and sum 192-bit). Decides the exact
+// zeros whose differences are inexact in FP64, which the exact-difference and
+// double-word stages cannot. Same range limits as the insphere stage.
+__host__ __device__ inline bool orient3d_sign_fixed(const double* pa, const double* pb,
+                                                    const double* pc, const double* pd,
+                                                    int& sign) {
+    const double* P[3] = {pa, pb, pc};
+    double dx[9], dy[9];
+    int lo = 1 << 20, hi = -(1 << 20);
+    for (int v = 0; v < 3; ++v)
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(P[v][a], -pd[a], x, y);
+            dx[3 * v + a] = x;
+            dy[3 * v + a] = y;
+            const double xy[2] = {x, y};
+            for (int q = 0; q < 2; ++q) {
+                if (xy[q] == 0.0) continue;
+                int l, h;
+                if (!bit_range(xy[q], l, h)) return false;
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+        }
+    if (hi < lo) {
+        sign = 0;
+        return true;
+    }
+    if (hi - lo > 62 || hi > 300 || lo < -300) return false;
+    long long D[9];
+    for (int i = 0; i < 9; ++i) D[i] = scaled_int(dx[i], lo) + scaled_int(dy[i], lo);
+    const long long adx = D[0], ady = D[1], adz = D[2], bdx = D[3], bdy = D[4], bdz = D[5],
+                    cdx = D[6], cdy = D[7], cdz = D[8];
+    I192 det = mul_z_minor(adz, smul(bdx, cdy) - smul(cdx, bdy));
+    const I192 t1 = mul_z_minor(bdz, smul(cdx, ady) - smul(adx, cdy));
+    const I192 t2 = mul_z_minor(cdz, smul(adx, bdy) - smul(bdx, ady));
+    limbs_add(det.w, t1.w);
+    limbs_add(det.w, t2.w);
+    sign = (det.w[0] | det.w[1] | det.w[2]) == 0 ? 0 : ((long long)det.w[2] < 0 ? -1 : 1);
+    return true;
+}
+
 // ---- wide fixed point: the same evaluation for differences spanning up to 126 bits ----
 
 // r = a * b (unsigned, little-endian 64-bit limbs)
@@ -722,6 +755,66 @@ __host__ __device__ inline bool insphere_sign_fixed_wide(const double* pa, const
     return true;
 }
 
+// orient3d in fixed point with 126-bit differences (minors 256-bit, products and sum
+// 384-bit), for the exact zeros whose differences span more than 62 bits.
+__host__ __device__ inline bool orient3d_sign_fixed_wide(const double* pa, const double* pb,
+                                                         const double* pc, const double* pd,
+                                                         int& sign) {
+    const double* P[3] = {pa, pb, pc};
+    double dx[9], dy[9];
+    int lo = 1 << 20, hi = -(1 << 20);
+    for (int v = 0; v < 3; ++v)
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(P[v][a], -pd[a], x, y);
+            dx[3 * v + a] = x;
+            dy[3 * v + a] = y;
+            const double xy[2] = {x, y};
+            for (int q = 0; q < 2; ++q) {
+                if (xy[q] == 0.0) continue;
+                int l, h;
+                if (!bit_range(xy[q], l, h)) return false;
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+        }
+    if (hi < lo) {
+        sign = 0;
+        return true;
+    }
+    if (hi - lo > 126 || hi > 300 || lo < -300) return false;
+    unsigned long long D[9][2];
+    for (int i = 0; i < 9; ++i) {
+        const __int128 q = scaled_int128(dx[i], lo) + scaled_int128(dy[i], lo);
+        D[i][0] = (unsigned long long)q;
+        D[i][1] = (unsigned long long)((u128)q >> 64);
+    }
+    typedef unsigned long long L4[4];
+    typedef unsigned long long L6[6];
+    auto minor = [&](int u, int v, L4& m) {  // D[u]x * D[v]y - D[v]x * D[u]y (rows a, b, c)
+        L4 t;
+        limbs_smul(D[3 * u], D[3 * v + 1], m);
+        limbs_smul(D[3 * v], D[3 * u + 1], t);
+        limbs_neg(t);
+        limbs_add(m, t);
+    };
+    // adz*(bdx*cdy - cdx*bdy) + bdz*(cdx*ady - adx*cdy) + cdz*(adx*bdy - bdx*ady)
+    L4 mbc, mca, mab;
+    minor(1, 2, mbc);
+    minor(2, 0, mca);
+    minor(0, 1, mab);
+    L6 det, t;
+    limbs_smul(D[2], mbc, det);
+    limbs_smul(D[5], mca, t);
+    limbs_add(det, t);
+    limbs_smul(D[8], mab, t);
+    limbs_add(det, t);
+    unsigned long long any = 0;
+    for (int i = 0; i < 6; ++i) any |= det[i];
+    sign = any == 0 ? 0 : ((long long)det[5] < 0 ? -1 : 1);
+    return true;
+}
+
 // insphere sign from the exact double-word differences p - pe, evaluated in double-word
 // arithmetic (dw_mul relative error <= 4u^2, dw_add <= 3u^2 + 13u^3, u = 2^-53). Following
 // the error through minors (<= 7.1u^2 of their permanents), cofactors (<= 17.4u^2), lifts
@@ -795,6 +888,21 @@ __host__ __device__ inline bool insphere_sign_dw(const double* pa, const double*
         return true;
     }
     return false;
+}
+
+// Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small. FAST = false
+// skips the register-only stages (for lanes that already failed them).
+template <int CAP, bool FAST = true>
+__device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
+                                          const double* pc, const double* pd) {
+    int fast;
+    if (FAST && orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
+    if (FAST && orient3d_sign_dw(pa, pb, pc, pd, fast)) return fast;
+    if (orient3d_sign_fixed(pa, pb, pc, pd, fast)) return fast;
+    if (orient3d_sign_fixed_wide(pa, pb, pc, pd, fast)) return fast;
+    double top;
+    if (!orient3d_exact<CAP>(pa, pb, pc, pd, top)) return kOverflow;
+    return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
 }
 
 // Exact sign of insphere(pa, pb, pc, pd, pe); kOverflow if a capacity is too small.

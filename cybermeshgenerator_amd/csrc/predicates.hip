@@ -223,8 +223,8 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
 // Register-only exact stage run inside the streaming kernels for the lanes the filters
 // leave undecided (orient3d only): an exact sum of the triple products when the
 // coordinate differences are exact, else double-word arithmetic with a rigorous bound.
-// Lanes it cannot decide (exact zeros with inexact differences, extreme exponents) go to
-// the expansion pass.
+// Lanes it cannot decide (in practice exact zeros with inexact differences) are queued
+// for the exact pass, whose fixed-point stages resolve them.
 template <int NPTS>
 __device__ __forceinline__ bool decide_inline_exact(const double* v, int& s) {
     if constexpr (NPTS == 4) {
@@ -353,6 +353,8 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                                                          PredWorkspace ws, unsigned iters) {
     constexpr int NPL = NPTS * 3;
     constexpr bool STAGE = NPTS == 4;
+    // the global counters are only touched by the passes after this one
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ws.ctr = PredCounters{};
     __shared__ double s_ring[STAGE ? 4 : 1][STAGE ? NPL * kRing : 1];
     __shared__ uint32_t s_idx[STAGE ? 4 : 1][STAGE ? kRing : 1];
     unsigned n32 = 0, n64 = 0, nx = 0;
@@ -511,10 +513,11 @@ __device__ __forceinline__ uint32_t queued(const PredWorkspace& ws, uint32_t k) 
     return ws.queue[(size_t)lo * ws.seg + (k - ws.qpref[lo])];
 }
 
-// Register-only exact stages (insphere), one lane per queued tuple: fixed point with
-// 62-bit differences, else the double-word determinant when it clears its error bound;
-// what is left goes to the rest list (one atomic per wavefront) for the exact pass.
-template <class Src>
+// Register-only exact stages over the insphere queue, one lane per queued tuple, no
+// private arrays: fixed point with 62-bit differences, else the double-word determinant
+// when it clears its error bound. What is left goes to the rest list (one atomic per
+// wavefront) for the exact pass.
+template <int NPTS, class Src>
 __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict__ out,
                                                     PredWorkspace ws) {
     const uint32_t total = ws.qpref[ws.nwaves];
@@ -527,11 +530,12 @@ __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict_
         uint32_t i = 0;
         if (k < total) {
             i = queued(ws, k);
-            double v[15];
+            double v[NPTS * 3];
             src.load(i, v);
             int s;
-            if (insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
-                insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s)) {
+            const bool ok = insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
+                            insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s);
+            if (ok) {
                 out[i] = (int8_t)s;
                 ++done;
             } else {
@@ -550,15 +554,15 @@ __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict_
     if (lane_id() == 0 && done) atomicAdd(&ws.ctr->fixed, done);
 }
 
-// Exact pass. Pass 1 (WIDE = 0): one lane per queued tuple (through the prefix of the
-// per-wave counts), with small private arrays; lanes that outgrow them are appended to
-// the wide queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide
+// Exact pass. Pass 1 (WIDE = 0): one lane per queued tuple (orient3d: the queue, through
+// the prefix of the per-wave counts; insphere: the rest list of the register-only pass):
+// fixed-point stages first, then expansions in small private arrays; lanes that outgrow
+// them are appended to the wide queue (rare, so one atomic each). Pass 2 (WIDE = 1): the wide queue with wide
 // arrays; a lane that still overflows is counted in ctr->failed and left at 0.
 template <int NPTS, bool WIDE, class Src>
 __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__ out,
                                                    PredWorkspace ws) {
     if (!WIDE) {
-        // insphere: the register-only stages ran first and left their lanes in `rest`
         const uint32_t total = NPTS == 5 ? ws.ctr->rest_n : ws.qpref[ws.nwaves];
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
@@ -687,10 +691,11 @@ void launch_exact(Src src, int8_t* out, const PredWorkspace& ws, hipStream_t str
     queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
     CMG_HIP_LAUNCH();
     if constexpr (NPTS == 5) {
-        fixed_kernel<Src><<<(unsigned)cu_count() * 8, 256, 0, stream>>>(src, out, ws);
+        fixed_kernel<NPTS, Src><<<(unsigned)cu_count() * 8, 256, 0, stream>>>(src, out, ws);
         CMG_HIP_LAUNCH();
     }
-    const unsigned g = (unsigned)cu_count() * 8;
+    // few lanes reach this pass: a small grid keeps the scratch set-up of its waves cheap
+    const unsigned g = (unsigned)cu_count();
     exact_kernel<NPTS, false, Src><<<g, 64, 0, stream>>>(src, out, ws);
     CMG_HIP_LAUNCH();
     unsigned gw = cu_count() / 4 > 0 ? cu_count() / 4 : 1;
@@ -752,9 +757,11 @@ void launch_pred_stats(const PredWorkspace& ws, hipStream_t stream) {
 void launch_signs_soa(int npts, int mode, const double* soa, size_t n, size_t ld,
                       int8_t* out, const PredPlan& p, const PredWorkspace& ws,
                       hipStream_t stream) {
-    zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
-    CMG_HIP_LAUNCH();
-    if (n == 0) return;
+    if (n == 0) {  // (otherwise the filter kernel clears the counters)
+        zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
+        CMG_HIP_LAUNCH();
+        return;
+    }
     const bool vec = (reinterpret_cast<uintptr_t>(soa) % 16 == 0) && (ld % 2 == 0);
 #define CMG_FILTER(NP, MD)                                                                \
     do {                                                                                  \

@@ -1,0 +1,54 @@
+// Host run of the register-only exact stages (the same source the GPU passes compile),
+// for checking them against the oracle without a GPU.
+// Usage: register_stages insphere|orient3d <in: n tuples of float64> <out: n x 8 int8>
+// insphere tuples are pa pb pc pd pe (15 doubles); per tuple, for each stage in turn
+// (fixed point with 62-bit differences, double-word bound, fixed point with 126-bit
+// differences): decided (0/1), sign. orient3d tuples are pa pb pc pd (12 doubles); the
+// stages are the exact-difference sum, the double-word bound and fixed point with 62- and
+// 126-bit differences. Unused slots read 0.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "expansion.hpp"
+
+int main(int argc, char** argv) {
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: %s insphere|orient3d in.bin out.bin\n", argv[0]);
+        return 2;
+    }
+    const bool ins = std::strcmp(argv[1], "insphere") == 0;
+    const int per = ins ? 15 : 12;
+    std::FILE* f = std::fopen(argv[2], "rb");
+    if (!f) return 2;
+    std::vector<double> v;
+    double buf[15];
+    while (std::fread(buf, sizeof(double), per, f) == (size_t)per) v.insert(v.end(), buf, buf + per);
+    std::fclose(f);
+    const size_t n = v.size() / per;
+    std::vector<signed char> out(8 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const double* t = v.data() + per * i;
+        int s[4] = {0, 0, 0, 0};
+        bool ok[4] = {false, false, false, false};
+        if (ins) {
+            ok[0] = cmg::hip::insphere_sign_fixed(t, t + 3, t + 6, t + 9, t + 12, s[0]);
+            ok[1] = cmg::hip::insphere_sign_dw(t, t + 3, t + 6, t + 9, t + 12, s[1]);
+            ok[2] = cmg::hip::insphere_sign_fixed_wide(t, t + 3, t + 6, t + 9, t + 12, s[2]);
+        } else {
+            ok[0] = cmg::hip::orient3d_sign_exact_diffs(t, t + 3, t + 6, t + 9, s[0]);
+            ok[1] = cmg::hip::orient3d_sign_dw(t, t + 3, t + 6, t + 9, s[1]);
+            ok[2] = cmg::hip::orient3d_sign_fixed(t, t + 3, t + 6, t + 9, s[2]);
+            ok[3] = cmg::hip::orient3d_sign_fixed_wide(t, t + 3, t + 6, t + 9, s[3]);
+        }
+        for (int k = 0; k < 4; ++k) {
+            out[8 * i + 2 * k] = ok[k] ? 1 : 0;
+            out[8 * i + 2 * k + 1] = (signed char)(ok[k] ? s[k] : 0);
+        }
+    }
+    std::FILE* g = std::fopen(argv[3], "wb");
+    if (!g) return 2;
+    std::fwrite(out.data(), 1, out.size(), g);
+    std::fclose(g);
+    return 0;
+}

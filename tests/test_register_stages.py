@@ -1,4 +1,4 @@
-"""The register-only insphere stages of the GPU exact passes (expansion.hpp: fixed point with
+"""The register-only exact stages of the GPU predicate passes. insphere (expansion.hpp: fixed point with
 62- and 126-bit differences, a double-word determinant with a rigorous error bound) built
 for the host and checked against the oracle's exact insphere:
 every tuple a stage decides must get the oracle's sign, and together they must decide the
@@ -13,17 +13,17 @@ import oracle
 from cybermeshgenerator_amd import workloads as W
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BIN = os.path.join(ROOT, "tests", "native", "fixed_insphere")
+BIN = os.path.join(ROOT, "tests", "native", "register_stages")
 
 
 def _run(tuples, tmp_path):
     if not os.path.exists(BIN):
-        pytest.skip("tests/native/fixed_insphere not built (run the package build)")
+        pytest.skip("tests/native/register_stages not built (run the package build)")
     t = np.ascontiguousarray(tuples, dtype=np.float64).reshape(-1, 15)
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     t.tofile(src)
-    subprocess.run([BIN, str(src), str(dst)], check=True)
-    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 3, 2).astype(np.int64)
+    subprocess.run([BIN, "insphere", str(src), str(dst)], check=True)
+    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 4, 2)[:, :3].astype(np.int64)
     ok, sign = out[:, :, 0] > 0, out[:, :, 1]
     # the GPU order: narrow fixed point (1), then the double-word bound (2)
     stage = np.where(ok[:, 0], 1, np.where(ok[:, 1], 2, 0))
@@ -145,3 +145,58 @@ def test_wide_fixed_point_stage(tmp_path):
     ok, _ = _run.last
     assert ok[:, 2].mean() > 0.95
     assert (want[ok[:, 2]] == 0).any()
+
+
+def _run_o3(tuples, tmp_path):
+    if not os.path.exists(BIN):
+        pytest.skip("tests/native/register_stages not built (run the package build)")
+    t = np.ascontiguousarray(tuples, dtype=np.float64).reshape(-1, 12)
+    src, dst = tmp_path / "o3.bin", tmp_path / "o3out.bin"
+    t.tofile(src)
+    subprocess.run([BIN, "orient3d", str(src), str(dst)], check=True)
+    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 4, 2).astype(np.int64)
+    return out[:, :, 0] > 0, out[:, :, 1]
+
+
+def test_orient3d_stages_against_oracle(tmp_path):
+    # orient3d: the exact-difference sum and the double-word bound (inside the streaming
+    # kernel) and fixed point (exact pass); each agrees with the oracle wherever it
+    # decides, and together they decide every workload tuple, including the exact zeros
+    # with inexact differences (a point moved by 1 ulp onto another)
+    soa, kind = W.tuples_orient3d(1 << 18, seed=2)
+    t = soa.T.copy()
+    ok, sign = _run_o3(t, tmp_path)
+    want = oracle.orient3d_batch(soa, oracle.EXACT).astype(np.int64)
+    for k in range(4):
+        assert np.array_equal(sign[ok[:, k], k], want[ok[:, k]]), k
+    left = ~ok[:, 0] & ~ok[:, 1]
+    assert left.sum() > 0 and (want[left] == 0).all()
+    assert ok[left, 2].mean() > 0.9              # the rest span more than 62 bits
+    assert (ok[left, 2] | ok[left, 3]).all()     # and take the 126-bit stage
+    # random integer tuples of up to 62-bit spread, scaled by powers of two
+    rng = np.random.default_rng(8)
+    rows = []
+    for bits in (5, 30, 61):
+        for scale in (-250, 0, 200):
+            q = rng.integers(-(2 ** bits), 2 ** bits, size=(50, 12)).astype(np.float64)
+            q[:10, 3:6] = q[:10, 0:3]                       # repeated point: det 0
+            rows.append(np.ldexp(q, scale))
+    t = np.concatenate(rows)
+    ok, sign = _run_o3(t, tmp_path)
+    want = oracle.orient3d_batch(np.ascontiguousarray(t.T), oracle.EXACT).astype(np.int64)
+    assert np.array_equal(sign[ok[:, 2], 2], want[ok[:, 2]])
+    assert ok[:, 2].mean() > 0.95 and (want[ok[:, 2]] == 0).any()
+    # 64..120-bit spreads: only the 126-bit stage decides
+    rows = []
+    for spread in (64, 90, 120):
+        big = np.ldexp(rng.integers(-(2 ** 40), 2 ** 40, size=(60, 12)).astype(np.float64),
+                       spread - 41)
+        small = rng.integers(1, 2 ** 20, size=(60, 12)).astype(np.float64)
+        q = big + np.where(rng.random((60, 12)) < 0.3, small, 0.0)
+        q[:10, 3:6] = q[:10, 0:3]
+        rows.append(q)
+    t = np.concatenate(rows)
+    ok, sign = _run_o3(t, tmp_path)
+    want = oracle.orient3d_batch(np.ascontiguousarray(t.T), oracle.EXACT).astype(np.int64)
+    assert np.array_equal(sign[ok[:, 3], 3], want[ok[:, 3]])
+    assert ok[:, 3].mean() > 0.95 and (want[ok[:, 3]] == 0).any()
