@@ -87,8 +87,20 @@ def bench_predicates(cm, W, steps, warmup):
     cm.kernel_timer("orient3d_filter", False)
     filt_ms = kms / max(cnt, 1)
     stats = cm.pred_stats(n, 4, workspace=ws, stream=stream)
+    signs = out.cpu().numpy()
+    # host-resident batch (pinned): copy in, signs, copy out, per batch (PCIe included)
+    host = torch.from_numpy(soa_np).pin_memory()
+    host_out = torch.empty(n, dtype=torch.int8).pin_memory()
+
+    def host_step():
+        soa.copy_(host, non_blocking=True)
+        step()
+        host_out.copy_(out, non_blocking=True)
+
+    pcie_ms = time_steps(host_step, 3, 1, torch.cuda.synchronize)
+    del host, host_out
     res = {"n": n, "ms": ms, "filter_ms": filt_ms, "stats": stats, "soa_np": soa_np,
-           "signs": out.cpu().numpy()}
+           "signs": signs, "pcie_ms": pcie_ms}
     del soa, ws
     # insphere
     n5 = 1 << 22
@@ -351,6 +363,8 @@ def main():
     st = pr["stats"]
     extra = {
         "orient3d": {"tuples": n, "ms_per_batch": pr["ms"], "filter_kernel_ms": pr["filter_ms"],
+                     "host_resident_signs_per_s": n / (pr["pcie_ms"] / 1e3),
+                     "host_resident_ms_per_batch": pr["pcie_ms"],
                      "stage_fraction": {"fp32": st["fp32"] / n, "fp64": st["fp64"] / n,
                                         "exact": st["exact"] / n,
                                         "exact_inline": st["exact_inline"] / n,
