@@ -996,11 +996,13 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         }
         if (wave_any(any_near)) {
             ++ver;  // new bounds: nothing on the stack was tested against them
-            float b2n[VPL];
+            float b2n[VPL], thr[VPL];
 #pragma unroll
             for (int v = 0; v < VPL; ++v) {
                 const float bb = bound_root(best[v]) + 2.0f * eps;
                 b2n[v] = near[v] ? bb * bb : -1.0f;
+                const float tb = bound_root(best[v]) + 3.0f * eps;
+                thr[v] = tb * tb;
             }
             walk(b2n, [&](int loc, int cnt) {
                 const long long s0 = leaf_rec(loc, cnt);
@@ -1011,6 +1013,16 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
 #pragma unroll
                     for (int v = 0; v < VPL; ++v) {
                         need[v] = tri_lb2(T, px, py, pz[v]) <= b2n[v];
+                        anyn |= need[v];
+                    }
+                    if (!wave_any(anyn)) continue;
+                    // only a triangle within twice the FP32 margin of the FP32 minimum can
+                    // be the FP64 minimum (or tie with it): its FP32 distance decides
+                    // whether the FP64 one is needed
+                    anyn = false;
+#pragma unroll
+                    for (int v = 0; v < VPL; ++v) {
+                        need[v] = need[v] && tri_d2(T, px, py, pz[v]) <= thr[v];
                         anyn |= need[v];
                     }
                     if (!wave_any(anyn)) continue;
