@@ -230,6 +230,31 @@ __global__ __launch_bounds__(256) void morton_kernel(Ctx C, unsigned* code) {
 }
 
 // Per tree slot: the triangle records. Per triangle: the xy bucket range and tile counts.
+// ctr[key] += 1 for every active lane that has a key, with one atomic per distinct key in
+// the wavefront; returns the lane's slot (the counter's old value plus its rank among the
+// lanes with that key). Triangles in slot order are spatially close, so the lanes of a
+// wavefront mostly share a column tile.
+__device__ __forceinline__ int wave_increment(int* ctr, int key, bool has) {
+    const int lane = (int)__lane_id();
+    int slot = 0;
+    while (true) {
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(has);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const int lk = __shfl(key, leader);
+        const bool match = has && key == lk;
+        const unsigned long long mm = __builtin_amdgcn_ballot_w64(match);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], __popcll(mm));
+        base = __shfl(base, leader);
+        if (match) {
+            slot = base + __popcll(mm & ((1ull << lane) - 1ull));
+            has = false;
+        }
+    }
+    return slot;
+}
+
 __global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < C.total_tris;
@@ -247,9 +272,14 @@ __global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
         C.rec64[s] = r64;
         const int4 x = xy_range(P, M.g);
         C.xyr[t] = x;
-        for (int tj = x.z / kTile; tj <= x.w / kTile; ++tj)
-            for (int ti = x.x / kTile; ti <= x.y / kTile; ++ti)
-                atomicAdd(&C.tile_count[M.tile_base + tj * M.ntx + ti], 1);
+        // the r-th column tile of this triangle, r = 0, 1, ... (all lanes together)
+        const int tx0 = x.x / kTile, nx = x.y / kTile - tx0 + 1;
+        const int ty0 = x.z / kTile, ntiles = nx * (x.w / kTile - ty0 + 1);
+        for (int r = 0; __builtin_amdgcn_ballot_w64(r < ntiles) != 0; ++r) {
+            const bool has = r < ntiles;
+            const int gt = has ? M.tile_base + (ty0 + r / nx) * M.ntx + tx0 + r % nx : 0;
+            wave_increment(C.tile_count, gt, has);
+        }
     }
 }
 
@@ -266,17 +296,20 @@ __global__ __launch_bounds__(256) void tile_alloc_kernel(Ctx C) {
 
 __global__ __launch_bounds__(256) void tile_fill_kernel(Ctx C) {
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.total_tris;
-         t += stride) {
-        const MeshDesc& M = C.mesh[mesh_of_tri(C, t)];
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < C.total_tris;
+         s += stride) {
+        const MeshDesc& M = C.mesh[mesh_of_tri(C, s)];
+        const int t = C.perm[s];  // slot order: neighbouring lanes share tiles
         const int4 x = C.xyr[t];
-        for (int tj = x.z / kTile; tj <= x.w / kTile; ++tj)
-            for (int ti = x.x / kTile; ti <= x.y / kTile; ++ti) {
-                const int gt = M.tile_base + tj * M.ntx + ti;
-                const int slot = atomicAdd(&C.tile_cursor[gt], 1);
-                const long long at = (long long)C.tile_off[gt] + slot;
-                if (at < C.tile_cap) C.tile_pool[at] = (int)t;
-            }
+        const int tx0 = x.x / kTile, nx = x.y / kTile - tx0 + 1;
+        const int ty0 = x.z / kTile, ntiles = nx * (x.w / kTile - ty0 + 1);
+        for (int r = 0; __builtin_amdgcn_ballot_w64(r < ntiles) != 0; ++r) {
+            const bool has = r < ntiles;
+            const int gt = has ? M.tile_base + (ty0 + r / nx) * M.ntx + tx0 + r % nx : 0;
+            const int slot = wave_increment(C.tile_cursor, gt, has);
+            const long long at = has ? (long long)C.tile_off[gt] + slot : 0;
+            if (has && at < C.tile_cap) C.tile_pool[at] = t;
+        }
     }
 }
 
