@@ -136,3 +136,24 @@ def test_invalid_inputs_raise(gpu):
         gpu.orient3d_signs(np.zeros((3, 3)), np.array([[0, 1, 2, 3]]))
     with pytest.raises(ValueError):
         gpu.orient3d_signs(np.zeros((4, 3)), np.array([[0, 1, 2, 3]]), mode="fast")
+
+
+def test_fixed_point_stages_on_device(gpu):
+    """Tuples whose exact differences span 64..120 bits (some with exact zeros from a
+    repeated point) and workload 1-ulp tuples: the register-only fixed-point and
+    double-word stages of the exact passes, run on the device, against the oracle."""
+    rng = np.random.default_rng(17)
+    for npts, batch in ((4, oracle.orient3d_batch), (5, oracle.insphere_batch)):
+        rows = []
+        for spread in (40, 64, 90, 120):
+            big = np.ldexp(rng.integers(-(2 ** 40), 2 ** 40, size=(200, 3 * npts))
+                           .astype(np.float64), spread - 41)
+            small = rng.integers(1, 2 ** 20, size=(200, 3 * npts)).astype(np.float64)
+            t = big + np.where(rng.random((200, 3 * npts)) < 0.3, small, 0.0)
+            t[:50, 3:6] = t[:50, 0:3]
+            rows.append(t)
+        soa = np.ascontiguousarray(np.concatenate(rows).T)
+        got = gpu.signs_soa(_dev(soa), "exact").cpu().numpy()
+        want = batch(soa, oracle.EXACT)
+        assert (want == 0).sum() >= 200
+        assert (got == want).all(), (npts, np.nonzero(got != want)[0][:10])
