@@ -221,19 +221,19 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
 }
 
 // Register-only exact stage run inside the streaming kernels for the lanes the filters
-// leave undecided (orient3d only): an exact sum of the triple products when the
-// coordinate differences are exact, else double-word arithmetic with a rigorous bound.
-// Lanes it cannot decide (in practice exact zeros with inexact differences) are queued
-// for the exact pass, whose fixed-point stages resolve them.
+// leave undecided. orient3d: an exact sum of the triple products when the coordinate
+// differences are exact, else double-word arithmetic with a rigorous bound; the lanes it
+// cannot decide (in practice exact zeros with inexact differences) are queued for the
+// exact pass, whose fixed-point stages resolve them. insphere: fixed point with 62-bit
+// differences, else the double-word bound; the rest is queued for the 126-bit stage.
 template <int NPTS>
 __device__ __forceinline__ bool decide_inline_exact(const double* v, int& s) {
     if constexpr (NPTS == 4) {
         if (orient3d_sign_exact_diffs(v, v + 3, v + 6, v + 9, s)) return true;
         return orient3d_sign_dw(v, v + 3, v + 6, v + 9, s);
     } else {
-        (void)v;
-        (void)s;
-        return false;
+        return insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
+               insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s);
     }
 }
 
@@ -302,10 +302,14 @@ __device__ __forceinline__ void wave_sync() {
 // Per-wave LDS ring of tuples the filters left undecided (orient3d): they are pushed as
 // the stream goes by and handed to the register-only exact stage 64 at a time, so that
 // stage runs on full wavefronts instead of on the ~10 % of lanes that need it.
-constexpr unsigned kRing = 192;  // >= 63 pending + 128 pushed per iteration
+// Ring entries: orient3d drains once per iteration (>= 63 pending + 128 pushed); insphere,
+// whose 15 coordinates take more LDS, drains after each of its two pushes (>= 63 + 64).
+template <int NPL>
+constexpr unsigned ring_size() { return NPL == 12 ? 192 : 128; }
 
 template <int NPL>
 struct StageRing {
+    static constexpr unsigned kRing = ring_size<NPL>();
     double* v;      // [NPL][kRing] for this wave
     uint32_t* idx;  // [kRing]
     unsigned head = 0, count = 0;  // wave-uniform
@@ -352,9 +356,10 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                                                          int8_t* __restrict__ out,
                                                          PredWorkspace ws, unsigned iters) {
     constexpr int NPL = NPTS * 3;
-    constexpr bool STAGE = NPTS == 4;
+    constexpr bool STAGE = true;
     // the global counters are only touched by the passes after this one
     if (blockIdx.x == 0 && threadIdx.x == 0) *ws.ctr = PredCounters{};
+    constexpr unsigned kRing = ring_size<NPL>();
     __shared__ double s_ring[STAGE ? 4 : 1][STAGE ? NPL * kRing : 1];
     __shared__ uint32_t s_idx[STAGE ? 4 : 1][STAGE ? kRing : 1];
     unsigned n32 = 0, n64 = 0, nx = 0;
@@ -417,10 +422,16 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
             n32 += (!u0 && st0 == 1) + (two && !u1 && st1 == 1);
             n64 += (!u0 && st0 == 2) + (two && !u1 && st1 == 2);
         }
-        if (STAGE) {
+        if (STAGE && NPTS == 4) {
             ring.push(u0, v0, (uint32_t)i0);
             ring.push(u1, v1, (uint32_t)(i0 + 1));
             while (ring.count >= 64) drain();
+        } else if (STAGE) {
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                ring.push(h ? u1 : u0, h ? v1 : v0, (uint32_t)(i0 + h));
+                if (ring.count >= 64) drain();
+            }
         } else {
             wave_push(u0, (uint32_t)i0, seg, off);
             wave_push(u1, (uint32_t)(i0 + 1), seg, off);
@@ -513,10 +524,10 @@ __device__ __forceinline__ uint32_t queued(const PredWorkspace& ws, uint32_t k) 
     return ws.queue[(size_t)lo * ws.seg + (k - ws.qpref[lo])];
 }
 
-// Register-only exact stages over the insphere queue, one lane per queued tuple, no
-// private arrays: fixed point with 62-bit differences, else the double-word determinant
-// when it clears its error bound. What is left goes to the rest list (one atomic per
-// wavefront) for the exact pass.
+// Register-only exact stage over the insphere queue (the lanes the streaming kernel's
+// fixed-point and double-word stages left), one lane per queued tuple, no private arrays:
+// fixed point with 126-bit differences. What is left goes to the rest list (one atomic
+// per wavefront) for the expansion pass.
 template <int NPTS, class Src>
 __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict__ out,
                                                     PredWorkspace ws) {
@@ -533,8 +544,7 @@ __global__ __launch_bounds__(256) void fixed_kernel(Src src, int8_t* __restrict_
             double v[NPTS * 3];
             src.load(i, v);
             int s;
-            const bool ok = insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
-                            insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s);
+            const bool ok = insphere_sign_fixed_wide(v, v + 3, v + 6, v + 9, v + 12, s);
             if (ok) {
                 out[i] = (int8_t)s;
                 ++done;
