@@ -15,6 +15,7 @@
 
 #include "../../include/cmg_hip.h"
 #include "batch_internal.hpp"
+#include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
 
@@ -26,6 +27,10 @@ struct cmg_plc {
 
 struct cmg_batch {
     std::unique_ptr<cmg::hip::BatchEngine> engine;
+};
+
+struct cmg_carve {
+    std::unique_ptr<cmg::hip::CarveEngine> engine;
 };
 
 struct cmg_voxels {
@@ -519,6 +524,101 @@ cmg_status cmg_hip_kernel_time(const char* name, double* total_ms, uint64_t* cou
                                char* errbuf, size_t len) {
     if (!name) return CMG_ERR_INVALID_INPUT;
     return guarded(errbuf, len, [&] { cmg::hip::timer_read(name, total_ms, count); });
+}
+
+// ---- 5. PLC carve ------------------------------------------------------------------------
+
+cmg_status cmg_carve_create(const double* xyz, size_t npts, const int* tris, size_t ntris,
+                            int mode, int resolution, void* stream, cmg_carve** out,
+                            char* errbuf, size_t len) {
+    if (out) *out = nullptr;
+    if (!out) {
+        set_err(errbuf, len, "carve: null output handle");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        auto c = std::make_unique<cmg_carve>();
+        c->engine = std::make_unique<cmg::hip::CarveEngine>(
+            xyz, (long long)npts, tris, (long long)ntris, mode, resolution, as_stream(stream));
+        *out = c.release();
+    });
+}
+
+void cmg_carve_destroy(cmg_carve* c) { delete c; }
+
+cmg_status cmg_carve_points(cmg_carve* c, const double* q, size_t nq, uint8_t* inside,
+                            void* stream, char* errbuf, size_t len) {
+    if (!c) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        g_last_backend = "hip";
+        c->engine->points(q, (long long)nq, inside, as_stream(stream));
+    });
+}
+
+cmg_status cmg_carve_tets(cmg_carve* c, const int* tets, size_t ntets, uint8_t* keep,
+                          void* stream, char* errbuf, size_t len) {
+    if (!c) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        g_last_backend = "hip";
+        c->engine->tets(tets, (long long)ntets, keep, as_stream(stream));
+    });
+}
+
+cmg_status cmg_carve_status(cmg_carve* c, void* stream, uint64_t* bad_tets, char* errbuf,
+                            size_t len) {
+    if (!c) return CMG_ERR_INVALID_INPUT;
+    return guarded(errbuf, len, [&] {
+        const unsigned long long b = c->engine->bad_tets(as_stream(stream));
+        if (bad_tets) *bad_tets = b;
+    });
+}
+
+cmg_status cmg_plc_carve(const cmg_plc* plc, const int* tets, size_t ntets, int mode,
+                         uint8_t* keep, char* errbuf, size_t len) {
+    if (!plc || (ntets && (!tets || !keep))) {
+        set_err(errbuf, len, "carve: null pointer");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        const size_t npts = plc->xyz.size() / 3;
+        if (npts == 0) throw cmg::hip::InvalidInput("carve: PLC has no points");
+        std::vector<int> tris = fan_triangles(*plc);
+        if (tris.empty())
+            throw cmg::hip::InvalidInput("PLC has facets but no boundary triangles");
+        for (int v : tris)
+            if (v < 0 || (size_t)v >= npts)
+                throw cmg::hip::InvalidInput("carve: facet vertex index out of range");
+        for (size_t i = 0; i < 4 * ntets; ++i)
+            if (tets[i] < 0 || (size_t)tets[i] >= npts)
+                throw cmg::hip::InvalidInput("carve: tet vertex index out of range");
+        hipStream_t st = nullptr;
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t b_xyz = al(plc->xyz.size() * sizeof(double)),
+                     b_tri = al(tris.size() * sizeof(int)), b_tet = al(4 * ntets * sizeof(int)),
+                     b_out = al(ntets);
+        cmg::hip::ScratchLease lease;
+        char* base = static_cast<char*>(lease.s->get(b_xyz + b_tri + b_tet + b_out));
+        double* d_xyz = reinterpret_cast<double*>(base);
+        int* d_tri = reinterpret_cast<int*>(base + b_xyz);
+        int* d_tet = reinterpret_cast<int*>(base + b_xyz + b_tri);
+        uint8_t* d_out = reinterpret_cast<uint8_t*>(base + b_xyz + b_tri + b_tet);
+        CMG_HIP(hipMemcpyAsync(d_xyz, plc->xyz.data(), plc->xyz.size() * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+        CMG_HIP(hipMemcpyAsync(d_tri, tris.data(), tris.size() * sizeof(int),
+                               hipMemcpyHostToDevice, st));
+        if (ntets)
+            CMG_HIP(hipMemcpyAsync(d_tet, tets, 4 * ntets * sizeof(int), hipMemcpyHostToDevice,
+                                   st));
+        cmg::hip::CarveEngine eng(d_xyz, (long long)npts, d_tri, (long long)(tris.size() / 3),
+                                  mode, 0, st);
+        eng.tets(d_tet, (long long)ntets, d_out, st);
+        if (ntets) CMG_HIP(hipMemcpyAsync(keep, d_out, ntets, hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+    });
 }
 
 }  // extern "C"

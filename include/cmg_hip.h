@@ -1,7 +1,7 @@
 /* cmg_hip.h — C ABI of libcmg_hip: CyberMeshGenerator's predicate and voxelization
  * path on AMD Instinct MI355X (gfx950).
  *
- * Plain C types only (pointers, sizes, status codes). Three groups of entry points:
+ * Plain C types only (pointers, sizes, status codes). Groups of entry points:
  *
  *  1. The stable C ABI subset the reference exports for voxelization, byte-compatible
  *     with bindings/c/include/cmg_c/cmg_c.h (declarations cited per function). A caller
@@ -10,6 +10,8 @@
  *     as C-ABI additions for this path; they have no counterpart in cmg_c.h.
  *  3. The device layer that replaces src/backend/cuda (include/cmg/backend/
  *     cuda_kernels.hpp:11-21): device-pointer, stream-ordered entry points.
+ *  4. Batched voxelization of many meshes (device pointers).
+ *  5. The PLC carve's point-in-domain classification (device and host pointers).
  *
  * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
  * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
@@ -245,14 +247,65 @@ cmg_status cmg_voxelize_batch_counters(cmg_batch*, void* stream, uint64_t* out,
 
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
- * "batch_occupancy", "batch_sdf", "batch_sdf_backward"). Events are recorded on the
- * stream the kernel is launched on; launches recorded into a graph are not timed. */
+ * "batch_occupancy", "batch_sdf", "batch_sdf_backward", "carve_query"). Events are
+ * recorded on the stream the kernel is launched on; launches recorded into a graph are
+ * not timed. */
 void cmg_hip_kernel_timer(const char* name, int enable);
 
 /* Total milliseconds and launch count of the timed launches of `name` since the last
  * read (synchronises on the recorded events, then clears them). */
 cmg_status cmg_hip_kernel_time(const char* name, double* total_ms, uint64_t* count,
                                char* errbuf, size_t errbuf_len);
+
+/* ---- 5. PLC carve: point-in-domain classification ---------------------------------- */
+
+/* The classification step of the PLC tetrahedralization (no counterpart in cmg_c.h): the
+ * loop of src/constrained/tetrahedralize_plc.cpp:258-268 that keeps the Delaunay
+ * tetrahedra whose centroid (:170-174) lies inside the boundary, as batched GPU calls.
+ *   CMG_CARVE_RAYS   the reference's classifier (CarveGrid, :60-168): majority of three
+ *                    Moller-Trumbore ray parities over its projection grids; results are
+ *                    bit-identical to it.
+ *   CMG_CARVE_PARITY the voxelizer's exact column parity (src/voxelize/voxelize.cpp:
+ *                    78-195) at the query point: compat orient3d crossing tests, kNudge
+ *                    retries, crossings strictly above the point, on the grid
+ *                    make_grid(points, resolution, 1). */
+enum { CMG_CARVE_RAYS = 0, CMG_CARVE_PARITY = 1 };
+
+typedef struct cmg_carve cmg_carve;
+
+/* Builds the classifier over a closed boundary surface given as device arrays: xyz
+ * (npts points, 3 doubles each) and tris (ntris triangles, 3 ints each), which must stay
+ * valid and unchanged for the handle's lifetime. resolution: parity grid cells per axis
+ * (<= 0: the rays grid's, ceil(sqrt(ntris)) capped at 256; ignored by CMG_CARVE_RAYS).
+ * Synchronises `stream`. ntris == 0 fails with the reference's message "PLC has facets but
+ * no boundary triangles" (tetrahedralize_plc.cpp:250-253). */
+cmg_status cmg_carve_create(const double* xyz, size_t npts, const int* tris, size_t ntris,
+                            int mode, int resolution, void* stream, cmg_carve** out,
+                            char* errbuf, size_t errbuf_len);
+void cmg_carve_destroy(cmg_carve*);
+
+/* inside[i] = 1 iff point q[3i..3i+2] is inside (device arrays; stream-ordered, no host
+ * synchronisation). */
+cmg_status cmg_carve_points(cmg_carve*, const double* q, size_t nq, uint8_t* inside,
+                            void* stream, char* errbuf, size_t errbuf_len);
+
+/* keep[i] = 1 iff the centroid of tets[4i..4i+3] (indices into xyz; device array, 16-byte
+ * aligned) is inside. A tet with an index outside [0, npts) gets 0 and is counted:
+ * cmg_carve_status reports the count (stream-ordered, no host synchronisation). */
+cmg_status cmg_carve_tets(cmg_carve*, const int* tets, size_t ntets, uint8_t* keep,
+                          void* stream, char* errbuf, size_t errbuf_len);
+
+/* Synchronises `stream`; *bad_tets = tets rejected for out-of-range indices since the
+ * last call. */
+cmg_status cmg_carve_status(cmg_carve*, void* stream, uint64_t* bad_tets, char* errbuf,
+                            size_t errbuf_len);
+
+/* Host-memory form of tetrahedralize_plc.cpp:256-268 over a PLC: the boundary triangles
+ * are the fan triangulation of its facets (:26-37), the points its points; keep[i] for
+ * each of the ntets tets (4 ints each, indices into the PLC's points, checked).
+ * Synchronous. */
+cmg_status cmg_plc_carve(const cmg_plc*, const int* tets, size_t ntets, int mode,
+                         uint8_t* keep, char* errbuf, size_t errbuf_len);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -65,6 +65,25 @@ def _load() -> ctypes.CDLL:
     lib.orc_sdf_dist2_to.argtypes = [P, P, P, P, P, i32]
     lib.orc_sdf_fd.restype = d
     lib.orc_sdf_fd.argtypes = [P, sz, P, sz, P, P, P, P, P, i32, i32, d, P, i32]
+    lib.orc_x86_trunc.restype = i32
+    lib.orc_x86_trunc.argtypes = [d]
+    lib.orc_ray_hits.restype = i32
+    lib.orc_ray_hits.argtypes = [P] * 5
+    lib.orc_carve_resolution.restype = i32
+    lib.orc_carve_resolution.argtypes = [sz]
+    lib.orc_carve_create.restype = P
+    lib.orc_carve_create.argtypes = [P, sz, P, sz]
+    lib.orc_carve_destroy.argtypes = [P]
+    lib.orc_carve_cell_entries.restype = i32
+    lib.orc_carve_cell_entries.argtypes = [P, i32]
+    lib.orc_carve_inside.restype = i32
+    lib.orc_carve_inside.argtypes = [P, P]
+    lib.orc_carve_inside_brute.restype = i32
+    lib.orc_carve_inside_brute.argtypes = [P, P, sz, P]
+    lib.orc_tet_centroid.argtypes = [P, P, P]
+    lib.orc_carve_points.argtypes = [P, P, sz, P, i32]
+    lib.orc_carve_tets.argtypes = [P, P, sz, P, i32]
+    lib.orc_carve_parity_points.argtypes = [P, P, sz, P, P, sz, P, i32]
     return lib
 
 
@@ -327,3 +346,78 @@ def voxelize(pts, polygons_or_tris, resolution=64, pad=1, mode="occupancy", nthr
     if mode in ("sdf", 1):
         return grid, sdf_bvh(pts, tris, grid, occ, nthreads=nthreads)[0]
     return grid, occ
+
+
+# ---- PLC carve ----------------------------------------------------------------------
+
+def x86_trunc(v: float) -> int:
+    return lib().orc_x86_trunc(float(v))
+
+
+def ray_hits(o, d, a, b, c) -> bool:
+    o, d, a, b, c = (_f64(x) for x in (o, d, a, b, c))
+    return bool(lib().orc_ray_hits(_p(o), _p(d), _p(a), _p(b), _p(c)))
+
+
+def carve_resolution(ntris: int) -> int:
+    return lib().orc_carve_resolution(ntris)
+
+
+class Carve:
+    """CarveGrid restated (tetrahedralize_plc.cpp:60-168) over pts and tris."""
+
+    def __init__(self, pts, tris):
+        self.pts = _f64(pts).reshape(-1, 3)
+        self.tris = _i32(tris).reshape(-1, 3)
+        self._h = lib().orc_carve_create(_p(self.pts), self.pts.shape[0], _p(self.tris),
+                                         self.tris.shape[0])
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            lib().orc_carve_destroy(h)
+
+    def entries(self, k: int) -> int:
+        return lib().orc_carve_cell_entries(self._h, k)
+
+    def inside(self, q) -> bool:
+        q = _f64(q)
+        return bool(lib().orc_carve_inside(self._h, _p(q)))
+
+    def points(self, q, nthreads=0) -> np.ndarray:
+        q = _f64(q).reshape(-1, 3)
+        out = np.zeros(q.shape[0], dtype=np.uint8)
+        lib().orc_carve_points(self._h, _p(q), q.shape[0], _p(out), nthreads)
+        return out
+
+    def tets(self, tets, nthreads=0) -> np.ndarray:
+        tets = _i32(tets).reshape(-1, 4)
+        out = np.zeros(tets.shape[0], dtype=np.uint8)
+        lib().orc_carve_tets(self._h, _p(tets), tets.shape[0], _p(out), nthreads)
+        return out
+
+
+def carve_inside_brute(pts, tris, q) -> bool:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    q = _f64(q)
+    return bool(lib().orc_carve_inside_brute(_p(pts), _p(tris), tris.shape[0], _p(q)))
+
+
+def tet_centroid(pts, tet) -> np.ndarray:
+    pts = _f64(pts).reshape(-1, 3)
+    tet = _i32(tet)
+    c = np.zeros(3)
+    lib().orc_tet_centroid(_p(pts), _p(tet), _p(c))
+    return c
+
+
+def carve_parity_points(pts, tris, grid: Grid, q, nthreads=0) -> np.ndarray:
+    pts = _f64(pts).reshape(-1, 3)
+    tris = _i32(tris).reshape(-1, 3)
+    q = _f64(q).reshape(-1, 3)
+    out = np.zeros(q.shape[0], dtype=np.uint8)
+    g = grid._c()
+    lib().orc_carve_parity_points(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g),
+                                  _p(q), q.shape[0], _p(out), nthreads)
+    return out

@@ -140,6 +140,42 @@ double orc_sdf_fd(const double* pts, size_t npts, const int* tris, size_t ntris,
                   const double* base_best, const int32_t* base_arg, int vert, int axis,
                   double h, double* tie_bound, int nthreads);
 
+/* ---------------- PLC carve: point-in-domain classification --------------------- */
+
+/* static_cast<int>(double) as the reference's x86-64 build executes it (cvttsd2si):
+ * truncation toward zero; INT_MIN for NaN and for values outside the int range. */
+int orc_x86_trunc(double v);
+
+/* Moller-Trumbore ray/triangle hit, forward only (tetrahedralize_plc.cpp:39-58). */
+int orc_ray_hits(const double* o, const double* d, const double* a, const double* b,
+                 const double* c);
+
+/* CarveGrid (tetrahedralize_plc.cpp:60-168): the projection grids of the three ray
+ * directions over pts (npts points) and tris (ntris triangles, 3 ints each); both
+ * arrays must outlive the object. */
+typedef struct orc_carve orc_carve;
+int orc_carve_resolution(size_t ntris); /* cells per axis, :73-75 */
+orc_carve* orc_carve_create(const double* pts, size_t npts, const int* tris, size_t ntris);
+void orc_carve_destroy(orc_carve* c);
+int orc_carve_cell_entries(const orc_carve* c, int k); /* list entries of direction k */
+int orc_carve_inside(const orc_carve* c, const double* q); /* :79-93 */
+int orc_carve_inside_brute(const double* pts, const int* tris, size_t ntris,
+                           const double* q);
+void orc_tet_centroid(const double* pts, const int* tet, double* c); /* :170-174 */
+void orc_carve_points(const orc_carve* c, const double* q, size_t nq, uint8_t* out,
+                      int nthreads);
+/* keep[i] = inside(centroid of tets[4i..4i+3]) (tetrahedralize_plc.cpp:258-268) */
+void orc_carve_tets(const orc_carve* c, const int* tets, size_t ntets, uint8_t* keep,
+                    int nthreads);
+
+/* Exact column parity (voxelize.cpp:78-195) at arbitrary points q (3 doubles each):
+ * out[n] = 1 iff an odd number of the crossings of the vertical line through
+ * (q.x, q.y), bucketed by the grid cell containing it and nudged as at
+ * voxelize.cpp:154-165, lie strictly above q.z. */
+void orc_carve_parity_points(const double* pts, const int* tris, size_t ntris,
+                             const orc_grid* g, const double* q, size_t nq, uint8_t* out,
+                             int nthreads);
+
 #ifdef __cplusplus
 }
 #endif
