@@ -13,8 +13,9 @@ steps. The JSON line also carries:
                 copy), read from the summary committed under profiles/
   cpu_baseline  the CPU oracle (a restatement of the reference's scalar predicates)
                 over the same batch on one host core
-  extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, and the SDF
-                workloads (forward+backward voxels/s, batched vs plain launches)
+  extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, the SDF
+                workloads (forward+backward voxels/s, batched vs plain launches) and the
+                PLC-carve classification (points/s and tets/s, both modes)
 All other output goes to stderr; stdout holds exactly one JSON line.
 """
 from __future__ import annotations
@@ -312,6 +313,61 @@ def bench_sdf(cm, W, steps, warmup, variant=None, full=True):
     return out
 
 
+def bench_carve(cm, W, steps, warmup, cpu=True):
+    """PLC-carve classification on the C3 torus (65,536 triangles, 256 x 256 projection
+    grids): 2^22 query points uniform in its bounding box padded by 10 % (rays and parity
+    modes) and 2^22 tets whose centroids are classified (rays), all resident in HBM."""
+    import torch
+    from cybermeshgenerator_amd.carve import CarveGrid
+    case = W.torus_c3(seed=3)
+    pts = torch.tensor(case.points, device="cuda")
+    tris = torch.tensor(case.tris, device="cuda")
+    rng = np.random.default_rng(31)
+    lo, hi = case.points.min(0), case.points.max(0)
+    span = hi - lo
+    n = 1 << 22
+    q_np = rng.uniform(lo - 0.1 * span, hi + 0.1 * span, (n, 3))
+    q = torch.tensor(q_np, device="cuda")
+    tets = torch.tensor(rng.integers(0, len(case.points), (n, 4)).astype(np.int32),
+                        device="cuda")
+    out = torch.empty(n, dtype=torch.uint8, device="cuda")
+    res = {"mesh": "C3 torus, 65,536 triangles", "queries": n}
+    for mode in ("rays", "parity"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cg = CarveGrid(pts, tris, mode=mode)
+        setup_ms = (time.perf_counter() - t0) * 1e3
+        cm.kernel_timer("carve_query", True)
+        ms = time_steps(lambda: cg.points_inside(q, out), steps, warmup,
+                        torch.cuda.synchronize)
+        k_ms, k_n = cm.kernel_time("carve_query")
+        cm.kernel_timer("carve_query", False)
+        r = {"setup_ms": setup_ms, "ms_per_batch": ms, "points_per_s": n / (ms / 1e3),
+             "query_kernel_ms": k_ms / max(k_n, 1),
+             "inside_fraction": float(out.float().mean().item())}
+        if mode == "rays":
+            tms = time_steps(lambda: cg.tets_inside(tets, out), steps, warmup,
+                             torch.cuda.synchronize)
+            r["tets_ms_per_batch"] = tms
+            r["tets_per_s"] = n / (tms / 1e3)
+        res[mode] = r
+        log(f"[bench] carve {mode}: {n / (ms / 1e3) / 1e9:.2f} G points/s "
+            f"({ms:.3f} ms per 2^22), set-up {setup_ms:.1f} ms")
+        del cg
+    if cpu:
+        import oracle
+        cg = oracle.Carve(case.points, case.tris)
+        m = 1 << 16
+        t0 = time.perf_counter()
+        want = cg.points(q_np[:m], nthreads=1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "points/s", "cores": 1,
+                               "kind": "port", "sample": f"{m} of the query points, rays"}
+        got = CarveGrid(pts, tris, "rays").points_inside(q[:m]).cpu().numpy()
+        res["cpu_baseline"]["identical"] = bool(np.array_equal(got, want))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +375,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
+    ap.add_argument("--no-carve", action="store_true")
+    ap.add_argument("--carve-only", action="store_true",
+                    help="time only the PLC-carve classification workload")
     ap.add_argument("--sdf-only", action="store_true",
                     help="time only the batched 64-mesh SDF step (no predicates, plain "
                          "path or single meshes) and print its figures: for profiling")
@@ -353,6 +412,12 @@ def main():
         print(json.dumps({"workload": "C5 batched SDF forward+backward, 64 meshes",
                           "sdf": bench_sdf(cm, W, args.steps, args.warmup, variant,
                                            full=False)}), flush=True)
+        return
+
+    if args.carve_only:
+        print(json.dumps({"workload": "PLC carve classification, C3 torus, 2^22 queries",
+                          "carve": bench_carve(cm, W, args.steps, args.warmup,
+                                               cpu=not args.no_cpu)}), flush=True)
         return
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
@@ -393,6 +458,13 @@ def main():
         except Exception as e:  # reported, never hides the predicate line
             log(f"[bench] sdf workloads failed: {e!r}")
             extra["sdf"] = {"error": repr(e)}
+
+    if not args.no_carve:
+        try:
+            extra["carve"] = bench_carve(cm, W, args.steps, args.warmup, not args.no_cpu)
+        except Exception as e:  # reported, never hides the predicate line
+            log(f"[bench] carve workload failed: {e!r}")
+            extra["carve"] = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu:
