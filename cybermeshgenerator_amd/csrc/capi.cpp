@@ -8,6 +8,7 @@
 // entry point fails with CMG_ERR_INTERNAL.
 #include <hip/hip_runtime.h>
 
+#include <cctype>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -18,11 +19,13 @@
 #include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
+#include "weld_internal.hpp"
 
 struct cmg_plc {
     std::vector<double> xyz;            // 3 per point
     std::vector<std::vector<int>> polys;  // one polygon per facet (cmg_c.cpp:88-100)
     std::vector<int> markers;
+    std::vector<int> tri_cache;           // cmg_plc_triangles (cmg_c.cpp:392-415)
 };
 
 struct cmg_batch {
@@ -618,6 +621,80 @@ cmg_status cmg_plc_carve(const cmg_plc* plc, const int* tets, size_t ntets, int 
         eng.tets(d_tet, (long long)ntets, d_out, st);
         if (ntets) CMG_HIP(hipMemcpyAsync(keep, d_out, ntets, hipMemcpyDeviceToHost, st));
         CMG_HIP(hipStreamSynchronize(st));
+    });
+}
+
+// ---- 6. surface files -------------------------------------------------------------------
+
+cmg_status cmg_read_plc(const char* path, cmg_plc** out, char* errbuf, size_t len) {
+    if (out) *out = nullptr;
+    if (!path || !out) return CMG_ERR_INVALID_INPUT;
+    // extension dispatch (src/io/dispatch.cpp:20-27, 41-55, 63-73)
+    std::string p(path), ext;
+    const size_t dot = p.rfind('.');
+    if (dot != std::string::npos)
+        for (char c : p.substr(dot + 1)) ext.push_back((char)std::tolower((unsigned char)c));
+    if (ext != "stl" && ext != "obj") {
+        if (ext == "poly" || ext == "smesh" || ext == "off" || ext == "ply") {
+            set_err(errbuf, len, "libcmg_hip reads .stl and .obj surfaces only: " + p);
+            return CMG_ERR_NOT_IMPLEMENTED;
+        }
+        set_err(errbuf, len, "unsupported file format for " + p);
+        return CMG_ERR_INVALID_INPUT;
+    }
+    return guarded(errbuf, len, [&] {
+        cmg::hip::SurfaceFile f;
+        if (ext == "stl") {
+            require_device();
+            g_last_backend = "hip";
+            f = cmg::hip::read_stl_file(p);
+        } else {
+            f = cmg::hip::read_obj_file(p);
+        }
+        auto plc = std::make_unique<cmg_plc>();
+        plc->xyz = std::move(f.xyz);
+        plc->polys = std::move(f.polys);
+        plc->markers.assign(plc->polys.size(), 0);
+        *out = plc.release();
+    });
+}
+
+size_t cmg_plc_num_points(const cmg_plc* p) { return p ? p->xyz.size() / 3 : 0; }
+
+const double* cmg_plc_points(cmg_plc* p) { return p ? p->xyz.data() : nullptr; }
+
+size_t cmg_plc_num_triangles(cmg_plc* p) {
+    if (!p) return 0;
+    p->tri_cache = fan_triangles(*p);
+    return p->tri_cache.size() / 3;
+}
+
+const int* cmg_plc_triangles(cmg_plc* p) {
+    if (!p) return nullptr;
+    p->tri_cache = fan_triangles(*p);
+    return p->tri_cache.data();
+}
+
+cmg_status cmg_hip_stl_decode(const void* bytes, size_t nbytes, double* xyz, int* tris,
+                              int64_t* npoints, void* stream, char* errbuf, size_t len) {
+    if (npoints) *npoints = 0;
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        const long long n = cmg::hip::stl_decode_weld(bytes, nbytes, xyz, tris, as_stream(stream));
+        if (npoints) *npoints = n;
+    });
+}
+
+cmg_status cmg_hip_weld(const double* soup, size_t nverts, double* xyz, int* idx,
+                        int64_t* npoints, void* stream, char* errbuf, size_t len) {
+    if (npoints) *npoints = 0;
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        const long long n = cmg::hip::weld_vertices(soup, (long long)nverts, xyz, idx,
+                                                   as_stream(stream));
+        if (npoints) *npoints = n;
     });
 }
 

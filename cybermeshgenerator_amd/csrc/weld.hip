@@ -1,0 +1,214 @@
+// weld.hip — surface input straight to device arrays: binary STL decoding and the merge
+// of coincident vertices, on the GPU.
+//
+// The reference reads an STL file into a vertex soup (three vertices per triangle) and
+// merges coincident vertices through an ordered map on the exact coordinate triple,
+// numbering the merged points in order of first occurrence (src/io/stl.cpp:24-61,
+// 94-113). Here:
+//   stl_decode_kernel  binary records (50 B each, 2-byte aligned) -> FP64 soup; a
+//                      workgroup stages 256 records (12,800 B, 4-byte aligned) in LDS
+//                      with coalesced dword loads, then each lane unpacks its 9 floats
+//   weld_insert_kernel open-addressing hash table of vertex numbers (2x capacity,
+//                      linear probing): the first vertex of a key claims a slot with a
+//                      CAS, later ones with an equal key lower it with atomicMin, so the
+//                      slot ends holding the key's first occurrence
+//   weld_rep_kernel    every vertex finds its key's slot: rep = first occurrence; flags
+//                      the vertices that are their own rep
+//   (exclusive scan of the flags: the merged number of each first occurrence)
+//   weld_emit_kernel   merged index of every vertex, first occurrences' coordinates
+// Keys compare as doubles (-0 == +0, as the map's operator< treats them) and hash on
+// bits with -0 folded to +0; the point written is the first occurrence's own value.
+// A NaN coordinate makes its vertex unequal to every vertex (the reference's map has no
+// defined behaviour for NaN keys).
+//
+// Bound: HBM and atomic latency; the table and soup accesses of the insert and rep
+// kernels are random.
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_scan.hpp>
+
+#include <cstring>
+#include <vector>
+
+#include "device_common.hpp"
+#include "voxelize_internal.hpp"
+#include "weld_internal.hpp"
+
+namespace cmg::hip {
+namespace {
+
+constexpr unsigned kEmpty = 0xffffffffu;
+constexpr int kDecodeTris = 256;  // records per workgroup (12,800 B)
+
+__global__ __launch_bounds__(256) void stl_decode_kernel(const uint8_t* bytes, long long ntris,
+                                                         double* soup) {
+    __shared__ uint32_t lds[kDecodeTris * 50 / 4];
+    const long long t0 = (long long)blockIdx.x * kDecodeTris;
+    const int nt = (int)min((long long)kDecodeTris, ntris - t0);
+    const uint8_t* base = bytes + 84 + 50 * t0;  // 84 + 12,800 * block: 4-byte aligned
+    const int nbytes = 50 * nt;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(base);
+    for (int i = threadIdx.x; i < nbytes / 4; i += blockDim.x) lds[i] = w[i];
+    if (threadIdx.x == 0 && (nbytes & 3))  // odd record count: a 2-byte tail
+        lds[nbytes / 4] = reinterpret_cast<const uint16_t*>(base)[nbytes / 2 - 1];
+    __syncthreads();
+    if ((int)threadIdx.x >= nt) return;
+    const uint16_t* h = reinterpret_cast<const uint16_t*>(lds);
+    const int o = (50 * threadIdx.x + 12) / 2;  // vertex floats start 12 B into a record
+    double* out = soup + 9 * (t0 + threadIdx.x);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        const uint32_t bits = (uint32_t)h[o + 2 * j] | ((uint32_t)h[o + 2 * j + 1] << 16);
+        out[j] = (double)__uint_as_float(bits);
+    }
+}
+
+__device__ __forceinline__ unsigned long long key_bits(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b << 1) == 0 ? 0ull : b;  // -0 and +0 hash alike
+}
+
+__device__ __forceinline__ unsigned hash_key(const double* p) {
+    unsigned long long h = key_bits(p[0]) * 0x9E3779B97F4A7C15ull;
+    h ^= key_bits(p[1]) + 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+    h ^= key_bits(p[2]) + 0x165667B19E3779F9ull + (h << 6) + (h >> 2);
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    return (unsigned)h;
+}
+
+__device__ __forceinline__ bool same_point(const double* soup, unsigned a, long long b) {
+    const double* p = soup + 3 * (long long)a;
+    const double* q = soup + 3 * b;
+    return p[0] == q[0] && p[1] == q[1] && p[2] == q[2];
+}
+
+__global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, long long nv,
+                                                          unsigned* tab, unsigned mask) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
+         v += (long long)gridDim.x * blockDim.x) {
+        unsigned s = hash_key(soup + 3 * v) & mask;
+        // at most nv keys in a table of >= 2 nv slots: an empty slot is always reached
+        for (;;) {
+            const unsigned cur = atomicCAS(&tab[s], kEmpty, (unsigned)v);
+            if (cur == kEmpty) break;
+            if (same_point(soup, cur, v)) {
+                atomicMin(&tab[s], (unsigned)v);
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void weld_rep_kernel(const double* soup, long long nv,
+                                                       const unsigned* tab, unsigned mask,
+                                                       unsigned* rep, unsigned* first) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
+         v += (long long)gridDim.x * blockDim.x) {
+        unsigned s = hash_key(soup + 3 * v) & mask;
+        unsigned r = (unsigned)v;
+        for (;;) {
+            const unsigned cur = tab[s];
+            if (cur == kEmpty) break;  // (not reached: every vertex's key was inserted)
+            if (cur == (unsigned)v || same_point(soup, cur, v)) {
+                r = cur;
+                break;
+            }
+            s = (s + 1) & mask;
+        }
+        rep[v] = r;
+        first[v] = r == (unsigned)v;
+    }
+}
+
+__global__ __launch_bounds__(256) void weld_emit_kernel(const double* soup, long long nv,
+                                                        const unsigned* rep,
+                                                        const unsigned* number, double* xyz,
+                                                        int* idx) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
+         v += (long long)gridDim.x * blockDim.x) {
+        const unsigned r = rep[v];
+        idx[v] = (int)number[r];
+        if (r == (unsigned)v) {
+            const long long m = number[v];
+            xyz[3 * m] = soup[3 * v];
+            xyz[3 * m + 1] = soup[3 * v + 1];
+            xyz[3 * m + 2] = soup[3 * v + 2];
+        }
+    }
+}
+
+long long weld_impl(DeviceArena& A, const double* d_soup, long long nv, double* d_xyz,
+                    int* d_idx, hipStream_t st) {
+    unsigned cap = 1024;  // nv <= 2^30: at most 2^31 slots
+    while ((long long)cap < 2 * nv) cap <<= 1;
+    unsigned* tab = A.alloc<unsigned>(cap);
+    unsigned* rep = A.alloc<unsigned>((size_t)nv);
+    unsigned* first = A.alloc<unsigned>((size_t)nv);
+    unsigned* number = A.alloc<unsigned>((size_t)nv);
+    CMG_HIP(hipMemsetAsync(tab, 0xff, sizeof(unsigned) * cap, st));
+    const unsigned nb = grid_for((size_t)nv, 256, 8);
+    {
+        TimedLaunch tl("weld", st);
+        weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1);
+        CMG_HIP_LAUNCH();
+        weld_rep_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep, first);
+        CMG_HIP_LAUNCH();
+        size_t tmp_bytes = 0;
+        CMG_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, first, number, 0u, (size_t)nv,
+                                        rocprim::plus<unsigned>(), st));
+        void* tmp = A.alloc<char>(tmp_bytes);
+        CMG_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, first, number, 0u, (size_t)nv,
+                                        rocprim::plus<unsigned>(), st));
+        weld_emit_kernel<<<nb, 256, 0, st>>>(d_soup, nv, rep, number, d_xyz, d_idx);
+        CMG_HIP_LAUNCH();
+    }
+    unsigned last[2];
+    CMG_HIP(hipMemcpyAsync(&last[0], number + nv - 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                           st));
+    CMG_HIP(hipMemcpyAsync(&last[1], first + nv - 1, sizeof(unsigned), hipMemcpyDeviceToHost,
+                           st));
+    CMG_HIP(hipStreamSynchronize(st));
+    return (long long)last[0] + last[1];
+}
+
+}  // namespace
+
+long long weld_vertices(const double* d_soup, long long nv, double* d_xyz, int* d_idx,
+                        hipStream_t st) {
+    if (nv < 1) throw InvalidInput("weld: no vertices");
+    if (nv > (1LL << 30)) throw InvalidInput("weld: more than 2^30 vertices");
+    if (!d_soup || !d_xyz || !d_idx) throw InvalidInput("weld: null pointer");
+    DeviceArena A(st);
+    return weld_impl(A, d_soup, nv, d_xyz, d_idx, st);
+}
+
+long long stl_decode_weld(const void* d_bytes, size_t nbytes, double* d_xyz, int* d_tris,
+                          hipStream_t st) {
+    if (!d_bytes || !d_xyz || !d_tris) throw InvalidInput("stl: null pointer");
+    if (reinterpret_cast<uintptr_t>(d_bytes) % 4)
+        throw InvalidInput("stl: the byte buffer must be 4-byte aligned");
+    if (nbytes < 84) throw InvalidInput("stl: not a binary STL (shorter than its header)");
+    uint32_t n = 0;
+    CMG_HIP(hipMemcpyAsync(&n, static_cast<const uint8_t*>(d_bytes) + 80, 4,
+                           hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    // stl.cpp:158-161: binary iff the size is exactly 84 + 50 * count
+    if (nbytes != 84ull + 50ull * n)
+        throw InvalidInput("stl: not a binary STL (size is not 84 + 50 * triangle count)");
+    // stl.cpp:50-51
+    if (n == 0) throw InvalidInput(".stl: vertex count is not a positive multiple of 3");
+    if (3ull * n > (1ull << 30)) throw InvalidInput("weld: more than 2^30 vertices");
+    DeviceArena A(st);
+    double* soup = A.alloc<double>(9 * (size_t)n);
+    {
+        TimedLaunch tl("stl_decode", st);
+        stl_decode_kernel<<<(unsigned)((n + kDecodeTris - 1) / kDecodeTris), 256, 0, st>>>(
+            static_cast<const uint8_t*>(d_bytes), n, soup);
+        CMG_HIP_LAUNCH();
+    }
+    return weld_impl(A, soup, 3LL * n, d_xyz, d_tris, st);
+}
+
+}  // namespace cmg::hip

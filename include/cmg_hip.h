@@ -12,6 +12,7 @@
  *     cuda_kernels.hpp:11-21): device-pointer, stream-ordered entry points.
  *  4. Batched voxelization of many meshes (device pointers).
  *  5. The PLC carve's point-in-domain classification (device and host pointers).
+ *  6. Surface files (STL, OBJ) to PLCs and to device arrays.
  *
  * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
  * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
@@ -306,6 +307,40 @@ cmg_status cmg_carve_status(cmg_carve*, void* stream, uint64_t* bad_tets, char* 
  * Synchronous. */
 cmg_status cmg_plc_carve(const cmg_plc*, const int* tets, size_t ntets, int mode,
                          uint8_t* keep, char* errbuf, size_t errbuf_len);
+
+/* ---- 6. surface files -------------------------------------------------------------- */
+
+/* cmg_c.h:104-106: read a PLC boundary from a surface file, format from the extension.
+ * This library reads .stl (ASCII or binary, coincident vertices merged in first-
+ * occurrence order on the GPU; src/io/stl.cpp) and .obj (src/io/obj.cpp) with the
+ * reference's grammar and messages; .poly/.smesh/.off/.ply give CMG_ERR_NOT_IMPLEMENTED,
+ * other extensions CMG_ERR_INVALID_INPUT ("unsupported file format for <path>",
+ * src/io/dispatch.cpp:33-36). On CMG_OK *out is a plc to free with cmg_plc_destroy. */
+cmg_status cmg_read_plc(const char* path, cmg_plc** out, char* errbuf, size_t errbuf_len);
+
+/* cmg_c.h:115-121: a PLC's geometry. Points as 3 * num_points doubles; facets
+ * fan-triangulated to 3 * num_triangles ints. Pointers are valid until the PLC is
+ * modified or destroyed. */
+size_t cmg_plc_num_points(const cmg_plc*);
+const double* cmg_plc_points(cmg_plc*);
+size_t cmg_plc_num_triangles(cmg_plc*);
+const int* cmg_plc_triangles(cmg_plc*);
+
+/* Device layer of the STL reader (no counterpart in the reference: it replaces the host
+ * path of stl.cpp:94-113 and 24-61 for callers that keep meshes on the GPU). A whole
+ * binary STL file in device memory (bytes, 4-byte aligned, nbytes = 84 + 50 * count) ->
+ * merged points xyz (room for 9 * count doubles) and triangles tris (3 * count ints),
+ * the points numbered in first-occurrence order as the reference numbers them.
+ * Synchronises `stream`; *npoints = number of points. */
+cmg_status cmg_hip_stl_decode(const void* bytes, size_t nbytes, double* xyz, int* tris,
+                              int64_t* npoints, void* stream, char* errbuf,
+                              size_t errbuf_len);
+
+/* The merge alone (stl.cpp:24-61) on a device vertex soup of nverts vertices (3 doubles
+ * each, nverts <= 2^30): xyz receives the merged points (room for 3 * nverts doubles),
+ * idx the merged index of every vertex. Synchronises `stream`. */
+cmg_status cmg_hip_weld(const double* soup, size_t nverts, double* xyz, int* idx,
+                        int64_t* npoints, void* stream, char* errbuf, size_t errbuf_len);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -84,6 +84,19 @@ def _load() -> ctypes.CDLL:
     lib.orc_carve_points.argtypes = [P, P, sz, P, i32]
     lib.orc_carve_tets.argtypes = [P, P, sz, P, i32]
     lib.orc_carve_parity_points.argtypes = [P, P, sz, P, P, sz, P, i32]
+    i64 = ctypes.c_int64
+    PP = ctypes.POINTER(P)
+    lib.orc_weld.restype = i64
+    lib.orc_weld.argtypes = [P, i64, P, P]
+    lib.orc_read_stl.restype = i64
+    lib.orc_read_stl.argtypes = [ctypes.c_char_p, PP, PP, ctypes.POINTER(i64),
+                                 ctypes.POINTER(i32), ctypes.c_char_p, sz]
+    lib.orc_stl_soup.restype = i64
+    lib.orc_stl_soup.argtypes = [ctypes.c_char_p, PP, ctypes.POINTER(i32), ctypes.c_char_p, sz]
+    lib.orc_read_obj.restype = i32
+    lib.orc_read_obj.argtypes = [ctypes.c_char_p, PP, ctypes.POINTER(i64), PP, PP,
+                                 ctypes.POINTER(i64), ctypes.c_char_p, sz]
+    lib.orc_free.argtypes = [P]
     return lib
 
 
@@ -421,3 +434,72 @@ def carve_parity_points(pts, tris, grid: Grid, q, nthreads=0) -> np.ndarray:
     lib().orc_carve_parity_points(_p(pts), _p(tris), tris.shape[0], ctypes.byref(g),
                                   _p(q), q.shape[0], _p(out), nthreads)
     return out
+
+
+# ---- surface file readers -------------------------------------------------------------
+
+def weld(soup):
+    """(points, idx): first-occurrence merge of a vertex soup (N, 3) (stl.cpp:24-61)."""
+    soup = _f64(soup).reshape(-1, 3)
+    n = soup.shape[0]
+    pts = np.zeros((max(n, 1), 3))
+    idx = np.zeros(max(n, 1), dtype=np.int32)
+    m = lib().orc_weld(_p(soup), n, _p(pts), _p(idx))
+    return pts[:m].copy(), idx[:n].copy()
+
+
+def _take(ptr, count, dtype):
+    if not ptr or count == 0:
+        if ptr:
+            lib().orc_free(ptr)
+        return np.zeros(0, dtype=dtype)
+    ct = ctypes.c_double if dtype == np.float64 else ctypes.c_int32
+    arr = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), shape=(count,)).copy()
+    lib().orc_free(ptr)
+    return arr
+
+
+class ReadError(ValueError):
+    pass
+
+
+def read_stl(path: str):
+    """(points (V,3), tris (T,3), binary) as src/io/stl.cpp reads them; ReadError with
+    the reference's message on failure."""
+    P_, T_ = ctypes.c_void_p(), ctypes.c_void_p()
+    nt = ctypes.c_int64()
+    binary = ctypes.c_int()
+    err = ctypes.create_string_buffer(512)
+    n = lib().orc_read_stl(path.encode(), ctypes.byref(P_), ctypes.byref(T_),
+                           ctypes.byref(nt), ctypes.byref(binary), err, 512)
+    if n < 0:
+        raise ReadError(err.value.decode())
+    pts = _take(P_.value, 3 * n, np.float64).reshape(-1, 3)
+    tris = _take(T_.value, 3 * nt.value, np.int32).reshape(-1, 3)
+    return pts, tris, bool(binary.value)
+
+
+def stl_soup(path: str):
+    """The vertex soup (3 per triangle, (N,3) float64) before merging, and binary."""
+    S = ctypes.c_void_p()
+    binary = ctypes.c_int()
+    err = ctypes.create_string_buffer(512)
+    n = lib().orc_stl_soup(path.encode(), ctypes.byref(S), ctypes.byref(binary), err, 512)
+    if n < 0:
+        raise ReadError(err.value.decode())
+    return _take(S.value, 3 * n, np.float64).reshape(-1, 3), bool(binary.value)
+
+
+def read_obj(path: str):
+    """(points (V,3), polygons: list of int32 arrays) as src/io/obj.cpp reads them."""
+    P_, O_, I_ = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    npts, nf = ctypes.c_int64(), ctypes.c_int64()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().orc_read_obj(path.encode(), ctypes.byref(P_), ctypes.byref(npts),
+                            ctypes.byref(O_), ctypes.byref(I_), ctypes.byref(nf), err, 512)
+    if rc != 0:
+        raise ReadError(err.value.decode())
+    pts = _take(P_.value, 3 * npts.value, np.float64).reshape(-1, 3)
+    off = _take(O_.value, nf.value + 1, np.int32)
+    idx = _take(I_.value, int(off[-1]) if off.size else 0, np.int32)
+    return pts, [idx[off[f]:off[f + 1]] for f in range(nf.value)]

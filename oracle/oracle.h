@@ -176,6 +176,30 @@ void orc_carve_parity_points(const double* pts, const int* tris, size_t ntris,
                              const orc_grid* g, const double* q, size_t nq, uint8_t* out,
                              int nthreads);
 
+/* ---------------- surface file readers ------------------------------------------ */
+
+/* Merge of coincident vertices on the exact coordinate triple, first-occurrence order
+ * (src/io/stl.cpp:24-61): soup holds nv vertices (3 doubles each, no NaN); pts receives
+ * the merged points (up to nv), idx the merged index of every vertex. Returns the
+ * number of points. */
+int64_t orc_weld(const double* soup, int64_t nv, double* pts, int32_t* idx);
+
+/* read_stl (stl.cpp:153-165): binary iff the size is 84 + 50 * count, else ASCII
+ * "vertex" records; merged as orc_weld. Returns the point count and mallocs *pts
+ * (3 per point) and *tris (3 per triangle), or -1 with err filled (the reference's
+ * messages). *binary tells which form was read. Free with orc_free. */
+int64_t orc_read_stl(const char* path, double** pts, int32_t** tris, int64_t* ntris,
+                     int* binary, char* err, size_t len);
+/* The STL file's vertex soup before merging (3 doubles per vertex). */
+int64_t orc_stl_soup(const char* path, double** soup, int* binary, char* err, size_t len);
+
+/* read_obj (src/io/obj.cpp:33-72): points, and one polygon per "f" statement with
+ * vertex indices idx[off[f] .. off[f+1]) (0-based, unchecked as in the reference).
+ * Returns 0, or -1 with err filled. Free with orc_free. */
+int orc_read_obj(const char* path, double** pts, int64_t* npts, int32_t** off,
+                 int32_t** idx, int64_t* nfacets, char* err, size_t len);
+void orc_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif
