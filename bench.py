@@ -14,8 +14,9 @@ steps. The JSON line also carries:
   cpu_baseline  the CPU oracle (a restatement of the reference's scalar predicates)
                 over the same batch on one host core
   extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, the SDF
-                workloads (forward+backward voxels/s, batched vs plain launches) and the
-                PLC-carve classification (points/s and tets/s, both modes)
+                workloads (forward+backward voxels/s, batched vs plain launches), the
+                PLC-carve classification (points/s and tets/s, both modes) and binary
+                STL input decoded and merged on the GPU
 All other output goes to stderr; stdout holds exactly one JSON line.
 """
 from __future__ import annotations
@@ -368,6 +369,55 @@ def bench_carve(cm, W, steps, warmup, cpu=True):
     return res
 
 
+def bench_stl(cm, W, steps, warmup, cpu=True):
+    """Binary STL to device arrays: a 4,194,304-triangle torus file (200 MiB) resident in
+    HBM, decoded and merged (coincident vertices, first-occurrence numbering) into the
+    float64 points and int32 triangles the voxelizer takes. One step = one call of
+    cmg_hip_stl_decode (its work buffers are allocated and freed inside the call)."""
+    import ctypes
+    import torch
+    from cybermeshgenerator_amd import files
+    files._bind()
+    data = W.binary_stl_bytes()
+    n = (len(data) - 84) // 50
+    dev = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
+    xyz = torch.empty((3 * n, 3), dtype=torch.float64, device="cuda")
+    tris = torch.empty((n, 3), dtype=torch.int32, device="cuda")
+    npts = ctypes.c_int64(0)
+    err = cm._err()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        cm._check(cm.lib().cmg_hip_stl_decode(
+            ctypes.c_void_p(dev.data_ptr()), len(data), ctypes.c_void_p(xyz.data_ptr()),
+            ctypes.c_void_p(tris.data_ptr()), ctypes.byref(npts), st, err, len(err)), err)
+
+    ms = time_steps(step, steps, warmup, torch.cuda.synchronize)
+    res = {"file": "binary STL, torus 2048 x 1024, 4,194,304 triangles",
+           "bytes": len(data), "points": int(npts.value), "ms_per_file": ms,
+           "triangles_per_s": n / (ms / 1e3), "input_gb_per_s": len(data) / (ms / 1e3) / 1e9}
+    log(f"[bench] stl: {n / (ms / 1e3) / 1e9:.3f} G triangles/s ({ms:.2f} ms per file)")
+    if cpu:
+        import tempfile
+        import oracle
+        m = 1 << 18
+        sample = W.binary_stl_bytes(512, 256)
+        with tempfile.NamedTemporaryFile(suffix=".stl", delete=False) as f:
+            f.write(sample)
+        t0 = time.perf_counter()
+        pts, otris, _ = oracle.read_stl(f.name)
+        dt = time.perf_counter() - t0
+        os.unlink(f.name)
+        sd = torch.frombuffer(bytearray(sample), dtype=torch.uint8).cuda()
+        gx, gt = files.stl_decode(sd)
+        res["cpu_baseline"] = {
+            "value": m / dt, "unit": "triangles/s", "cores": 1, "kind": "port",
+            "sample": "262,144-triangle torus file read from disk and merged by the oracle",
+            "identical": bool(np.array_equal(gx.cpu().numpy(), pts)
+                              and np.array_equal(gt.cpu().numpy(), otris))}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,9 +425,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
-    ap.add_argument("--no-carve", action="store_true")
+    ap.add_argument("--no-carve", action="store_true",
+                    help="skip the PLC-carve and STL-input workloads")
     ap.add_argument("--carve-only", action="store_true",
-                    help="time only the PLC-carve classification workload")
+                    help="time only the PLC-carve classification and STL-input workloads")
     ap.add_argument("--sdf-only", action="store_true",
                     help="time only the batched 64-mesh SDF step (no predicates, plain "
                          "path or single meshes) and print its figures: for profiling")
@@ -415,9 +466,11 @@ def main():
         return
 
     if args.carve_only:
-        print(json.dumps({"workload": "PLC carve classification, C3 torus, 2^22 queries",
+        print(json.dumps({"workload": "PLC carve classification and STL input",
                           "carve": bench_carve(cm, W, args.steps, args.warmup,
-                                               cpu=not args.no_cpu)}), flush=True)
+                                               cpu=not args.no_cpu),
+                          "stl": bench_stl(cm, W, args.steps, args.warmup,
+                                           cpu=not args.no_cpu)}), flush=True)
         return
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
@@ -460,6 +513,11 @@ def main():
             extra["sdf"] = {"error": repr(e)}
 
     if not args.no_carve:
+        try:
+            extra["stl"] = bench_stl(cm, W, args.steps, args.warmup, not args.no_cpu)
+        except Exception as e:  # reported, never hides the predicate line
+            log(f"[bench] stl workload failed: {e!r}")
+            extra["stl"] = {"error": repr(e)}
         try:
             extra["carve"] = bench_carve(cm, W, args.steps, args.warmup, not args.no_cpu)
         except Exception as e:  # reported, never hides the predicate line

@@ -22,7 +22,8 @@
 // defined behaviour for NaN keys).
 //
 // Bound: HBM and atomic latency; the table and soup accesses of the insert and rep
-// kernels are random.
+// kernels are random. Work buffers come from the library's per-device scratch (reused
+// across calls; calls on one device serialise on it).
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_scan.hpp>
 
@@ -139,31 +140,57 @@ __global__ __launch_bounds__(256) void weld_emit_kernel(const double* soup, long
     }
 }
 
-long long weld_impl(DeviceArena& A, const double* d_soup, long long nv, double* d_xyz,
-                    int* d_idx, hipStream_t st) {
+// Bump allocation of the work buffers out of one scratch block.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t n) {
+        T* p = reinterpret_cast<T*>(base + off);
+        off += (n * sizeof(T) + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+unsigned table_slots(long long nv) {
     unsigned cap = 1024;  // nv <= 2^30: at most 2^31 slots
     while ((long long)cap < 2 * nv) cap <<= 1;
-    unsigned* tab = A.alloc<unsigned>(cap);
-    unsigned* rep = A.alloc<unsigned>((size_t)nv);
-    unsigned* first = A.alloc<unsigned>((size_t)nv);
-    unsigned* number = A.alloc<unsigned>((size_t)nv);
+    return cap;
+}
+
+size_t scan_tmp_bytes(long long nv, hipStream_t st) {
+    size_t b = 0;
+    CMG_HIP(rocprim::exclusive_scan(nullptr, b, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                    0u, (size_t)nv, rocprim::plus<unsigned>(), st));
+    return b;
+}
+
+// Bytes of the merge's work buffers for nv vertices.
+size_t weld_bytes(long long nv, hipStream_t st) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return al(sizeof(unsigned) * table_slots(nv)) + 3 * al(sizeof(unsigned) * (size_t)nv) +
+           al(scan_tmp_bytes(nv, st));
+}
+
+long long weld_impl(Carve& W, const double* d_soup, long long nv, double* d_xyz, int* d_idx,
+                    hipStream_t st) {
+    const unsigned cap = table_slots(nv);
+    unsigned* tab = W.take<unsigned>(cap);
+    unsigned* rep = W.take<unsigned>((size_t)nv);
+    unsigned* first = W.take<unsigned>((size_t)nv);
+    unsigned* number = W.take<unsigned>((size_t)nv);
+    size_t tmp_bytes = scan_tmp_bytes(nv, st);
+    void* tmp = W.take<char>(tmp_bytes);
     CMG_HIP(hipMemsetAsync(tab, 0xff, sizeof(unsigned) * cap, st));
     const unsigned nb = grid_for((size_t)nv, 256, 8);
-    {
-        TimedLaunch tl("weld", st);
-        weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1);
-        CMG_HIP_LAUNCH();
-        weld_rep_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep, first);
-        CMG_HIP_LAUNCH();
-        size_t tmp_bytes = 0;
-        CMG_HIP(rocprim::exclusive_scan(nullptr, tmp_bytes, first, number, 0u, (size_t)nv,
-                                        rocprim::plus<unsigned>(), st));
-        void* tmp = A.alloc<char>(tmp_bytes);
-        CMG_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, first, number, 0u, (size_t)nv,
-                                        rocprim::plus<unsigned>(), st));
-        weld_emit_kernel<<<nb, 256, 0, st>>>(d_soup, nv, rep, number, d_xyz, d_idx);
-        CMG_HIP_LAUNCH();
-    }
+    weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1);
+    CMG_HIP_LAUNCH();
+    weld_rep_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep, first);
+    CMG_HIP_LAUNCH();
+    CMG_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, first, number, 0u, (size_t)nv,
+                                    rocprim::plus<unsigned>(), st));
+    weld_emit_kernel<<<nb, 256, 0, st>>>(d_soup, nv, rep, number, d_xyz, d_idx);
+    CMG_HIP_LAUNCH();
     unsigned last[2];
     CMG_HIP(hipMemcpyAsync(&last[0], number + nv - 1, sizeof(unsigned), hipMemcpyDeviceToHost,
                            st));
@@ -180,8 +207,9 @@ long long weld_vertices(const double* d_soup, long long nv, double* d_xyz, int* 
     if (nv < 1) throw InvalidInput("weld: no vertices");
     if (nv > (1LL << 30)) throw InvalidInput("weld: more than 2^30 vertices");
     if (!d_soup || !d_xyz || !d_idx) throw InvalidInput("weld: null pointer");
-    DeviceArena A(st);
-    return weld_impl(A, d_soup, nv, d_xyz, d_idx, st);
+    ScratchLease lease;
+    Carve W{static_cast<char*>(lease.s->get(weld_bytes(nv, st)))};
+    return weld_impl(W, d_soup, nv, d_xyz, d_idx, st);
 }
 
 long long stl_decode_weld(const void* d_bytes, size_t nbytes, double* d_xyz, int* d_tris,
@@ -200,15 +228,14 @@ long long stl_decode_weld(const void* d_bytes, size_t nbytes, double* d_xyz, int
     // stl.cpp:50-51
     if (n == 0) throw InvalidInput(".stl: vertex count is not a positive multiple of 3");
     if (3ull * n > (1ull << 30)) throw InvalidInput("weld: more than 2^30 vertices");
-    DeviceArena A(st);
-    double* soup = A.alloc<double>(9 * (size_t)n);
-    {
-        TimedLaunch tl("stl_decode", st);
-        stl_decode_kernel<<<(unsigned)((n + kDecodeTris - 1) / kDecodeTris), 256, 0, st>>>(
-            static_cast<const uint8_t*>(d_bytes), n, soup);
-        CMG_HIP_LAUNCH();
-    }
-    return weld_impl(A, soup, 3LL * n, d_xyz, d_tris, st);
+    ScratchLease lease;
+    const size_t soup_bytes = (9 * (size_t)n * sizeof(double) + 255) & ~size_t(255);
+    Carve W{static_cast<char*>(lease.s->get(soup_bytes + weld_bytes(3LL * n, st)))};
+    double* soup = W.take<double>(9 * (size_t)n);
+    stl_decode_kernel<<<(unsigned)((n + kDecodeTris - 1) / kDecodeTris), 256, 0, st>>>(
+        static_cast<const uint8_t*>(d_bytes), n, soup);
+    CMG_HIP_LAUNCH();
+    return weld_impl(W, soup, 3LL * n, d_xyz, d_tris, st);
 }
 
 }  // namespace cmg::hip

@@ -366,3 +366,20 @@ def tuples_insphere(n: int, seed: int = 22, degenerate: float = 0.10):
         _ulp_perturb(rng, soa, half)
         kind[half] = 2
     return np.ascontiguousarray(soa), kind
+
+
+def binary_stl_bytes(nu: int = 2048, nv: int = 1024, seed: int = 6) -> bytes:
+    """A binary STL file (80-byte header, uint32 count, 50-byte records) of a closed torus
+    (R = 0.3, r = 0.12, centred at 0.5) tessellated nu x nv (2 * nu * nv triangles, every
+    vertex shared by six of them), float32 coordinates with seeded jitter: the input of
+    the STL decode and vertex-merge workload."""
+    rng = np.random.default_rng(seed)
+    pts, tris, _ = torus(0.3, 0.12, nu, nv)
+    pts = pts + 0.5 + rng.normal(scale=1e-4, size=pts.shape)
+    soup = pts.astype("<f4")[tris]
+    rec = np.zeros(len(tris), dtype=[("normal", "<f4", 3), ("v", "<f4", (3, 3)),
+                                     ("attr", "<u2")])
+    rec["v"] = soup
+    head = b"torus workload".ljust(80, b"\0") + np.uint32(len(tris)).astype("<u4").tobytes()
+    return head + rec.tobytes()
+
