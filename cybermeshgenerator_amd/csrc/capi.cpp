@@ -19,6 +19,7 @@
 #include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
+#include "quality_internal.hpp"
 #include "weld_internal.hpp"
 
 struct cmg_plc {
@@ -695,6 +696,39 @@ cmg_status cmg_hip_weld(const double* soup, size_t nverts, double* xyz, int* idx
         const long long n = cmg::hip::weld_vertices(soup, (long long)nverts, xyz, idx,
                                                    as_stream(stream));
         if (npoints) *npoints = n;
+    });
+}
+
+// ---- 7. mesh quality report --------------------------------------------------------------
+
+cmg_status cmg_hip_quality_report(const double* xyz, size_t npts, const int* tets,
+                                  size_t ntets, int worst_count, cmg_quality_report* out,
+                                  int* worst, void* stream, char* errbuf, size_t len) {
+    if (!out) {
+        set_err(errbuf, len, "quality: null report");
+        return CMG_ERR_INVALID_INPUT;
+    }
+    *out = cmg_quality_report{};
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        std::vector<int> w;
+        const cmg::hip::QualityReport r = cmg::hip::quality_report(
+            xyz, (long long)npts, tets, (long long)ntets, worst_count, worst ? &w : nullptr,
+            as_stream(stream));
+        out->num_tets = (uint64_t)r.num_tets;
+        out->min_radius_edge = r.min_radius_edge;
+        out->max_radius_edge = r.max_radius_edge;
+        out->mean_radius_edge = r.mean_radius_edge;
+        out->min_dihedral = r.min_dihedral;
+        out->max_dihedral = r.max_dihedral;
+        out->mean_dihedral = r.mean_dihedral;
+        out->min_volume = r.min_volume;
+        out->max_volume = r.max_volume;
+        out->total_volume = r.total_volume;
+        for (int b = 0; b < 18; ++b) out->dihedral_histogram[b] = r.dihedral_histogram[b];
+        out->num_worst = (uint64_t)w.size();
+        if (worst && !w.empty()) std::memcpy(worst, w.data(), sizeof(int) * w.size());
     });
 }
 

@@ -383,3 +383,28 @@ def binary_stl_bytes(nu: int = 2048, nv: int = 1024, seed: int = 6) -> bytes:
     head = b"torus workload".ljust(80, b"\0") + np.uint32(len(tris)).astype("<u4").tobytes()
     return head + rec.tobytes()
 
+
+def lattice_tets(m: int, jitter: float = 0.0, seed: int = 7):
+    """A tetrahedral mesh of the unit cube: an (m+1)^3 lattice of points, every cell split
+    into the 6 tets of its Kuhn (path) subdivision (6 m^3 tets, all congruent when
+    jitter = 0: dihedral angles of 45, 60 and 90 degrees), points moved by a seeded
+    uniform jitter of up to `jitter` cells on interior points."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(m + 1)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pts = g.astype(np.float64) / m
+    if jitter:
+        inner = ((g > 0) & (g < m)).all(1)
+        pts[inner] += rng.uniform(-jitter, jitter, (int(inner.sum()), 3)) / m
+    idx = lambda i, j, k: (i * (m + 1) + j) * (m + 1) + k  # noqa: E731
+    c = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    tets = []
+    for perm in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        cur = c.copy()
+        verts = [idx(*cur.T)]
+        for ax in perm:
+            cur = cur.copy()
+            cur[:, ax] += 1
+            verts.append(idx(*cur.T))
+        tets.append(np.stack(verts, 1))
+    return pts, np.concatenate(tets).astype(np.int32)
+

@@ -13,6 +13,7 @@
  *  4. Batched voxelization of many meshes (device pointers).
  *  5. The PLC carve's point-in-domain classification (device and host pointers).
  *  6. Surface files (STL, OBJ) to PLCs and to device arrays.
+ *  7. The mesh quality report (device pointers).
  *
  * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
  * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
@@ -248,9 +249,9 @@ cmg_status cmg_voxelize_batch_counters(cmg_batch*, void* stream, uint64_t* out,
 
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
- * "batch_occupancy", "batch_sdf", "batch_sdf_backward", "carve_query"). Events are
- * recorded on the stream the kernel is launched on; launches recorded into a graph are
- * not timed. */
+ * "batch_occupancy", "batch_sdf", "batch_sdf_backward", "carve_query", "quality").
+ * Events are recorded on the stream the kernel is launched on; launches recorded into a
+ * graph are not timed. */
 void cmg_hip_kernel_timer(const char* name, int enable);
 
 /* Total milliseconds and launch count of the timed launches of `name` since the last
@@ -341,6 +342,32 @@ cmg_status cmg_hip_stl_decode(const void* bytes, size_t nbytes, double* xyz, int
  * idx the merged index of every vertex. Synchronises `stream`. */
 cmg_status cmg_hip_weld(const double* soup, size_t nverts, double* xyz, int* idx,
                         int64_t* npoints, void* stream, char* errbuf, size_t errbuf_len);
+
+/* ---- 7. mesh quality report -------------------------------------------------------- */
+
+/* cmg::quality::QualityReport (include/cmg/quality/metrics.hpp:14-40); the worst tets are
+ * returned separately (num_worst of them). */
+typedef struct cmg_quality_report {
+    uint64_t num_tets;
+    double min_radius_edge, max_radius_edge, mean_radius_edge;
+    double min_dihedral, max_dihedral, mean_dihedral; /* degrees */
+    double min_volume, max_volume, total_volume;
+    int dihedral_histogram[18]; /* [0,10), [10,20), ... degrees; the last bucket is >= 170 */
+    uint64_t num_worst;
+} cmg_quality_report;
+
+/* cmg::quality::report (src/quality/metrics.cpp:81-169), the accelerable QualityScan
+ * phase (include/cmg/backend/dispatch.hpp:25), over device arrays: xyz (npts points, 3
+ * doubles each) and tets (ntets tets, 4 ints each, 16-byte aligned). worst (host array,
+ * optional; room for worst_count entries, or ntets when worst_count < 0) receives the
+ * tets of largest radius-edge ratio, largest first, equal ratios by ascending index.
+ * Min, max, volumes, ratios and the worst list equal the reference's; the sums (total
+ * volume, means) are summed in a different order; dihedral angles use the device acos.
+ * A tet index outside [0, npts) fails with CMG_ERR_INVALID_INPUT. Synchronises
+ * `stream`. */
+cmg_status cmg_hip_quality_report(const double* xyz, size_t npts, const int* tets,
+                                  size_t ntets, int worst_count, cmg_quality_report* out,
+                                  int* worst, void* stream, char* errbuf, size_t errbuf_len);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -97,6 +97,12 @@ def _load() -> ctypes.CDLL:
     lib.orc_read_obj.argtypes = [ctypes.c_char_p, PP, ctypes.POINTER(i64), PP, PP,
                                  ctypes.POINTER(i64), ctypes.c_char_p, sz]
     lib.orc_free.argtypes = [P]
+    lib.orc_tet_volume.restype = d
+    lib.orc_tet_volume.argtypes = [P] * 4
+    lib.orc_tet_dihedrals.argtypes = [P] * 5
+    lib.orc_tet_quality.argtypes = [P, P, P, P, P, P]
+    lib.orc_quality_report.argtypes = [P, P, sz, i32, P, P]
+    lib.orc_dihedrals_batch.argtypes = [P, P, sz, P]
     return lib
 
 
@@ -503,3 +509,58 @@ def read_obj(path: str):
     off = _take(O_.value, nf.value + 1, np.int32)
     idx = _take(I_.value, int(off[-1]) if off.size else 0, np.int32)
     return pts, [idx[off[f]:off[f + 1]] for f in range(nf.value)]
+
+
+# ---- mesh quality report ----------------------------------------------------------------
+
+class _Quality(ctypes.Structure):
+    _fields_ = [("num_tets", ctypes.c_int64),
+                ("min_radius_edge", ctypes.c_double), ("max_radius_edge", ctypes.c_double),
+                ("mean_radius_edge", ctypes.c_double),
+                ("min_dihedral", ctypes.c_double), ("max_dihedral", ctypes.c_double),
+                ("mean_dihedral", ctypes.c_double),
+                ("min_volume", ctypes.c_double), ("max_volume", ctypes.c_double),
+                ("total_volume", ctypes.c_double),
+                ("dihedral_histogram", ctypes.c_int32 * 18), ("num_worst", ctypes.c_int64)]
+
+
+def tet_dihedrals(a, b, c, d) -> np.ndarray:
+    a, b, c, d = (_f64(x) for x in (a, b, c, d))
+    out = np.zeros(6)
+    lib().orc_tet_dihedrals(_p(a), _p(b), _p(c), _p(d), _p(out))
+    return out
+
+
+def tet_quality(pts, tet):
+    """(ratio, valid, volume, dihedrals[6]) of one tet."""
+    pts = _f64(pts).reshape(-1, 3)
+    tet = _i32(tet)
+    r, v, vol = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+    dih = np.zeros(6)
+    lib().orc_tet_quality(_p(pts), _p(tet), ctypes.byref(r), ctypes.byref(v),
+                          ctypes.byref(vol), _p(dih))
+    return r.value, bool(v.value), vol.value, dih
+
+
+def quality_report(pts, tets, worst_count=10) -> dict:
+    """quality::report (metrics.cpp:81-169) as a dict; 'worst_tets' descending ratio,
+    ties by ascending index."""
+    pts = _f64(pts).reshape(-1, 3)
+    tets = _i32(tets).reshape(-1, 4)
+    q = _Quality()
+    n = tets.shape[0]
+    worst = np.zeros(max(n, 1), dtype=np.int32)
+    lib().orc_quality_report(_p(pts), _p(tets), n, worst_count, ctypes.byref(q), _p(worst))
+    out = {f: getattr(q, f) for f, _ in _Quality._fields_ if f != "dihedral_histogram"}
+    out["dihedral_histogram"] = list(q.dihedral_histogram)
+    out["worst_tets"] = worst[: q.num_worst].tolist()
+    return out
+
+
+def dihedrals(pts, tets) -> np.ndarray:
+    """(T, 6) dihedral angles in degrees (metrics.cpp:52-77)."""
+    pts = _f64(pts).reshape(-1, 3)
+    tets = _i32(tets).reshape(-1, 4)
+    out = np.zeros((tets.shape[0], 6))
+    lib().orc_dihedrals_batch(_p(pts), _p(tets), tets.shape[0], _p(out))
+    return out

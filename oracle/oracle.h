@@ -200,6 +200,33 @@ int orc_read_obj(const char* path, double** pts, int64_t* npts, int32_t** off,
                  int32_t** idx, int64_t* nfacets, char* err, size_t len);
 void orc_free(void* p);
 
+/* ---------------- mesh quality report ------------------------------------------- */
+
+/* src/quality/metrics.hpp:14-40 (QualityReport), with the worst list's length */
+typedef struct {
+    int64_t num_tets;
+    double min_radius_edge, max_radius_edge, mean_radius_edge;
+    double min_dihedral, max_dihedral, mean_dihedral;
+    double min_volume, max_volume, total_volume;
+    int32_t dihedral_histogram[18];
+    int64_t num_worst;
+} orc_quality;
+
+/* metrics.cpp:36-48, 52-77 */
+double orc_tet_volume(const double* a, const double* b, const double* c, const double* d);
+void orc_tet_dihedrals(const double* P0, const double* P1, const double* P2,
+                       const double* P3, double* out);
+/* One tet's radius-edge ratio (DBL_MAX and *valid = 0 when degenerate, metrics.cpp:
+ * 109-131), volume and six dihedral angles. */
+void orc_tet_quality(const double* pts, const int* tet, double* ratio, int* valid,
+                     double* vol, double* dih);
+/* quality::report (metrics.cpp:81-169) over n tets (4 ints each); worst (optional)
+ * receives the num_worst indices, descending ratio, ties by ascending index. */
+void orc_quality_report(const double* pts, const int* tets, size_t n, int worst_count,
+                        orc_quality* r, int* worst);
+/* The six dihedral angles of each of n tets (6n doubles, metrics.cpp:52-77). */
+void orc_dihedrals_batch(const double* pts, const int* tets, size_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif
