@@ -15,8 +15,8 @@ steps. The JSON line also carries:
                 over the same batch on one host core
   extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, the SDF
                 workloads (forward+backward voxels/s, batched vs plain launches), the
-                PLC-carve classification (points/s and tets/s, both modes) and binary
-                STL input decoded and merged on the GPU
+                PLC-carve classification (points/s and tets/s, both modes), binary
+                STL input decoded and merged on the GPU, and the mesh quality report
 All other output goes to stderr; stdout holds exactly one JSON line.
 """
 from __future__ import annotations
@@ -369,6 +369,38 @@ def bench_carve(cm, W, steps, warmup, cpu=True):
     return res
 
 
+def bench_quality(cm, W, steps, warmup, cpu=True):
+    """Mesh quality report over a jittered Kuhn lattice of the unit cube: 128^3 cells, 6
+    tets each (12,582,912 tets, 2,146,689 points) resident in HBM; the report and its 10
+    worst tets per step."""
+    import torch
+    from cybermeshgenerator_amd import quality
+    pts_np, tets_np = W.lattice_tets(128, jitter=0.3, seed=8)
+    pts = torch.tensor(pts_np, device="cuda")
+    tets = torch.tensor(tets_np, device="cuda")
+    n = len(tets_np)
+    cm.kernel_timer("quality", True)
+    ms = time_steps(lambda: quality.report(pts, tets, 10), steps, warmup,
+                    torch.cuda.synchronize)
+    k_ms, k_n = cm.kernel_time("quality")
+    cm.kernel_timer("quality", False)
+    res = {"mesh": "Kuhn lattice 128^3 x 6, jitter 0.3", "tets": n, "ms_per_report": ms,
+           "tets_per_s": n / (ms / 1e3), "scan_kernel_ms": k_ms / max(k_n, 1)}
+    log(f"[bench] quality: {n / (ms / 1e3) / 1e9:.3f} G tets/s ({ms:.2f} ms per report, "
+        f"scan kernel {res['scan_kernel_ms']:.3f} ms)")
+    if cpu:
+        import oracle
+        m = 1 << 18
+        t0 = time.perf_counter()
+        c = oracle.quality_report(pts_np, tets_np[:m], 10)
+        dt = time.perf_counter() - t0
+        g = quality.report(pts, tets[:m], 10)
+        res["cpu_baseline"] = {"value": m / dt, "unit": "tets/s", "cores": 1, "kind": "port",
+                               "sample": "first 262,144 tets of the lattice",
+                               "same_worst": g["worst_tets"] == c["worst_tets"]}
+    return res
+
+
 def bench_stl(cm, W, steps, warmup, cpu=True):
     """Binary STL to device arrays: a 4,194,304-triangle torus file (200 MiB) resident in
     HBM, decoded and merged (coincident vertices, first-occurrence numbering) into the
@@ -426,9 +458,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
     ap.add_argument("--no-carve", action="store_true",
-                    help="skip the PLC-carve and STL-input workloads")
+                    help="skip the PLC-carve, STL-input and quality-report workloads")
     ap.add_argument("--carve-only", action="store_true",
-                    help="time only the PLC-carve classification and STL-input workloads")
+                    help="time only the PLC-carve, STL-input and quality-report workloads")
     ap.add_argument("--sdf-only", action="store_true",
                     help="time only the batched 64-mesh SDF step (no predicates, plain "
                          "path or single meshes) and print its figures: for profiling")
@@ -466,11 +498,13 @@ def main():
         return
 
     if args.carve_only:
-        print(json.dumps({"workload": "PLC carve classification and STL input",
+        print(json.dumps({"workload": "PLC carve, STL input and quality report",
                           "carve": bench_carve(cm, W, args.steps, args.warmup,
                                                cpu=not args.no_cpu),
                           "stl": bench_stl(cm, W, args.steps, args.warmup,
-                                           cpu=not args.no_cpu)}), flush=True)
+                                           cpu=not args.no_cpu),
+                          "quality": bench_quality(cm, W, args.steps, args.warmup,
+                                                   cpu=not args.no_cpu)}), flush=True)
         return
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
@@ -518,6 +552,12 @@ def main():
         except Exception as e:  # reported, never hides the predicate line
             log(f"[bench] stl workload failed: {e!r}")
             extra["stl"] = {"error": repr(e)}
+        try:
+            extra["quality"] = bench_quality(cm, W, args.steps, args.warmup,
+                                             not args.no_cpu)
+        except Exception as e:  # reported, never hides the predicate line
+            log(f"[bench] quality workload failed: {e!r}")
+            extra["quality"] = {"error": repr(e)}
         try:
             extra["carve"] = bench_carve(cm, W, args.steps, args.warmup, not args.no_cpu)
         except Exception as e:  # reported, never hides the predicate line
