@@ -19,6 +19,7 @@
 #include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
+#include "locate_internal.hpp"
 #include "quality_internal.hpp"
 #include "weld_internal.hpp"
 
@@ -729,6 +730,23 @@ cmg_status cmg_hip_quality_report(const double* xyz, size_t npts, const int* tet
         for (int b = 0; b < 18; ++b) out->dihedral_histogram[b] = r.dihedral_histogram[b];
         out->num_worst = (uint64_t)w.size();
         if (worst && !w.empty()) std::memcpy(worst, w.data(), sizeof(int) * w.size());
+    });
+}
+
+// ---- 8. point location -------------------------------------------------------------------
+
+cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const int* nbrs,
+                          size_t ntets, const double* q, size_t nq, const int* start,
+                          int start_tet, int* out, int* steps, uint64_t* brute,
+                          void* stream, char* errbuf, size_t len) {
+    if (brute) *brute = 0;
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        const cmg::hip::LocateStats s = cmg::hip::locate_points(
+            xyz, (long long)npts, tets, nbrs, (long long)ntets, q, (long long)nq, start,
+            start_tet, out, steps, as_stream(stream));
+        if (brute) *brute = (uint64_t)s.brute;
     });
 }
 

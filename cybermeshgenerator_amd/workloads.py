@@ -408,3 +408,36 @@ def lattice_tets(m: int, jitter: float = 0.0, seed: int = 7):
         tets.append(np.stack(verts, 1))
     return pts, np.concatenate(tets).astype(np.int32)
 
+
+def positive_tets(pts: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """tets reordered (v2 <-> v3 where needed) so that det[v1-v0, v2-v0, v3-v0] > 0, the
+    orientation the Delaunay insertion keeps internally (incremental.cpp:290-293)."""
+    p = pts[tets]
+    det = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0],
+                                                             p[:, 3] - p[:, 0]))
+    # orient3d(a,b,c,d) = det[a-d; b-d; c-d] has the opposite sign of det[b-a; c-a; d-a]
+    out = tets.copy()
+    flip = det > 0
+    out[flip, 2], out[flip, 3] = tets[flip, 3], tets[flip, 2]
+    return out
+
+
+def tet_neighbors(tets: np.ndarray) -> np.ndarray:
+    """nbrs[t, k] = the tet sharing the face opposite vertex k of tet t, or -1."""
+    n = len(tets)
+    faces = []
+    for k in range(4):
+        f = np.sort(np.delete(tets, k, axis=1), axis=1)
+        faces.append(f)
+    F = np.concatenate(faces)                      # (4n, 3)
+    owner = np.tile(np.arange(n), 4)
+    slot = np.repeat(np.arange(4), n)
+    order = np.lexsort((F[:, 2], F[:, 1], F[:, 0]))
+    Fs = F[order]
+    same = (Fs[1:] == Fs[:-1]).all(1)
+    nbrs = np.full((n, 4), -1, dtype=np.int32)
+    a, b = order[:-1][same], order[1:][same]
+    nbrs[owner[a], slot[a]] = owner[b]
+    nbrs[owner[b], slot[b]] = owner[a]
+    return nbrs
+

@@ -14,6 +14,7 @@
  *  5. The PLC carve's point-in-domain classification (device and host pointers).
  *  6. Surface files (STL, OBJ) to PLCs and to device arrays.
  *  7. The mesh quality report (device pointers).
+ *  8. Batched point location in a tetrahedral mesh (device pointers).
  *
  * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
  * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
@@ -249,7 +250,8 @@ cmg_status cmg_voxelize_batch_counters(cmg_batch*, void* stream, uint64_t* out,
 
 /* Profiling hook (no counterpart in the reference): enable or disable HIP-event timing
  * of the launches of one kernel, by name ("orient3d_filter", "insphere_filter",
- * "batch_occupancy", "batch_sdf", "batch_sdf_backward", "carve_query", "quality").
+ * "batch_occupancy", "batch_sdf", "batch_sdf_backward", "carve_query", "quality",
+ * "locate").
  * Events are recorded on the stream the kernel is launched on; launches recorded into a
  * graph are not timed. */
 void cmg_hip_kernel_timer(const char* name, int enable);
@@ -368,6 +370,26 @@ typedef struct cmg_quality_report {
 cmg_status cmg_hip_quality_report(const double* xyz, size_t npts, const int* tets,
                                   size_t ntets, int worst_count, cmg_quality_report* out,
                                   int* worst, void* stream, char* errbuf, size_t errbuf_len);
+
+/* ---- 8. point location ------------------------------------------------------------- */
+
+/* Batched form of the Delaunay insertion's point location (Triangulation::locate,
+ * src/delaunay/incremental.cpp:321-358; the PointLocation phase the reference marks as
+ * accelerable at :424-427): one visibility walk per query point q[3i..3i+2] through a
+ * fixed mesh of ntets tets (device arrays): tets (4 ints each, positively oriented,
+ * orient3d(v0,v1,v2,v3) > 0) and nbrs (4 ints each: the tet across the face opposite
+ * v_k, -1 on the hull), both 16-byte aligned. The walk starts at start[i] (device array)
+ * or, when start is NULL, at start_tet; it tests the faces in the reference's order with
+ * the compat orient3d and steps through the first negative one that has a neighbour;
+ * after 2 * ntets + 16 steps it falls back to the reference's brute_locate. out[i]
+ * (device) = the tet found, identical to the reference's walk from the same start;
+ * steps[i] (device, optional) = the walk length, -1 where brute force answered;
+ * *brute (optional) = how many walks did. Synchronises `stream`; a walk that meets an
+ * index out of range fails the call with CMG_ERR_INVALID_INPUT. */
+cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const int* nbrs,
+                          size_t ntets, const double* q, size_t nq, const int* start,
+                          int start_tet, int* out, int* steps, uint64_t* brute,
+                          void* stream, char* errbuf, size_t errbuf_len);
 
 #ifdef __cplusplus
 } /* extern "C" */

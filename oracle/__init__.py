@@ -103,6 +103,9 @@ def _load() -> ctypes.CDLL:
     lib.orc_tet_quality.argtypes = [P, P, P, P, P, P]
     lib.orc_quality_report.argtypes = [P, P, sz, i32, P, P]
     lib.orc_dihedrals_batch.argtypes = [P, P, sz, P]
+    lib.orc_locate.restype = i32
+    lib.orc_locate.argtypes = [P, P, P, sz, P, i32, P]
+    lib.orc_locate_batch.argtypes = [P, P, P, sz, P, sz, P, i32, P, P, i32]
     return lib
 
 
@@ -564,3 +567,24 @@ def dihedrals(pts, tets) -> np.ndarray:
     out = np.zeros((tets.shape[0], 6))
     lib().orc_dihedrals_batch(_p(pts), _p(tets), tets.shape[0], _p(out))
     return out
+
+
+# ---- point location ------------------------------------------------------------------
+
+def locate(pts, tets, nbrs, q, start=0, nthreads=0):
+    """(tet, steps) per query: incremental.cpp:321-358's walk from start (an int or one
+    start tet per query); steps -1 where brute force answered."""
+    pts = _f64(pts).reshape(-1, 3)
+    tets = _i32(tets).reshape(-1, 4)
+    nbrs = _i32(nbrs).reshape(-1, 4)
+    q = _f64(q).reshape(-1, 3)
+    out = np.zeros(q.shape[0], dtype=np.int32)
+    steps = np.zeros(q.shape[0], dtype=np.int32)
+    if np.isscalar(start):
+        lib().orc_locate_batch(_p(pts), _p(tets), _p(nbrs), tets.shape[0], _p(q), q.shape[0],
+                               None, int(start), _p(out), _p(steps), nthreads)
+    else:
+        st = _i32(start)
+        lib().orc_locate_batch(_p(pts), _p(tets), _p(nbrs), tets.shape[0], _p(q), q.shape[0],
+                               _p(st), 0, _p(out), _p(steps), nthreads)
+    return out, steps

@@ -227,6 +227,20 @@ void orc_quality_report(const double* pts, const int* tets, size_t n, int worst_
 /* The six dihedral angles of each of n tets (6n doubles, metrics.cpp:52-77). */
 void orc_dihedrals_batch(const double* pts, const int* tets, size_t n, double* out);
 
+/* ---------------- point location ------------------------------------------------- */
+
+/* Triangulation::locate (src/delaunay/incremental.cpp:321-358) in a fixed mesh: tets
+ * (4 ints each, orient3d(v0,v1,v2,v3) > 0), nbrs (4 ints each: the tet across the face
+ * opposite v_k, or -1), walking from tet `start` with the compat orient3d; after
+ * 2 * ntets + 16 steps, brute_locate. *steps (optional): the walk length, -1 when brute
+ * force answered. */
+int orc_locate(const double* pts, const int* tets, const int* nbrs, size_t ntets,
+               const double* q, int start, int* steps);
+/* One walk per query (start[i], or start_tet when start is NULL). */
+void orc_locate_batch(const double* pts, const int* tets, const int* nbrs, size_t ntets,
+                      const double* q, size_t nq, const int* start, int start_tet,
+                      int* out, int* steps, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

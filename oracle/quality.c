@@ -7,9 +7,13 @@
  * histogram, sequential sums in tet order) operation for operation. The worst-tets list
  * is ordered by descending ratio with ties by ascending index (the reference's std::sort
  * leaves the order of equal ratios unspecified).
+ *
+ * Also the point-location walk of src/delaunay/incremental.cpp:321-358 over a fixed
+ * tetrahedral mesh with neighbours.
  */
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -214,4 +218,58 @@ void orc_dihedrals_batch(const double* pts, const int* tets, size_t n, double* o
         orc_tet_dihedrals(pts + 3 * (size_t)q[0], pts + 3 * (size_t)q[1],
                           pts + 3 * (size_t)q[2], pts + 3 * (size_t)q[3], out + 6 * t);
     }
+}
+
+/* ---- point location (src/delaunay/incremental.cpp:321-358) ------------------------- */
+
+static double o3(const double* a, const double* b, const double* c, const double* d) {
+    return (double)orc_orient3d_sign(a, b, c, d, ORC_COMPAT);
+}
+
+/* brute_locate (:345-358): the first tet with all four orientations >= 0, else 0 */
+static int locate_brute(const double* pts, const int* tets, size_t ntets, const double* q) {
+    for (size_t i = 0; i < ntets; ++i) {
+        const int* v = tets + 4 * i;
+        const double* P0 = pts + 3 * (size_t)v[0];
+        const double* P1 = pts + 3 * (size_t)v[1];
+        const double* P2 = pts + 3 * (size_t)v[2];
+        const double* P3 = pts + 3 * (size_t)v[3];
+        if (o3(q, P1, P2, P3) >= 0 && o3(P0, q, P2, P3) >= 0 && o3(P0, P1, q, P3) >= 0 &&
+            o3(P0, P1, P2, q) >= 0)
+            return (int)i;
+    }
+    return 0;
+}
+
+/* locate (:321-337) from tet `start`; *steps (optional) receives the walk length, -1
+ * when the walk gave up and brute_locate answered */
+int orc_locate(const double* pts, const int* tets, const int* nbrs, size_t ntets,
+               const double* q, int start, int* steps) {
+    int t = start;
+    const long long limit = 2 * (long long)ntets + 16;
+    for (long long s = 0; s < limit; ++s) {
+        const int* v = tets + 4 * (size_t)t;
+        const int* n = nbrs + 4 * (size_t)t;
+        const double* P0 = pts + 3 * (size_t)v[0];
+        const double* P1 = pts + 3 * (size_t)v[1];
+        const double* P2 = pts + 3 * (size_t)v[2];
+        const double* P3 = pts + 3 * (size_t)v[3];
+        if (o3(q, P1, P2, P3) < 0 && n[0] >= 0) { t = n[0]; continue; }
+        if (o3(P0, q, P2, P3) < 0 && n[1] >= 0) { t = n[1]; continue; }
+        if (o3(P0, P1, q, P3) < 0 && n[2] >= 0) { t = n[2]; continue; }
+        if (o3(P0, P1, P2, q) < 0 && n[3] >= 0) { t = n[3]; continue; }
+        if (steps) *steps = (int)s;
+        return t;
+    }
+    if (steps) *steps = -1;
+    return locate_brute(pts, tets, ntets, q);
+}
+
+void orc_locate_batch(const double* pts, const int* tets, const int* nbrs, size_t ntets,
+                      const double* q, size_t nq, const int* start, int start_tet,
+                      int* out, int* steps, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (size_t i = 0; i < nq; ++i)
+        out[i] = orc_locate(pts, tets, nbrs, ntets, q + 3 * i, start ? start[i] : start_tet,
+                            steps ? steps + i : 0);
 }

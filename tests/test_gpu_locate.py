@@ -1,0 +1,48 @@
+"""Parity of the batched GPU point location (cmg_hip_locate) with the CPU oracle of
+src/delaunay/incremental.cpp:321-358: same tet and same walk length for every query,
+from a common start and from per-query starts, inside and outside the hull, and through
+the step-limit fallback."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.test_oracle_locate import cycling_mesh, lattice_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(pts, tets, nbrs, q, start):
+    from cybermeshgenerator_amd.locate import locate
+    out, steps, brute = locate(pts, tets, nbrs, q, start)
+    return out.cpu().numpy(), steps.cpu().numpy(), brute
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.3])
+def test_walks_match_the_oracle(gpu, jitter):
+    pts, tets, nbrs = lattice_mesh(24, jitter=jitter, seed=3)
+    rng = np.random.default_rng(4)
+    q = np.concatenate([rng.uniform(0, 1, (30000, 3)),
+                        rng.uniform(-0.5, 1.5, (2000, 3)),          # outside the hull
+                        np.round(rng.uniform(0, 1, (2000, 3)) * 48) / 48])  # on faces
+    print(f"locate: {len(tets)} tets, {len(q)} queries, jitter {jitter}", flush=True)
+    for start in (0, rng.integers(0, len(tets), len(q)).astype(np.int32)):
+        got, steps, brute = _run(pts, tets, nbrs, q, start)
+        want, wsteps = oracle.locate(pts, tets, nbrs, q, start)
+        assert np.array_equal(got, want)
+        assert np.array_equal(steps, wsteps)
+        assert brute == int((wsteps < 0).sum())
+
+
+def test_step_limit_fallback_and_errors(gpu):
+    pts, tets, nbrs, q = cycling_mesh()
+    got, steps, brute = _run(pts, tets, nbrs, q, 0)
+    assert got[0] == 0 and steps[0] == -1 and brute == 1
+    pts, tets, nbrs = lattice_mesh(4)
+    empty, _, _ = _run(pts, tets, nbrs, np.zeros((0, 3)), 0)
+    assert empty.size == 0
+    bad = nbrs.copy()
+    bad[bad >= 0] = len(tets) + 5            # every neighbour out of range
+    with pytest.raises(RuntimeError, match="out of range"):
+        _run(pts, tets, bad, np.array([[0.9, 0.9, 0.9]]), 0)
+    with pytest.raises(RuntimeError, match="start tet out of range"):
+        _run(pts, tets, nbrs, np.array([[0.5, 0.5, 0.5]]), len(tets))
