@@ -16,7 +16,8 @@ steps. The JSON line also carries:
   extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, the SDF
                 workloads (forward+backward voxels/s, batched vs plain launches), the
                 PLC-carve classification (points/s and tets/s, both modes), binary
-                STL input decoded and merged on the GPU, and the mesh quality report
+                STL input decoded and merged on the GPU, the mesh quality report and
+                batched point location
 All other output goes to stderr; stdout holds exactly one JSON line.
 """
 from __future__ import annotations
@@ -401,6 +402,45 @@ def bench_quality(cm, W, steps, warmup, cpu=True):
     return res
 
 
+def bench_locate(cm, W, steps, warmup, cpu=True):
+    """Batched point location: 2^20 points uniform in the unit cube located in a jittered
+    Kuhn lattice of it (64^3 cells x 6 = 1,572,864 tets, jitter 0.1 cell), every walk
+    starting at tet 0 (long walks: ~170 steps of up to four orientation tests)."""
+    import torch
+    from cybermeshgenerator_amd.locate import locate
+    pts_np, tets_np = W.lattice_tets(64, jitter=0.1, seed=9)
+    tets_np = W.positive_tets(pts_np, tets_np)
+    nbrs_np = W.tet_neighbors(tets_np)
+    n = 1 << 20
+    q_np = np.random.default_rng(10).uniform(0, 1, (n, 3))
+    pts, tets, nbrs, q = (torch.tensor(a, device="cuda") for a in (pts_np, tets_np, nbrs_np,
+                                                                 q_np))
+    last = {}
+
+    def step():
+        last["r"] = locate(pts, tets, nbrs, q, 0)
+
+    ms = time_steps(step, steps, warmup, torch.cuda.synchronize)
+    walk = last["r"][1].float()
+    res = {"mesh": "Kuhn lattice 64^3 x 6, jitter 0.1", "tets": len(tets_np), "queries": n,
+           "start": "tet 0 for every query", "ms_per_batch": ms,
+           "queries_per_s": n / (ms / 1e3), "mean_walk_steps": float(walk.mean().item()),
+           "brute_force_walks": last["r"][2]}
+    log(f"[bench] locate: {n / (ms / 1e3) / 1e6:.1f} M queries/s ({ms:.2f} ms per 2^20, "
+        f"{res['mean_walk_steps']:.0f} steps per walk)")
+    if cpu:
+        import oracle
+        m = 1 << 15
+        t0 = time.perf_counter()
+        want, _ = oracle.locate(pts_np, tets_np, nbrs_np, q_np[:m], 0, nthreads=1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "queries/s", "cores": 1,
+                               "kind": "port", "sample": "first 32,768 queries",
+                               "identical": bool(np.array_equal(
+                                   last["r"][0][:m].cpu().numpy(), want))}
+    return res
+
+
 def bench_stl(cm, W, steps, warmup, cpu=True):
     """Binary STL to device arrays: a 4,194,304-triangle torus file (200 MiB) resident in
     HBM, decoded and merged (coincident vertices, first-occurrence numbering) into the
@@ -458,9 +498,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sdf", action="store_true")
     ap.add_argument("--no-carve", action="store_true",
-                    help="skip the PLC-carve, STL-input and quality-report workloads")
+                    help="skip the carve, STL, quality and locate workloads")
     ap.add_argument("--carve-only", action="store_true",
-                    help="time only the PLC-carve, STL-input and quality-report workloads")
+                    help="time only the carve, STL, quality and locate workloads")
     ap.add_argument("--sdf-only", action="store_true",
                     help="time only the batched 64-mesh SDF step (no predicates, plain "
                          "path or single meshes) and print its figures: for profiling")
@@ -498,13 +538,15 @@ def main():
         return
 
     if args.carve_only:
-        print(json.dumps({"workload": "PLC carve, STL input and quality report",
+        print(json.dumps({"workload": "PLC carve, STL input, quality report, point location",
                           "carve": bench_carve(cm, W, args.steps, args.warmup,
                                                cpu=not args.no_cpu),
                           "stl": bench_stl(cm, W, args.steps, args.warmup,
                                            cpu=not args.no_cpu),
                           "quality": bench_quality(cm, W, args.steps, args.warmup,
-                                                   cpu=not args.no_cpu)}), flush=True)
+                                                   cpu=not args.no_cpu),
+                          "locate": bench_locate(cm, W, args.steps, args.warmup,
+                                                 cpu=not args.no_cpu)}), flush=True)
         return
 
     pr = bench_predicates(cm, W, args.steps, args.warmup)
@@ -558,6 +600,11 @@ def main():
         except Exception as e:  # reported, never hides the predicate line
             log(f"[bench] quality workload failed: {e!r}")
             extra["quality"] = {"error": repr(e)}
+        try:
+            extra["locate"] = bench_locate(cm, W, args.steps, args.warmup, not args.no_cpu)
+        except Exception as e:  # reported, never hides the predicate line
+            log(f"[bench] locate workload failed: {e!r}")
+            extra["locate"] = {"error": repr(e)}
         try:
             extra["carve"] = bench_carve(cm, W, args.steps, args.warmup, not args.no_cpu)
         except Exception as e:  # reported, never hides the predicate line
