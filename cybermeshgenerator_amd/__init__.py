@@ -44,7 +44,19 @@ class PredStats(ctypes.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+def _init_torch_runtime_first() -> None:
+    """torch ships its own HIP and HSA runtime libraries; this library links the system
+    ROCm ones, so a process that uses both holds two runtimes. torch's finds no device
+    when it starts after the other, so it is started first whenever torch is present."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def _load() -> ctypes.CDLL:
+    _init_torch_runtime_first()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m cybermeshgenerator_amd.build`"
