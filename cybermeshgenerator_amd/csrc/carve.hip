@@ -36,6 +36,8 @@
 #include <cmath>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
 #include "device_common.hpp"
@@ -273,37 +275,88 @@ __device__ __forceinline__ bool query_point(const double* q, const int* tets,
     }
 }
 
-template <bool TETS>
-__global__ __launch_bounds__(256) void rays_query_kernel(const double* q, const int* tets,
+// Query set-up: the query point of item i (stored for tets), and one (cell, i) pair per
+// grid: key = global cell id, ncell for a tet with an index out of range (answered 0,
+// counted, sorted after every valid key). Sorting the pairs by key orders the queries by
+// cell, grid by grid.
+template <bool TETS, bool RAYS>
+__global__ __launch_bounds__(256) void query_prep_kernel(const double* q, const int* tets,
                                                          const double* xyz, long long npts,
-                                                         long long n, RayParams P,
-                                                         const unsigned* start,
-                                                         const int* items, const double* tv,
-                                                         uint8_t* out,
+                                                         long long n, RayParams P, GridSpec g,
+                                                         unsigned ncell, double* qbuf,
+                                                         unsigned* key, unsigned* val,
+                                                         uint8_t* out, unsigned* nbad,
                                                          unsigned long long* bad) {
-    const int r = P.r;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         double x, y, z;
-        if (!query_point<TETS>(q, tets, xyz, npts, i, x, y, z)) {
+        const bool ok = query_point<TETS>(q, tets, xyz, npts, i, x, y, z);
+        if (TETS) {
+            qbuf[3 * i] = x, qbuf[3 * i + 1] = y, qbuf[3 * i + 2] = z;
+        }
+        if (!ok) {
             out[i] = 0;
             atomicAdd(bad, 1ull);
-            continue;
+            atomicAdd(nbad, 1u);
         }
-        // tetrahedralize_plc.cpp:79-93
-        int votes = 0;
+        if constexpr (RAYS) {
+            const int r = P.r;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const RayDir& G = P.k[k];
-            const int cell = k * r * r + cell_u(G, r, x, y, z) * r + cell_v(G, r, x, y, z);
-            const unsigned e1 = start[cell + 1];
-            bool odd = false;
-            for (unsigned e = start[cell]; e < e1; ++e)
-                odd ^= ray_hits(x, y, z, G.d, tv + 9 * (long long)items[e]);
-            votes += odd;
+            for (int k = 0; k < 3; ++k) {
+                key[k * n + i] = ok ? (unsigned)(k * r * r + cell_u(P.k[k], r, x, y, z) * r +
+                                                 cell_v(P.k[k], r, x, y, z))
+                                    : ncell;
+                val[k * n + i] = (unsigned)i;
+            }
+        } else {
+            unsigned c = ncell;
+            if (ok) {
+                const int ci = occdev::clamp_cell(x, g.corner[0], g.spacing, g.nx);
+                const int cj = occdev::clamp_cell(y, g.corner[1], g.spacing, g.ny);
+                c = (unsigned)((long long)cj * g.nx + ci);
+            }
+            key[i] = c;
+            val[i] = (unsigned)i;
         }
-        out[i] = (uint8_t)(votes >= 2);
     }
+}
+
+// One ray direction (tetrahedralize_plc.cpp:82-90) for the queries of grid K, taken in
+// cell order so that the lanes of a wavefront mostly share one cell list (uniform trip
+// counts, one triangle record per step for the whole wavefront). odd[K n + i] = parity
+// of the crossings of query i.
+template <int K>
+__global__ __launch_bounds__(256) void rays_pass_kernel(const double* qpts, long long n,
+                                                        RayParams P, const unsigned* nbad,
+                                                        const unsigned* skey,
+                                                        const unsigned* sval,
+                                                        const unsigned* start,
+                                                        const int* items, const double* tv,
+                                                        uint8_t* odd) {
+    const long long m = n - *nbad;  // valid queries: each has one pair per grid
+    const long long base = K * m;
+    const RayDir& G = P.k[K];
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m;
+         j += (long long)gridDim.x * blockDim.x) {
+        const unsigned i = sval[base + j];
+        const unsigned c = skey[base + j];
+        const double x = qpts[3 * (long long)i], y = qpts[3 * (long long)i + 1],
+                     z = qpts[3 * (long long)i + 2];
+        const unsigned e1 = start[c + 1];
+        bool o = false;
+        for (unsigned e = start[c]; e < e1; ++e)
+            o ^= ray_hits(x, y, z, G.d, tv + 9 * (long long)items[e]);
+        odd[K * n + i] = o;
+    }
+}
+
+// tetrahedralize_plc.cpp:91-92: inside iff at least two of the three parities are odd
+__global__ __launch_bounds__(256) void rays_vote_kernel(const unsigned* key, unsigned ncell,
+                                                        long long n, const uint8_t* odd,
+                                                        uint8_t* out) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        if (key[i] != ncell) out[i] = (uint8_t)(odd[i] + odd[n + i] + odd[2 * n + i] >= 2);
 }
 
 // One attempt of voxelize.cpp:121-143 for the line through (x, y): false if a sign is
@@ -363,28 +416,25 @@ __device__ int parity_partial(double x, double y, double zmin, double zmax, doub
     return above;
 }
 
-template <bool TETS>
-__global__ __launch_bounds__(256) void parity_query_kernel(const double* q, const int* tets,
-                                                           const double* xyz, long long npts,
-                                                           long long n, GridSpec g,
-                                                           const unsigned* start,
-                                                           const int* items,
-                                                           const double* tv, uint8_t* out,
-                                                           unsigned long long* bad) {
+// The column rule (voxelize.cpp:148-195) for the queries in column order: the lanes of a
+// wavefront mostly share one bucket.
+__global__ __launch_bounds__(256) void parity_pass_kernel(const double* qpts, long long n,
+                                                          GridSpec g, const unsigned* nbad,
+                                                          const unsigned* skey,
+                                                          const unsigned* sval,
+                                                          const unsigned* start,
+                                                          const int* items, const double* tv,
+                                                          uint8_t* out) {
     // voxelize.cpp:176-177
     const double zmin = g.corner[2] - g.spacing;
     const double zmax = g.corner[2] + (g.nz + 1) * g.spacing;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        double x, y, z;
-        if (!query_point<TETS>(q, tets, xyz, npts, i, x, y, z)) {
-            out[i] = 0;
-            atomicAdd(bad, 1ull);
-            continue;
-        }
-        const int ci = occdev::clamp_cell(x, g.corner[0], g.spacing, g.nx);
-        const int cj = occdev::clamp_cell(y, g.corner[1], g.spacing, g.ny);
-        const long long col = (long long)cj * g.nx + ci;
+    const long long m = n - *nbad;
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m;
+         j += (long long)gridDim.x * blockDim.x) {
+        const unsigned i = sval[j];
+        const double x = qpts[3 * (long long)i], y = qpts[3 * (long long)i + 1],
+                     z = qpts[3 * (long long)i + 2];
+        const unsigned col = skey[j];
         const unsigned e0 = start[col], e1 = start[col + 1];
         int above = 0, gmin = INT_MAX;
         // voxelize.cpp:154-165
@@ -567,41 +617,97 @@ int CarveEngine::cells_per_axis() const {
 }
 long long CarveEngine::list_entries() const { return (long long)p_->entries; }
 
+// Orders the queries by cell (a radix sort of (cell, query) pairs) and runs the pass
+// kernels. Work buffers come from the stream-ordered allocator: calls stay
+// asynchronous.
+template <bool TETS>
+void classify(const CarveEngine::Impl& I, const double* d_q, const int* d_tets, long long n,
+              uint8_t* d_out, hipStream_t st);
+
 void CarveEngine::points(const double* d_q, long long nq, uint8_t* d_inside,
                          hipStream_t st) {
-    const Impl& I = *p_;
     if (nq < 0) throw InvalidInput("carve: negative count");
     if (nq == 0) return;
     if (!d_q || !d_inside) throw InvalidInput("carve: null pointer");
-    const unsigned nb = grid_for((size_t)nq, 256, 8);
-    TimedLaunch tl("carve_query", st);
-    if (I.mode == kCarveRays)
-        rays_query_kernel<false><<<nb, 256, 0, st>>>(d_q, nullptr, I.xyz, I.npts, nq, I.P,
-                                                     I.start, I.items, I.tv, d_inside,
-                                                     I.bad);
-    else
-        parity_query_kernel<false><<<nb, 256, 0, st>>>(d_q, nullptr, I.xyz, I.npts, nq, I.g,
-                                                       I.start, I.items, I.tv, d_inside,
-                                                       I.bad);
-    CMG_HIP_LAUNCH();
+    classify<false>(*p_, d_q, nullptr, nq, d_inside, st);
 }
 
 void CarveEngine::tets(const int* d_tets, long long nt, uint8_t* d_keep, hipStream_t st) {
-    const Impl& I = *p_;
     if (nt < 0) throw InvalidInput("carve: negative count");
     if (nt == 0) return;
     if (!d_tets || !d_keep) throw InvalidInput("carve: null pointer");
     if (reinterpret_cast<uintptr_t>(d_tets) % 16)
         throw InvalidInput("carve: tets must be 16-byte aligned");
-    const unsigned nb = grid_for((size_t)nt, 256, 8);
-    TimedLaunch tl("carve_query", st);
-    if (I.mode == kCarveRays)
-        rays_query_kernel<true><<<nb, 256, 0, st>>>(nullptr, d_tets, I.xyz, I.npts, nt, I.P,
-                                                    I.start, I.items, I.tv, d_keep, I.bad);
-    else
-        parity_query_kernel<true><<<nb, 256, 0, st>>>(nullptr, d_tets, I.xyz, I.npts, nt, I.g,
-                                                      I.start, I.items, I.tv, d_keep, I.bad);
-    CMG_HIP_LAUNCH();
+    classify<true>(*p_, nullptr, d_tets, nt, d_keep, st);
+}
+
+template <bool TETS>
+void classify(const CarveEngine::Impl& I, const double* d_q, const int* d_tets, long long n,
+              uint8_t* d_out, hipStream_t st) {
+    if (n >= (1LL << 30)) throw InvalidInput("carve: more than 2^30 queries per call");
+    const bool rays = I.mode == kCarveRays;
+    const int grids = rays ? 3 : 1;
+    const unsigned ncell = (unsigned)I.ncell;
+    const long long pairs = grids * n;
+    int bits = 1;
+    while ((1ull << bits) <= ncell) ++bits;  // keys 0..ncell
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t sort_tmp = 0;
+    CMG_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp, (const unsigned*)nullptr,
+                                      (unsigned*)nullptr, (const unsigned*)nullptr,
+                                      (unsigned*)nullptr, (size_t)pairs, 0, bits, st));
+    const size_t b_q = TETS ? al(sizeof(double) * 3 * n) : 0;
+    const size_t b_pair = al(sizeof(unsigned) * pairs);
+    const size_t b_odd = rays ? al((size_t)3 * n) : 0;
+    const size_t total = b_q + 4 * b_pair + b_odd + al(8) + al(sort_tmp);
+    char* base = nullptr;
+    CMG_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), total, st));
+    char* cur = base;
+    auto take = [&](size_t b) {
+        char* p = cur;
+        cur += b;
+        return p;
+    };
+    double* qbuf = TETS ? reinterpret_cast<double*>(take(b_q)) : nullptr;
+    unsigned* key = reinterpret_cast<unsigned*>(take(b_pair));
+    unsigned* val = reinterpret_cast<unsigned*>(take(b_pair));
+    unsigned* skey = reinterpret_cast<unsigned*>(take(b_pair));
+    unsigned* sval = reinterpret_cast<unsigned*>(take(b_pair));
+    uint8_t* odd = rays ? reinterpret_cast<uint8_t*>(take(b_odd)) : nullptr;
+    unsigned* nbad = reinterpret_cast<unsigned*>(take(al(8)));
+    void* tmp = take(al(sort_tmp));
+    const double* qpts = TETS ? qbuf : d_q;
+    const unsigned nb = grid_for((size_t)n, 256, 8);
+    {
+        TimedLaunch tl("carve_query", st);
+        CMG_HIP(hipMemsetAsync(nbad, 0, sizeof(unsigned), st));
+        if (rays)
+            query_prep_kernel<TETS, true><<<nb, 256, 0, st>>>(d_q, d_tets, I.xyz, I.npts, n,
+                                                              I.P, I.g, ncell, qbuf, key, val,
+                                                              d_out, nbad, I.bad);
+        else
+            query_prep_kernel<TETS, false><<<nb, 256, 0, st>>>(d_q, d_tets, I.xyz, I.npts, n,
+                                                               I.P, I.g, ncell, qbuf, key, val,
+                                                               d_out, nbad, I.bad);
+        CMG_HIP_LAUNCH();
+        CMG_HIP(rocprim::radix_sort_pairs(tmp, sort_tmp, key, skey, val, sval, (size_t)pairs,
+                                          0, bits, st));
+        if (rays) {
+            rays_pass_kernel<0><<<nb, 256, 0, st>>>(qpts, n, I.P, nbad, skey, sval, I.start,
+                                                    I.items, I.tv, odd);
+            rays_pass_kernel<1><<<nb, 256, 0, st>>>(qpts, n, I.P, nbad, skey, sval, I.start,
+                                                    I.items, I.tv, odd);
+            rays_pass_kernel<2><<<nb, 256, 0, st>>>(qpts, n, I.P, nbad, skey, sval, I.start,
+                                                    I.items, I.tv, odd);
+            CMG_HIP_LAUNCH();
+            rays_vote_kernel<<<nb, 256, 0, st>>>(key, ncell, n, odd, d_out);
+        } else {
+            parity_pass_kernel<<<nb, 256, 0, st>>>(qpts, n, I.g, nbad, skey, sval, I.start,
+                                                   I.items, I.tv, d_out);
+        }
+        CMG_HIP_LAUNCH();
+    }
+    CMG_HIP(hipFreeAsync(base, st));
 }
 
 unsigned long long CarveEngine::bad_tets(hipStream_t st) {

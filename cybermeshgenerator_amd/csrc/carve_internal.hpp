@@ -41,8 +41,9 @@ class CarveEngine {
     int cells_per_axis() const;      // rays: r; parity: nx of the grid
     long long list_entries() const;  // total entries of the cell lists
 
+    struct Impl;  // defined in carve.hip
+
   private:
-    struct Impl;
     std::unique_ptr<Impl> p_;
 };
 
