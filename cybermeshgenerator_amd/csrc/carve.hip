@@ -342,10 +342,30 @@ __global__ __launch_bounds__(256) void rays_pass_kernel(const double* qpts, long
         const unsigned c = skey[base + j];
         const double x = qpts[3 * (long long)i], y = qpts[3 * (long long)i + 1],
                      z = qpts[3 * (long long)i + 2];
-        const unsigned e1 = start[c + 1];
+        const unsigned e0 = start[c], e1 = start[c + 1];
         bool o = false;
-        for (unsigned e = start[c]; e < e1; ++e)
-            o ^= ray_hits(x, y, z, G.d, tv + 9 * (long long)items[e]);
+        if (e0 < e1) {
+            // software pipeline: the next record and the index after it are loaded while
+            // the current test computes
+            double cur[9], nxt[9];
+            int ib = e0 + 1 < e1 ? items[e0 + 1] : 0;
+            const double* T0 = tv + 9 * (long long)items[e0];
+#pragma unroll
+            for (int a = 0; a < 9; ++a) cur[a] = T0[a];
+            for (unsigned e = e0; e < e1; ++e) {
+                const bool more = e + 1 < e1;
+                const int ic = e + 2 < e1 ? items[e + 2] : 0;
+                if (more) {
+                    const double* Tn = tv + 9 * (long long)ib;
+#pragma unroll
+                    for (int a = 0; a < 9; ++a) nxt[a] = Tn[a];
+                }
+                o ^= ray_hits(x, y, z, G.d, cur);
+#pragma unroll
+                for (int a = 0; a < 9; ++a) cur[a] = nxt[a];
+                ib = ic;
+            }
+        }
         odd[K * n + i] = o;
     }
 }
