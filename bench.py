@@ -335,6 +335,7 @@ def bench_carve(cm, W, steps, warmup, cpu=True):
     out = torch.empty(n, dtype=torch.uint8, device="cuda")
     res = {"mesh": "C3 torus, 65,536 triangles", "queries": n}
     for mode in ("rays", "parity"):
+        CarveGrid(pts, tris, mode=mode)  # the first launches of the process load code
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         cg = CarveGrid(pts, tris, mode=mode)
