@@ -37,6 +37,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "carve_internal.hpp"
 #include "cmg_hip_internal.hpp"
@@ -202,33 +203,6 @@ __global__ __launch_bounds__(256) void rect_kernel(const double* xyz, const int*
                 for (int b = R.b0; b <= R.b1; ++b)
                     atomicAdd(&count[M.base + a * M.sa + b * M.sb], 1u);
         }
-    }
-}
-
-// Exclusive scan of n counts into start[0..n] (one workgroup); *total receives the sum.
-__global__ __launch_bounds__(1024) void scan_kernel(const unsigned* count, unsigned* start,
-                                                    long long n, unsigned long long* total) {
-    __shared__ unsigned long long part[1024];
-    const long long per = (n + 1023) / 1024;
-    const long long lo = threadIdx.x * per, hi = lo + per < n ? lo + per : n;
-    unsigned long long s = 0;
-    for (long long i = lo; i < hi; ++i) s += count[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const unsigned long long v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    unsigned long long run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
-    for (long long i = lo; i < hi; ++i) {
-        start[i] = (unsigned)run;
-        run += count[i];
-    }
-    if (threadIdx.x == 1023) {
-        start[n] = (unsigned)part[1023];
-        *total = part[1023];
     }
 }
 
@@ -600,7 +574,6 @@ CarveEngine::CarveEngine(const double* d_xyz, long long npts, const int* d_tris,
     Rect* d_rect = I.dalloc<Rect>((size_t)ntris * ng);
     unsigned* d_count = I.dalloc<unsigned>((size_t)I.ncell);
     I.start = I.dalloc<unsigned>((size_t)I.ncell + 1);
-    unsigned long long* d_total = I.dalloc<unsigned long long>(1);
     I.bad = I.dalloc<unsigned long long>(1);
     CMG_HIP(hipMemsetAsync(d_count, 0, sizeof(unsigned) * (size_t)I.ncell, st));
     CMG_HIP(hipMemsetAsync(I.bad, 0, sizeof(unsigned long long), st));
@@ -612,12 +585,29 @@ CarveEngine::CarveEngine(const double* d_xyz, long long npts, const int* d_tris,
         rect_kernel<false><<<tb, 256, 0, st>>>(d_xyz, d_tris, ntris, I.P, I.g, I.tv, d_rect,
                                                d_count);
     CMG_HIP_LAUNCH();
-    scan_kernel<<<1, 1024, 0, st>>>(d_count, I.start, I.ncell, d_total);
-    CMG_HIP_LAUNCH();
-    CMG_HIP(hipMemcpyAsync(&I.entries, d_total, sizeof I.entries, hipMemcpyDeviceToHost, st));
+    // start = exclusive prefix of the counts; the total (start[ncell]) is the last start
+    // plus the last count. Counts are 32-bit; more than 2^31 entries in all is rejected.
+    size_t scan_tmp = 0;
+    CMG_HIP(rocprim::exclusive_scan(nullptr, scan_tmp, d_count, I.start, 0u, (size_t)I.ncell,
+                                    rocprim::plus<unsigned>(), st));
+    void* tmp = I.dalloc<char>(scan_tmp);
+    CMG_HIP(rocprim::exclusive_scan(tmp, scan_tmp, d_count, I.start, 0u, (size_t)I.ncell,
+                                    rocprim::plus<unsigned>(), st));
+    unsigned last[2];
+    CMG_HIP(hipMemcpyAsync(&last[0], I.start + I.ncell - 1, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipMemcpyAsync(&last[1], d_count + I.ncell - 1, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
+    I.entries = (unsigned long long)last[0] + last[1];
     if (I.entries >= (1ULL << 31))
         throw InvalidInput("carve: cell lists exceed 2^31 entries");
+    {
+        const unsigned tot = (unsigned)I.entries;
+        CMG_HIP(hipMemcpyAsync(I.start + I.ncell, &tot, sizeof(unsigned),
+                               hipMemcpyHostToDevice, st));
+        CMG_HIP(hipStreamSynchronize(st));
+    }
     I.items = I.dalloc<int>((size_t)I.entries);
     CMG_HIP(hipMemcpyAsync(d_count, I.start, sizeof(unsigned) * (size_t)I.ncell,
                            hipMemcpyDeviceToDevice, st));
