@@ -7,8 +7,10 @@
 // (include/cmg/core/circumcenter.hpp:15-46) and radius-edge ratio, and the six dihedral
 // angles (:52-77), then accumulates min / max / sum / count and the 18-bucket dihedral
 // histogram (LDS atomics) per workgroup. The ratio of every tet (DBL_MAX for degenerate
-// ones, as the reference's sentinel) is written for the worst-tets selection: a stable
-// descending radix sort of (ratio, index), so equal ratios keep ascending index order.
+// ones, as the reference's sentinel) is written for the worst-tets selection: for up to 64
+// worst tets, each workgroup selects its chunk's largest and a stable descending radix
+// sort orders the candidates; otherwise all (ratio, index) pairs are sorted. Either way
+// equal ratios come out by ascending index.
 //
 // Per tet: 16 B of indices and 96 B of gathered points in, 8 B out; ~300 FP64
 // operations including 6 acos, 7 sqrt and 4 divisions: FP64 VALU bound once the points
@@ -20,6 +22,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -190,6 +193,63 @@ __global__ __launch_bounds__(256) void quality_kernel(const double* xyz, long lo
     if (threadIdx.x < kBuckets) part[blockIdx.x].hist[threadIdx.x] = hist[threadIdx.x];
 }
 
+// Per-workgroup selection of the k largest ratios of a chunk of kChunk tets, largest first,
+// equal ratios by ascending index (k rounds of a workgroup arg-max with the winner taken
+// out). Every tet of the global k largest is among its chunk's k largest under this
+// order, and chunks are in index order, so a stable descending sort of the candidates
+// yields the global list.
+constexpr int kTopPer = 32;               // ratios per lane
+constexpr int kChunk = 256 * kTopPer;     // tets per workgroup
+constexpr int kMaxTopK = 64;
+
+__global__ __launch_bounds__(256) void topk_kernel(const double* ratio, long long n, int k,
+                                                   double* cand_r, int* cand_i) {
+    const long long c0 = (long long)blockIdx.x * kChunk;
+    double r[kTopPer];
+#pragma unroll
+    for (int q = 0; q < kTopPer; ++q) {
+        const long long t = c0 + (long long)q * 256 + threadIdx.x;
+        r[q] = t < n ? ratio[t] : -INFINITY;
+    }
+    __shared__ double sr[256];
+    __shared__ long long si[256];
+    for (int round = 0; round < k; ++round) {
+        double br = -INFINITY;
+        long long bi = LLONG_MAX;
+#pragma unroll
+        for (int q = 0; q < kTopPer; ++q) {
+            const long long t = c0 + (long long)q * 256 + threadIdx.x;
+            if (r[q] > br || (r[q] == br && t < bi)) br = r[q], bi = t;
+        }
+        sr[threadIdx.x] = br;
+        si[threadIdx.x] = bi;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) {
+                const double orr = sr[threadIdx.x + o];
+                const long long oi = si[threadIdx.x + o];
+                if (orr > sr[threadIdx.x] || (orr == sr[threadIdx.x] && oi < si[threadIdx.x]))
+                    sr[threadIdx.x] = orr, si[threadIdx.x] = oi;
+            }
+            __syncthreads();
+        }
+        const double wr = sr[0];
+        const long long wi = si[0];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            cand_r[(long long)blockIdx.x * k + round] = wr;
+            cand_i[(long long)blockIdx.x * k + round] = wi < n ? (int)wi : -1;
+        }
+        // the winner leaves the chunk
+        if (wi < n && (int)((wi - c0) % 256) == (int)threadIdx.x) {
+            const int q = (int)((wi - c0) / 256);
+#pragma unroll
+            for (int u = 0; u < kTopPer; ++u)
+                if (u == q) r[u] = -INFINITY;
+        }
+    }
+}
+
 __global__ void iota_kernel(int* v, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
@@ -214,15 +274,19 @@ QualityReport quality_report(const double* d_xyz, long long npts, const int* d_t
         throw InvalidInput("quality: tets must be 16-byte aligned");
     const unsigned nb = grid_for((size_t)n, 256, 4);
     const bool want_worst = worst && worst_count != 0;
+    // small lists: per-workgroup selection, then a sort of the candidates only
+    const bool select = want_worst && worst_count > 0 && worst_count <= kMaxTopK;
+    const long long nchunk = (n + kChunk - 1) / kChunk;
+    const long long nsort = select ? nchunk * worst_count : n;
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t sort_tmp = 0;
     if (want_worst)
         CMG_HIP(rocprim::radix_sort_pairs_desc(nullptr, sort_tmp, (const double*)nullptr,
                                                (double*)nullptr, (const int*)nullptr,
-                                               (int*)nullptr, (size_t)n, 0, 64, st));
+                                               (int*)nullptr, (size_t)nsort, 0, 64, st));
     const size_t bytes = al(sizeof(double) * n) + al(sizeof(Partial) * nb) + al(8) +
-                         (want_worst ? al(sizeof(double) * n) + 2 * al(sizeof(int) * n) +
-                                           al(sort_tmp)
+                         (want_worst ? 2 * al(sizeof(double) * nsort) +
+                                           2 * al(sizeof(int) * nsort) + al(sort_tmp)
                                      : 0);
     ScratchLease lease;
     char* base = static_cast<char*>(lease.s->get(bytes));
@@ -249,14 +313,21 @@ QualityReport quality_report(const double* d_xyz, long long npts, const int* d_t
     std::vector<int> top;
     if (want_worst) {
         k = worst_count < 0 ? n : std::min<long long>(worst_count, n);
-        double* keys_out = reinterpret_cast<double*>(take(sizeof(double) * n));
-        int* idx_in = reinterpret_cast<int*>(take(sizeof(int) * n));
-        int* idx_out = reinterpret_cast<int*>(take(sizeof(int) * n));
+        double* keys_in = select ? reinterpret_cast<double*>(take(sizeof(double) * nsort))
+                                 : ratio;
+        double* keys_out = reinterpret_cast<double*>(take(sizeof(double) * nsort));
+        int* idx_in = reinterpret_cast<int*>(take(sizeof(int) * nsort));
+        int* idx_out = reinterpret_cast<int*>(take(sizeof(int) * nsort));
         void* tmp = take(sort_tmp);
-        iota_kernel<<<grid_for((size_t)n, 256, 4), 256, 0, st>>>(idx_in, n);
+        if (select) {
+            topk_kernel<<<(unsigned)nchunk, 256, 0, st>>>(ratio, n, worst_count, keys_in,
+                                                          idx_in);
+        } else {
+            iota_kernel<<<grid_for((size_t)n, 256, 4), 256, 0, st>>>(idx_in, n);
+        }
         CMG_HIP_LAUNCH();
-        CMG_HIP(rocprim::radix_sort_pairs_desc(tmp, sort_tmp, ratio, keys_out, idx_in,
-                                               idx_out, (size_t)n, 0, 64, st));
+        CMG_HIP(rocprim::radix_sort_pairs_desc(tmp, sort_tmp, keys_in, keys_out, idx_in,
+                                               idx_out, (size_t)nsort, 0, 64, st));
         top.resize((size_t)k);
         CMG_HIP(hipMemcpyAsync(top.data(), idx_out, sizeof(int) * k, hipMemcpyDeviceToHost,
                                st));

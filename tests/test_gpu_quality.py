@@ -66,6 +66,18 @@ def test_exact_lattice_angles_on_bucket_edges(gpu):
     print(f"histogram entries moved across an edge: {moved}", flush=True)
 
 
+def test_worst_list_selection_path(gpu):
+    """Up to 64 worst tets come from a per-workgroup selection: same list as the oracle's
+    full order, including all-equal ratios (ascending index) and partial last chunks."""
+    pts, tets = W.lattice_tets(24, jitter=0.25, seed=6)
+    for k in (1, 17, 64, 65):
+        _compare(pts, tets, k)
+    pts, tets = W.lattice_tets(16)          # every ratio equal
+    g = _gpu(pts, tets, 64)
+    assert g["worst_tets"] == list(range(64))
+    _compare(pts, tets[:8193], 64)          # one full chunk and a single-tet chunk
+
+
 def test_all_worst_and_small_meshes(gpu):
     pts, tets = W.lattice_tets(6, jitter=0.2, seed=4)
     _compare(pts, tets, -1)
