@@ -11,9 +11,17 @@
 // with all four orientations >= 0 (else tet 0). Identical walks, hence identical answers,
 // to the reference's for the same mesh and start tet.
 //
+// Queries are walked in Morton order of their positions (a radix sort of 30-bit codes):
+// the lanes of a wavefront then walk nearly the same tets, which they load together and
+// in similar numbers of steps. The order changes no walk.
+//
 // Bound: latency of the dependent gathers (16 B of indices, 16 B of neighbours and 96 B
 // of points per step); lanes diverge in walk length.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <cmath>
+#include <vector>
 
 #include "device_common.hpp"
 #include "locate_internal.hpp"
@@ -35,15 +43,70 @@ __device__ __forceinline__ bool tet_ok(int4 v, long long npts) {
            v.z < npts && v.w < npts;
 }
 
+// Per-block partial bounds of the query points (min and max per axis; exact).
+__global__ __launch_bounds__(256) void qbounds_kernel(const double* q, long long nq,
+                                                      double* part) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nq;
+         i += (long long)gridDim.x * blockDim.x)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmin(lo[a], q[3 * i + a]);
+            hi[a] = fmax(hi[a], q[3 * i + a]);
+        }
+    __shared__ double s[6][256];
+    for (int a = 0; a < 3; ++a) s[a][threadIdx.x] = lo[a], s[3 + a][threadIdx.x] = hi[a];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int a = 0; a < 3; ++a) {
+                s[a][threadIdx.x] = fmin(s[a][threadIdx.x], s[a][threadIdx.x + o]);
+                s[3 + a][threadIdx.x] = fmax(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__device__ __forceinline__ unsigned spread10(unsigned v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// 30-bit Morton code of each query on a 1024^3 grid over the queries' bounds: sorting by
+// it puts nearby queries (which walk nearly the same tets) in the same wavefront.
+__global__ __launch_bounds__(256) void qmorton_kernel(const double* q, long long nq,
+                                                      double lx, double ly, double lz,
+                                                      double sx, double sy, double sz,
+                                                      unsigned* key, unsigned* val) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nq;
+         i += (long long)gridDim.x * blockDim.x) {
+        const double f[3] = {(q[3 * i] - lx) * sx, (q[3 * i + 1] - ly) * sy,
+                             (q[3 * i + 2] - lz) * sz};
+        unsigned c[3];
+        for (int a = 0; a < 3; ++a) {
+            const double x = f[a];
+            c[a] = x > 0.0 ? (x < 1023.0 ? (unsigned)x : 1023u) : 0u;  // NaN -> 0
+        }
+        key[i] = spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2);
+        val[i] = (unsigned)i;
+    }
+}
+
 __global__ __launch_bounds__(256) void locate_kernel(const double* xyz, long long npts,
                                                      const int* tets, const int* nbrs,
                                                      long long ntets, const double* q,
-                                                     long long nq, const int* start,
-                                                     int start_tet, int* out, int* steps_out,
+                                                     long long nq, const unsigned* order,
+                                                     const int* start, int start_tet,
+                                                     int* out, int* steps_out,
                                                      unsigned long long* ctr) {
     const long long limit = 2 * ntets + 16;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nq;
-         i += (long long)gridDim.x * blockDim.x) {
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < nq;
+         j += (long long)gridDim.x * blockDim.x) {
+        const long long i = order ? (long long)order[j] : j;
         const double Q[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
         int t = start ? start[i] : start_tet;
         int result = -1;
@@ -126,14 +189,55 @@ LocateStats locate_points(const double* d_xyz, long long npts, const int* d_tets
         throw InvalidInput("locate: tets and nbrs must be 16-byte aligned");
     if (!d_start && (start_tet < 0 || start_tet >= ntets))
         throw InvalidInput("locate: start tet out of range");
+    // Morton order of the queries (walk order only: every walk is the same as in input
+    // order, so the results are too)
+    const unsigned nb = grid_for((size_t)nq, 256, 4);
+    size_t sort_tmp = 0;
+    CMG_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp, (const unsigned*)nullptr,
+                                      (unsigned*)nullptr, (const unsigned*)nullptr,
+                                      (unsigned*)nullptr, (size_t)nq, 0, 30, st));
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t b_pair = al(sizeof(unsigned) * nq);
     ScratchLease lease;
-    auto* ctr = static_cast<unsigned long long*>(lease.s->get(2 * sizeof(unsigned long long)));
+    char* base = static_cast<char*>(lease.s->get(al(16) + al(sizeof(double) * 6 * nb) +
+                                                 4 * b_pair + al(sort_tmp)));
+    auto* ctr = reinterpret_cast<unsigned long long*>(base);
+    double* part = reinterpret_cast<double*>(base + al(16));
+    char* pb = base + al(16) + al(sizeof(double) * 6 * nb);
+    unsigned* key = reinterpret_cast<unsigned*>(pb);
+    unsigned* val = reinterpret_cast<unsigned*>(pb + b_pair);
+    unsigned* skey = reinterpret_cast<unsigned*>(pb + 2 * b_pair);
+    unsigned* sval = reinterpret_cast<unsigned*>(pb + 3 * b_pair);
+    void* tmp = pb + 4 * b_pair;
     CMG_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned long long), st));
+    qbounds_kernel<<<nb, 256, 0, st>>>(d_q, nq, part);
+    CMG_HIP_LAUNCH();
+    std::vector<double> hp((size_t)6 * nb);
+    CMG_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                           st));
+    CMG_HIP(hipStreamSynchronize(st));
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (unsigned b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::fmin(lo[a], hp[6 * b + a]);
+            hi[a] = std::fmax(hi[a], hp[6 * b + 3 + a]);
+        }
+    double sc[3];
+    for (int a = 0; a < 3; ++a) {
+        const double e = hi[a] - lo[a];
+        sc[a] = e > 0 && std::isfinite(e) ? 1024.0 / e : 0.0;
+        if (!std::isfinite(lo[a])) lo[a] = 0.0;
+    }
+    qmorton_kernel<<<nb, 256, 0, st>>>(d_q, nq, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2], key,
+                                       val);
+    CMG_HIP_LAUNCH();
+    CMG_HIP(rocprim::radix_sort_pairs(tmp, sort_tmp, key, skey, val, sval, (size_t)nq, 0, 30,
+                                      st));
     {
         TimedLaunch tl("locate", st);
         locate_kernel<<<grid_for((size_t)nq, 256, 8), 256, 0, st>>>(
-            d_xyz, npts, d_tets, d_nbrs, ntets, d_q, nq, d_start, start_tet, d_out, d_steps,
-            ctr);
+            d_xyz, npts, d_tets, d_nbrs, ntets, d_q, nq, sval, d_start, start_tet, d_out,
+            d_steps, ctr);
         CMG_HIP_LAUNCH();
     }
     unsigned long long h[2];
