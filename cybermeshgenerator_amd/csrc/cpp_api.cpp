@@ -1,4 +1,8 @@
 // cpp_api.cpp — the C++20 API (include/cmg/cmg_hip.hpp) over the C ABI of this library.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -6,6 +10,8 @@
 
 #include "../../include/cmg/cmg_hip.hpp"
 #include "../../include/cmg_hip.h"
+#include "device_common.hpp"
+#include "weld_internal.hpp"
 
 namespace cmg {
 
@@ -162,6 +168,119 @@ std::vector<int> orient3d_signs(std::span<const Point3> pts,
 }
 
 }  // namespace robust
+
+namespace io {
+
+namespace {
+
+expected<PLC, MeshError> to_plc(cmg_plc* h, cmg_status st, const char* err) {
+    if (st != CMG_OK) return unexpected(MeshError{code_of(st), err});
+    PLC out;
+    const std::size_t n = cmg_plc_num_points(h);
+    const double* xyz = cmg_plc_points(h);
+    out.points.resize(n);
+    for (std::size_t i = 0; i < n; ++i) out.points[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    const std::size_t m = cmg_plc_num_triangles(h);
+    const int* t = cmg_plc_triangles(h);
+    out.facets.resize(m);
+    for (std::size_t f = 0; f < m; ++f)
+        out.facets[f] = Facet{{Polygon{{t[3 * f], t[3 * f + 1], t[3 * f + 2]}}}, 0};
+    cmg_plc_destroy(h);
+    return out;
+}
+
+template <class F>
+expected<PLC, MeshError> read_with(F&& f) {
+    hip::SurfaceFile sf;
+    try {
+        sf = f();
+    } catch (const hip::InvalidInput& e) {
+        return unexpected(MeshError{MeshErrorCode::InvalidInput, e.what()});
+    }
+    PLC out;
+    out.points.resize(sf.xyz.size() / 3);
+    for (std::size_t i = 0; i < out.points.size(); ++i)
+        out.points[i] = {sf.xyz[3 * i], sf.xyz[3 * i + 1], sf.xyz[3 * i + 2]};
+    out.facets.reserve(sf.polys.size());
+    for (auto& poly : sf.polys) out.facets.push_back(Facet{{Polygon{std::move(poly)}}, 0});
+    return out;
+}
+
+}  // namespace
+
+expected<PLC, MeshError> read_stl(const std::string& path) {
+    auto r = read_with([&] { return hip::read_stl_file(path); });
+    if (r) g_last = backend::Backend::Hip;
+    return r;
+}
+
+expected<PLC, MeshError> read_obj(const std::string& path) {
+    return read_with([&] { return hip::read_obj_file(path); });
+}
+
+expected<PLC, MeshError> read_plc(const std::string& path) {
+    cmg_plc* h = nullptr;
+    char err[512] = {0};
+    const cmg_status st = cmg_read_plc(path.c_str(), &h, err, sizeof err);
+    return to_plc(h, st, err);
+}
+
+}  // namespace io
+
+namespace quality {
+
+QualityReport report(const Mesh& mesh, int worst_count) {
+    QualityReport r;
+    const std::size_t n = mesh.tetrahedra.size();
+    r.num_tets = n;
+    if (n == 0) return r;
+    std::vector<double> xyz;
+    flatten(mesh.points, xyz);
+    std::vector<int> tets(4 * n);
+    for (std::size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 4; ++k) tets[4 * i + k] = mesh.tetrahedra[i][k];
+    double* d_xyz = nullptr;
+    int* d_tets = nullptr;
+    auto chk = [](hipError_t e) {
+        if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+    };
+    chk(hipMalloc(&d_xyz, sizeof(double) * (xyz.size() ? xyz.size() : 1)));
+    chk(hipMalloc(&d_tets, sizeof(int) * tets.size()));
+    struct Free {
+        void* a;
+        void* b;
+        ~Free() {
+            (void)hipFree(a);
+            (void)hipFree(b);
+        }
+    } release{d_xyz, d_tets};
+    if (!xyz.empty())
+        chk(hipMemcpy(d_xyz, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice));
+    chk(hipMemcpy(d_tets, tets.data(), sizeof(int) * tets.size(), hipMemcpyHostToDevice));
+    cmg_quality_report q;
+    std::vector<int> worst(worst_count < 0 ? n : std::min<std::size_t>(worst_count, n));
+    char err[512] = {0};
+    const cmg_status st = cmg_hip_quality_report(d_xyz, mesh.points.size(), d_tets, n,
+                                                 worst_count, &q, worst.data(), nullptr, err,
+                                                 sizeof err);
+    if (st != CMG_OK) throw std::runtime_error(err);
+    g_last = backend::Backend::Hip;
+    r.min_radius_edge = q.min_radius_edge;
+    r.max_radius_edge = q.max_radius_edge;
+    r.mean_radius_edge = q.mean_radius_edge;
+    r.min_dihedral = q.min_dihedral;
+    r.max_dihedral = q.max_dihedral;
+    r.mean_dihedral = q.mean_dihedral;
+    r.min_volume = q.min_volume;
+    r.max_volume = q.max_volume;
+    r.total_volume = q.total_volume;
+    for (int b = 0; b < kHistogramBuckets; ++b) r.dihedral_histogram[b] = q.dihedral_histogram[b];
+    worst.resize(q.num_worst);
+    r.worst_tets = std::move(worst);
+    return r;
+}
+
+}  // namespace quality
 
 namespace backend {
 

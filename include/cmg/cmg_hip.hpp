@@ -7,6 +7,8 @@
 //                                              orient3d_batch, orient3d_signs)
 //   include/cmg/backend/dispatch.hpp:17-39    (Backend, Op, should_offload, last_backend,
 //                                              name)
+//   include/cmg/io/formats.hpp:41,58, io.hpp:50 (read_stl, read_obj, read_plc)
+//   include/cmg/quality/metrics.hpp:14-44     (QualityReport, report)
 // with minimal value types (Point3, Tetrahedron, PLC, MeshError, expected) standing in
 // for include/cmg/core. Every computation runs on the GPU; recoverable input errors come
 // back as unexpected(MeshError{InvalidInput, ...}) exactly where the reference returns
@@ -47,6 +49,11 @@ struct Facet {
 struct PLC {
     std::vector<Point3> points;
     std::vector<Facet> facets;
+};
+
+struct Mesh {  // include/cmg/core/mesh.hpp (the parts the quality report reads)
+    std::vector<Point3> points;
+    std::vector<Tetrahedron> tetrahedra;
 };
 
 enum class MeshErrorCode {  // include/cmg/core/error.hpp:40-48
@@ -135,6 +142,38 @@ std::vector<int> orient3d_signs(std::span<const Point3> pts,
                                 std::span<const Tetrahedron> candidates);
 
 }  // namespace robust
+
+namespace io {
+
+// formats.hpp:41 / 58 and io.hpp:50: the reference's readers (grammar and messages of
+// src/io/stl.cpp, src/io/obj.cpp); STL vertices are merged on the GPU. read_plc picks the
+// reader from the extension (.stl, .obj; other formats: NotImplemented).
+expected<PLC, MeshError> read_stl(const std::string& path);
+expected<PLC, MeshError> read_obj(const std::string& path);
+expected<PLC, MeshError> read_plc(const std::string& path);
+
+}  // namespace io
+
+namespace quality {
+
+inline constexpr int kHistogramBuckets = 18;
+
+struct QualityReport {  // metrics.hpp:14-40
+    std::size_t num_tets = 0;
+    double min_radius_edge = 0, max_radius_edge = 0, mean_radius_edge = 0;
+    double min_dihedral = 0, max_dihedral = 0, mean_dihedral = 0;
+    double min_volume = 0, max_volume = 0, total_volume = 0;
+    std::array<int, kHistogramBuckets> dihedral_histogram{};
+    std::vector<int> worst_tets;
+};
+
+// metrics.hpp:44, on the GPU. Equal radius-edge ratios are listed by ascending index;
+// the sums are accumulated in a different order than the reference's loop, and the
+// dihedral angles use the device acos. Throws std::runtime_error on a tet index out of
+// range or a device failure.
+QualityReport report(const Mesh& mesh, int worst_count = 10);
+
+}  // namespace quality
 
 namespace backend {
 

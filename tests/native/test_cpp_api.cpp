@@ -1,8 +1,11 @@
 // C++20 API test: the reference's own test cases for this path (tests/test_voxelize.cpp,
-// tests/test_backend.cpp, tests/test_foundation.cpp), run against cmg::voxelize /
-// cmg::robust / cmg::backend of libcmg_hip. Exit 0 on success.
+// tests/test_backend.cpp, tests/test_foundation.cpp, tests/io/test_stl.cpp,
+// tests/io/test_obj.cpp, tests/test_metrics.cpp), run against cmg::voxelize /
+// cmg::robust / cmg::backend / cmg::io / cmg::quality of libcmg_hip. Exit 0 on success.
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -123,6 +126,70 @@ int main() {
     }
     CHECK(backend::last_backend() == backend::Backend::Hip);
     CHECK(backend::name(backend::Backend::Hip) == "hip");
+    // tests/io/test_stl.cpp:35-109 — ascii round trip merges the 4 corners, one binary
+    // triangle, truncated vertex record
+    {
+        const char* path = "/tmp/cmg_hip_cpp_api.stl";
+        FILE* f = std::fopen(path, "w");
+        const double P[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        const int F[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+        std::fprintf(f, "solid cmg\n");
+        for (auto& t : F) {
+            std::fprintf(f, "  facet normal 0 0 1\n    outer loop\n");
+            for (int v : t) std::fprintf(f, "      vertex %g %g %g\n", P[v][0], P[v][1], P[v][2]);
+            std::fprintf(f, "    endloop\n  endfacet\n");
+        }
+        std::fprintf(f, "endsolid cmg\n");
+        std::fclose(f);
+        auto r = io::read_stl(path);
+        CHECK(bool(r) && r->facets.size() == 4 && r->points.size() == 4);
+        auto r2 = io::read_plc(path);
+        CHECK(bool(r2) && r2->points.size() == 4);
+        const char* bpath = "/tmp/cmg_hip_cpp_api_bin.stl";
+        f = std::fopen(bpath, "wb");
+        unsigned char buf[134] = {0};
+        const uint32_t count = 1;
+        std::memcpy(buf + 80, &count, 4);
+        const float coords[12] = {0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 3, 0};
+        std::memcpy(buf + 84, coords, sizeof coords);
+        std::fwrite(buf, 1, sizeof buf, f);
+        std::fclose(f);
+        auto rb = io::read_stl(bpath);
+        CHECK(bool(rb) && rb->facets.size() == 1 && rb->points.size() == 3);
+        f = std::fopen(path, "w");
+        std::fprintf(f, "solid bad\n  facet normal 0 0 1\n    outer loop\n      vertex 0 0\n");
+        std::fclose(f);
+        auto bad = io::read_stl(path);
+        CHECK(!bad && bad.error().code == MeshErrorCode::InvalidInput);
+    }
+    // tests/io/test_obj.cpp:58-77 — v/vt/vn corners use the vertex index; missing file
+    {
+        const char* path = "/tmp/cmg_hip_cpp_api.obj";
+        FILE* f = std::fopen(path, "w");
+        std::fprintf(f, "# comment line\nv 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nv 1 1 0\n"
+                        "v 1 0 1\nv 0 1 1\nvt 0 0\nvn 0 0 1\nf 1/2/3 4/5/6 7/8/9\n");
+        std::fclose(f);
+        auto r = io::read_obj(path);
+        CHECK(bool(r) && r->facets.size() == 1);
+        if (r) {
+            const auto& v = r->facets[0].polygons[0].vertices;
+            CHECK(v.size() == 3 && v[0] == 0 && v[1] == 3 && v[2] == 6);
+        }
+        CHECK(!io::read_obj("/tmp/cmg_hip_cpp_api_missing.obj"));
+    }
+    // tests/test_metrics.cpp:79-101 — a single regular-ish tet
+    {
+        Mesh m;
+        m.points = {{0, 0, 0}, {1, 0, 0}, {0.5, 0.866, 0}, {0.5, 0.289, 0.816}};
+        m.tetrahedra = {Tetrahedron{0, 1, 2, 3}};
+        auto q = quality::report(m, 10);
+        CHECK(q.num_tets == 1);
+        CHECK(q.min_dihedral >= 60.0 && q.min_dihedral <= 75.0);
+        CHECK(q.min_radius_edge >= 0.5 && q.min_radius_edge <= 0.75);
+        int hist = 0;
+        for (int h : q.dihedral_histogram) hist += h;
+        CHECK(hist == 6 && q.worst_tets.size() == 1 && q.worst_tets[0] == 0);
+    }
     CHECK(backend::should_offload(backend::Op::OrientFilter, 10));
     std::printf("C++ API: %s (%d failures)\n", failures ? "FAIL" : "OK", failures);
     return failures ? 1 : 0;
