@@ -174,6 +174,32 @@ __device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, 
 // true when the predicate holds on any lane of the wavefront
 __device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
+__device__ __forceinline__ float wave_max(float x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
+
+// Squared lower bound from the oriented box N to every point of the axis-aligned block
+// centred at (cx, cy, cz) with half-sizes (hx, hy, hz): along each box axis the block
+// projects to an interval of half-width hx|a_x| + hy|a_y| + hz|a_z|, and the squared
+// separations along the three orthonormal axes add up to at most the squared distance.
+// kBlockSlack (search units) covers the FP32 rounding of this bound and of node_lb2's, so
+// the result never exceeds node_lb2 at a point of the block.
+constexpr float kBlockSlack = 1e-5f;
+__device__ __forceinline__ float block_lb2(const NodeRec& N, float cx, float cy, float cz,
+                                           float hx, float hy, float hz) {
+    const float rn = fma3(hx, fabsf(N.nx), hy, fabsf(N.ny), hz, fabsf(N.nz));
+    const float ru = fma3(hx, fabsf(N.ux), hy, fabsf(N.uy), hz, fabsf(N.uz));
+    const float rw = fma3(hx, fabsf(N.wx), hy, fabsf(N.wy), hz, fabsf(N.wz));
+    const float tn = clamp01(fabsf(proj(N.nx, N.ny, N.nz, N.dn, cx, cy, cz)) - N.en - rn -
+                             kBlockSlack);
+    const float tu = clamp01(fabsf(proj(N.ux, N.uy, N.uz, N.du, cx, cy, cz)) - N.eu - ru -
+                             kBlockSlack);
+    const float tw = clamp01(fabsf(proj(N.wx, N.wy, N.wz, N.dw, cx, cy, cz)) - N.ew - rw -
+                             kBlockSlack);
+    return fma3(tn, tn, tu, tu, tw, tw);
+}
+
 // ---- phases ----------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void reset_kernel(Ctx C) {
@@ -854,6 +880,9 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         // centre of the wavefront's voxel block (relative to the grid corner)
         const float wcx = (ix * kLeaf + 2) * h, wcy = (iy * kLeaf + 2) * h,
                     wcz = (iz * L0.sz + 0.5f * L0.sz) * h;
+        // half-sizes of the block of voxel centres (cells 0..3 around the centre 2, and
+        // 0..sz-1 around sz/2)
+        const float whx = 1.5f * h, why = 1.5f * h, whz = (0.5f * L0.sz - 0.5f) * h;
 
         // Walks the tree with the squared bounds b2[] (which on_leaf may tighten; lanes
         // that take no part hold a negative bound); on_leaf(loc, cnt) visits the cnt
@@ -866,6 +895,8 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
         const int root = M.node_base[M.nlev];
         const long long tbase = M.tbase;
         auto walk = [&](const float* b2, auto&& on_leaf) {
+            float mb = 0.0f;  // largest bound of the wavefront, valid at version mb_ver
+            int mb_ver = -1;
             int stk = root;  // entry 0: the root, never tested before
             int stl = 0;
             int stv = -1;
@@ -897,17 +928,35 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 // bits order as unsigned integers
                 float ckey = 0.0f;
                 int clink = 0;  // link of child `lane`
+                // block-level test of child `lane`: a child whose box is farther from the
+                // whole voxel block than the largest bound of the wavefront fails every
+                // lane's test below, which is then skipped
+                if (mb_ver != ver) {  // the bounds only change when `ver` does
+                    mb = b2[0];
+#pragma unroll
+                    for (int v = 1; v < VPL; ++v) mb = fmaxf(mb, b2[v]);
+                    mb = wave_max(mb);
+                    mb_ver = ver;
+                }
+                bool bkeep = false;
                 if (lane < cnt) {
                     const float4 cc = C.nctr[first + lane];
                     const float kx = cc.x - wcx, ky = cc.y - wcy, kz = cc.z - wcz;
                     ckey = fma3(kx, kx, ky, ky, kz, kz);
                     clink = __float_as_int(cc.w);
+                    const float4* np = reinterpret_cast<const float4*>(C.node + first + lane);
+                    NodeRec K;
+                    float4* kd = reinterpret_cast<float4*>(&K);
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; ++q4) kd[q4] = np[q4];
+                    bkeep = block_lb2(K, wcx, wcy, wcz, whx, why, whz) <= mb;
                 }
+                const unsigned bmask = (unsigned)__builtin_amdgcn_ballot_w64(bkeep);
                 unsigned mask = 0, near_k = ~0u;
                 int near_c = -1;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
-                    if (c < cnt) {
+                    if (c < cnt && ((bmask >> c) & 1u)) {
                         ++nvis;
                         const NodeRec K = load_node(C.node, first + c);
                         bool need = false;
