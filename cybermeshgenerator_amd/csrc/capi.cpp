@@ -100,6 +100,8 @@ void host_signs(int npts_per, int mode, const double* xyz, size_t npts, const in
     if (mode != cmg::hip::kModeExact && mode != cmg::hip::kModeCompat)
         throw cmg::hip::InvalidInput("mode must be CMG_PRED_EXACT (0) or CMG_PRED_COMPAT (1)");
     if (n && (!xyz || !idx || !out)) throw cmg::hip::InvalidInput("null pointer");
+    // the compaction queue holds 32-bit tuple indices
+    if (n > 0xFFFFFFFFull) throw cmg::hip::InvalidInput("n must be < 2^32");
     for (size_t i = 0; i < n * (size_t)npts_per; ++i)
         if (idx[i] < 0 || (size_t)idx[i] >= npts)
             throw cmg::hip::InvalidInput("point index out of range");
@@ -276,6 +278,7 @@ cmg_status cmg_orient3d_values(const double* xyz, size_t npts, const int* quads,
     return guarded(errbuf, len, [&] {
         require_device();
         if (n && (!xyz || !quads || !out)) throw cmg::hip::InvalidInput("null pointer");
+        if (n > 0xFFFFFFFFull) throw cmg::hip::InvalidInput("n must be < 2^32");
         for (size_t i = 0; i < 4 * n; ++i)
             if (quads[i] < 0 || (size_t)quads[i] >= npts)
                 throw cmg::hip::InvalidInput("point index out of range");

@@ -86,6 +86,9 @@ const char* cmg_version(void);
  * exact sign when it is 0; src/predicates/robust.cpp:28-50, predicates.cpp:2230,4113). */
 enum { CMG_PRED_EXACT = 0, CMG_PRED_COMPAT = 1 };
 
+/* The batched predicate entry points below take n < 2^32 tuples per call (their
+ * compaction queue holds 32-bit tuple indices); larger n returns CMG_ERR_INVALID_INPUT. */
+
 /* Batched orient3d signs over index quadruples into AoS points: the batch of
  * cmg::robust::orient3d_signs (src/predicates/robust.cpp:68-105), int8 output. */
 cmg_status cmg_orient3d_signs(const double* xyz, size_t npts, const int* quads, size_t n,

@@ -138,6 +138,25 @@ def test_invalid_inputs_raise(gpu):
         gpu.orient3d_signs(np.zeros((4, 3)), np.array([[0, 1, 2, 3]]), mode="fast")
 
 
+def test_batch_size_limit(gpu):
+    """n >= 2^32 tuples is rejected before any index is read (the queue holds 32-bit
+    tuple indices); the buffers here are far smaller than n, and are never touched."""
+    import ctypes
+    lib = gpu.lib()
+    pts = np.zeros((8, 3))
+    idx = np.zeros(8, np.int32)
+    out = np.zeros(8, np.float64)
+    err = ctypes.create_string_buffer(128)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    n = 1 << 32
+    assert lib.cmg_orient3d_signs(p(pts), 8, p(idx), n, 0, p(out), err, 128) == 1
+    assert b"2^32" in err.value
+    assert lib.cmg_insphere_signs(p(pts), 8, p(idx), n, 0, p(out), err, 128) == 1
+    assert b"2^32" in err.value
+    assert lib.cmg_orient3d_values(p(pts), 8, p(idx), n, p(out), err, 128) == 1
+    assert b"2^32" in err.value
+
+
 def test_fixed_point_stages_on_device(gpu):
     """Tuples whose exact differences span 64..120 bits (some with exact zeros from a
     repeated point) and workload 1-ulp tuples: the register-only fixed-point and
