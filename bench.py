@@ -155,14 +155,27 @@ def bench_predicates(cm, W, steps, warmup):
     def step5():
         cm.signs_soa(s5, "exact", out=o5, workspace=ws5, stream=stream)
 
-    ms5 = time_steps(step5, steps, warmup, torch.cuda.synchronize)
+    o5b = torch.empty_like(o5)
+    ws5b = torch.empty_like(ws5)
+    lanes5 = [(stream, o5, ws5), (stream2, o5b, ws5b)]
+
+    def pipelined_step5():
+        st, o, w = lanes5[turn[0] & 1]
+        turn[0] += 1
+        cm.signs_soa(s5, "exact", out=o, workspace=w, stream=st)
+
+    ms5 = time_steps_streams(pipelined_step5, steps, warmup, [stream, stream2])
+    single5 = time_steps(step5, steps, warmup, torch.cuda.synchronize)
+    if not torch.equal(o5, o5b):
+        raise RuntimeError("insphere: the two streams' signs differ")
+    del o5b, ws5b
     cm.kernel_timer("insphere_filter", True)
     for _ in range(steps):
         step5()
     torch.cuda.synchronize()
     k5, c5 = cm.kernel_time("insphere_filter")
     cm.kernel_timer("insphere_filter", False)
-    res["insphere"] = {"n": n5, "ms": ms5, "filter_ms": k5 / max(c5, 1),
+    res["insphere"] = {"n": n5, "ms": ms5, "single_ms": single5, "filter_ms": k5 / max(c5, 1),
                        "stats": cm.pred_stats(n5, 5, workspace=ws5, stream=stream)}
     return res
 
@@ -614,6 +627,8 @@ def main():
     s5 = ins["stats"]
     extra["insphere"] = {
         "tuples": n5, "signs_per_s": n5 / (ins["ms"] / 1e3), "ms_per_batch": ins["ms"],
+        "streams": 2, "single_stream_ms_per_batch": ins["single_ms"],
+        "single_stream_signs_per_s": n5 / (ins["single_ms"] / 1e3),
         "filter_kernel_ms": ins["filter_ms"],
         "filter_hbm_gbs": INSPHERE_BYTES * n5 / (ins["filter_ms"] / 1e3) / 1e9,
         "stage_fraction": {"fp32": s5["fp32"] / n5, "fp64": s5["fp64"] / n5,
