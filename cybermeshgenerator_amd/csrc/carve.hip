@@ -161,6 +161,21 @@ __global__ __launch_bounds__(256) void bounds_kernel(const double* xyz, long lon
     if ((int)threadIdx.x < NV) part[(size_t)blockIdx.x * NV + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// Triangle vertex indices outside [0, npts), counted before any kernel gathers points
+// through them (one atomic per wavefront that found some).
+__global__ __launch_bounds__(256) void tri_check_kernel(const int* tris, long long ntris,
+                                                        long long npts,
+                                                        unsigned long long* bad) {
+    unsigned long long c = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < 3 * ntris;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int v = tris[i];
+        c += (v < 0 || v >= npts) ? 1ull : 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(bad, c);
+}
+
 // Packs the triangles' vertices (9 doubles each) and computes every triangle's cell
 // rectangle on each grid, counting the cells it covers.
 template <bool RAYS>
@@ -534,10 +549,20 @@ CarveEngine::CarveEngine(const double* d_xyz, long long npts, const int* d_tris,
     else
         bounds_kernel<false><<<nb, 256, 0, st>>>(d_xyz, npts, I.P, d_part);
     CMG_HIP_LAUNCH();
+    unsigned long long* d_badtri = I.dalloc<unsigned long long>(1);
+    CMG_HIP(hipMemsetAsync(d_badtri, 0, sizeof(unsigned long long), st));
+    tri_check_kernel<<<grid_for((size_t)(3 * ntris), 256, 4), 256, 0, st>>>(d_tris, ntris,
+                                                                           npts, d_badtri);
+    CMG_HIP_LAUNCH();
     std::vector<double> part((size_t)nb * nv);
     CMG_HIP(hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(),
                            hipMemcpyDeviceToHost, st));
+    unsigned long long badtri = 0;
+    CMG_HIP(hipMemcpyAsync(&badtri, d_badtri, sizeof badtri, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
+    if (badtri)
+        throw InvalidInput("carve: " + std::to_string(badtri) +
+                           " triangle vertex indices outside the point array");
     std::vector<double> b(nv);
     for (int j = 0; j < nv; j += 2) b[j] = 1e300, b[j + 1] = -1e300;
     for (unsigned blk = 0; blk < nb; ++blk)

@@ -285,7 +285,8 @@ typedef struct cmg_carve cmg_carve;
  * valid and unchanged for the handle's lifetime. resolution: parity grid cells per axis
  * (<= 0: the rays grid's, ceil(sqrt(ntris)) capped at 256; ignored by CMG_CARVE_RAYS).
  * Synchronises `stream`. ntris == 0 fails with the reference's message "PLC has facets but
- * no boundary triangles" (tetrahedralize_plc.cpp:250-253). */
+ * no boundary triangles" (tetrahedralize_plc.cpp:250-253); a triangle vertex index outside
+ * [0, npts) fails with CMG_ERR_INVALID_INPUT (checked on the device before any use). */
 cmg_status cmg_carve_create(const double* xyz, size_t npts, const int* tris, size_t ntris,
                             int mode, int resolution, void* stream, cmg_carve** out,
                             char* errbuf, size_t errbuf_len);

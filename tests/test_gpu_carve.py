@@ -110,6 +110,20 @@ def test_tets_bit_exact_and_bad_indices(gpu, mode):
     assert np.array_equal(got[~bad], want)
 
 
+@pytest.mark.parametrize("mode", ["rays", "parity"])
+def test_bad_triangle_indices_rejected_at_setup(gpu, mode):
+    """Device triangles with a vertex index outside the point array are counted on the
+    device and refused before any kernel gathers points through them."""
+    pts, tris = CASES["harmonic_c4"]
+    for bad in (len(pts), -1):
+        t = np.array(tris, np.int32)
+        t[5, 1] = bad
+        t[17, 2] = bad
+        with pytest.raises(RuntimeError, match="2 triangle vertex indices outside"):
+            _grid(pts, t, mode)
+    _grid(pts, tris, mode)   # the same triangles with their indices in range
+
+
 def test_plc_carve_host_drop_in(gpu):
     from cybermeshgenerator_amd.carve import plc_carve
     pts, tris = _l_prism()
