@@ -89,12 +89,20 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, lo
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
          v += (long long)gridDim.x * blockDim.x) {
         unsigned s = hash_key(soup + 3 * v) & mask;
-        // at most nv keys in a table of >= 2 nv slots: an empty slot is always reached
+        // at most nv keys in a table of >= 2 nv slots: an empty slot is always reached.
+        // A slot's key never changes once set and its value only decreases, so a plain
+        // read that finds it set (even a stale value) names the key correctly: the CAS is
+        // needed only where the slot looks empty.
         for (;;) {
-            const unsigned cur = atomicCAS(&tab[s], kEmpty, (unsigned)v);
-            if (cur == kEmpty) break;
+            unsigned cur = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == kEmpty) {
+                cur = atomicCAS(&tab[s], kEmpty, (unsigned)v);
+                if (cur == kEmpty) break;
+            }
             if (same_point(soup, cur, v)) {
-                atomicMin(&tab[s], (unsigned)v);
+                // the slot only decreases: with cur < v it already holds less than v
+                if ((unsigned)v < cur) atomicMin(&tab[s], (unsigned)v);
                 break;
             }
             s = (s + 1) & mask;
