@@ -11,9 +11,10 @@
 //   weld_insert_kernel open-addressing hash table of vertex numbers (2x capacity,
 //                      linear probing): the first vertex of a key claims a slot with a
 //                      CAS, later ones with an equal key lower it with atomicMin, so the
-//                      slot ends holding the key's first occurrence
-//   weld_rep_kernel    every vertex finds its key's slot: rep = first occurrence; flags
-//                      the vertices that are their own rep
+//                      slot ends holding the key's first occurrence; each vertex records
+//                      the slot of its key
+//   weld_rep_kernel    rep = the final value of the vertex's slot (its key's first
+//                      occurrence); flags the vertices that are their own rep
 //   (exclusive scan of the flags: the merged number of each first occurrence)
 //   weld_emit_kernel   merged index of every vertex, first occurrences' coordinates
 // Keys compare as doubles (-0 == +0, as the map's operator< treats them) and hash on
@@ -85,7 +86,8 @@ __device__ __forceinline__ bool same_point(const double* soup, unsigned a, long 
 }
 
 __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, long long nv,
-                                                          unsigned* tab, unsigned mask) {
+                                                          unsigned* tab, unsigned mask,
+                                                          unsigned* slot) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
          v += (long long)gridDim.x * blockDim.x) {
         unsigned s = hash_key(soup + 3 * v) & mask;
@@ -107,25 +109,18 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, lo
             }
             s = (s + 1) & mask;
         }
+        slot[v] = s;  // the slot of v's key
     }
 }
 
-__global__ __launch_bounds__(256) void weld_rep_kernel(const double* soup, long long nv,
-                                                       const unsigned* tab, unsigned mask,
+// After every insert: each slot holds the first occurrence of its key, so a vertex's
+// representative is the final value of the slot the insert recorded for it (rep holds the
+// slot on entry and the representative on exit).
+__global__ __launch_bounds__(256) void weld_rep_kernel(long long nv, const unsigned* tab,
                                                        unsigned* rep, unsigned* first) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
          v += (long long)gridDim.x * blockDim.x) {
-        unsigned s = hash_key(soup + 3 * v) & mask;
-        unsigned r = (unsigned)v;
-        for (;;) {
-            const unsigned cur = tab[s];
-            if (cur == kEmpty) break;  // (not reached: every vertex's key was inserted)
-            if (cur == (unsigned)v || same_point(soup, cur, v)) {
-                r = cur;
-                break;
-            }
-            s = (s + 1) & mask;
-        }
+        const unsigned r = tab[rep[v]];
         rep[v] = r;
         first[v] = r == (unsigned)v;
     }
@@ -191,9 +186,9 @@ long long weld_impl(Carve& W, const double* d_soup, long long nv, double* d_xyz,
     void* tmp = W.take<char>(tmp_bytes);
     CMG_HIP(hipMemsetAsync(tab, 0xff, sizeof(unsigned) * cap, st));
     const unsigned nb = grid_for((size_t)nv, 256, 8);
-    weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1);
+    weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep);
     CMG_HIP_LAUNCH();
-    weld_rep_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep, first);
+    weld_rep_kernel<<<nb, 256, 0, st>>>(nv, tab, rep, first);
     CMG_HIP_LAUNCH();
     CMG_HIP(rocprim::exclusive_scan(tmp, tmp_bytes, first, number, 0u, (size_t)nv,
                                     rocprim::plus<unsigned>(), st));

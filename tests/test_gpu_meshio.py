@@ -68,3 +68,32 @@ def test_device_decode_and_weld(gpu, tmp_path):
     raw = torch.zeros(100, dtype=torch.uint8, device="cuda")
     with pytest.raises(RuntimeError, match="not a binary STL"):
         files.stl_decode(raw)
+
+
+def test_weld_nan_vertices_never_merge(gpu):
+    """A vertex with a NaN coordinate is equal to no vertex, itself included, so each one
+    becomes a point of its own; the other vertices merge by exact value (-0 == +0) in
+    order of first occurrence. Expected numbering from a dict of first occurrences."""
+    import math
+    import torch
+    from cybermeshgenerator_amd import files
+    soup = shared_surface(3000, seed=11).astype(np.float64).reshape(-1, 3)
+    soup[::97, 1] = np.nan                      # NaN vertices, some at repeated positions
+    soup[5::211] = soup[4::211][: len(soup[5::211])]  # repeats next to them
+    soup[7::131, 0] = -0.0
+    soup[8::131, 0] = 0.0
+    first, pts, idx = {}, [], []
+    for v in soup:
+        if any(math.isnan(c) for c in v):
+            idx.append(len(pts))
+            pts.append(v)
+            continue
+        key = tuple(0.0 if c == 0 else c for c in v)
+        if key not in first:
+            first[key] = len(pts)
+            pts.append(v)
+        idx.append(first[key])
+    wp, wi = files.weld(torch.tensor(soup, device="cuda"))
+    assert np.array_equal(wi.cpu().numpy().ravel(), np.array(idx))
+    assert same_bits(wp.cpu().numpy(), np.array(pts))
+    assert int(np.isnan(wp.cpu().numpy()).any(1).sum()) == int(np.isnan(soup).any(1).sum())
