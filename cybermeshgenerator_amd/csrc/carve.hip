@@ -712,7 +712,10 @@ void classify(const CarveEngine::Impl& I, const double* d_q, const int* d_tets, 
     unsigned* nbad = reinterpret_cast<unsigned*>(take(al(8)));
     void* tmp = take(al(sort_tmp));
     const double* qpts = TETS ? qbuf : d_q;
-    const unsigned nb = grid_for((size_t)n, 256, 8);
+    // 20 waves per CU: the ray passes' registers allow 5 per SIMD to hide their dependent
+    // loads; the parity pass (2 per SIMD) keeps a grid of 8 per CU, measured faster
+    const unsigned nb = grid_for((size_t)n, 256, 20);
+    const unsigned nb_parity = grid_for((size_t)n, 256, 8);
     {
         TimedLaunch tl("carve_query", st);
         CMG_HIP(hipMemsetAsync(nbad, 0, sizeof(unsigned), st));
@@ -737,8 +740,8 @@ void classify(const CarveEngine::Impl& I, const double* d_q, const int* d_tets, 
             CMG_HIP_LAUNCH();
             rays_vote_kernel<<<nb, 256, 0, st>>>(key, ncell, n, odd, d_out);
         } else {
-            parity_pass_kernel<<<nb, 256, 0, st>>>(qpts, n, I.g, nbad, skey, sval, I.start,
-                                                   I.items, I.tv, d_out);
+            parity_pass_kernel<<<nb_parity, 256, 0, st>>>(qpts, n, I.g, nbad, skey, sval,
+                                                          I.start, I.items, I.tv, d_out);
         }
         CMG_HIP_LAUNCH();
     }

@@ -272,7 +272,8 @@ QualityReport quality_report(const double* d_xyz, long long npts, const int* d_t
     if (!d_xyz || !d_tets) throw InvalidInput("quality: null pointer");
     if (reinterpret_cast<uintptr_t>(d_tets) % 16)
         throw InvalidInput("quality: tets must be 16-byte aligned");
-    const unsigned nb = grid_for((size_t)n, 256, 4);
+    // 3 workgroups of 256 per CU: as many waves as the scan kernel's registers let run
+    const unsigned nb = grid_for((size_t)n, 256, 12);
     const bool want_worst = worst && worst_count != 0;
     // small lists: per-workgroup selection, then a sort of the candidates only
     const bool select = want_worst && worst_count > 0 && worst_count <= kMaxTopK;

@@ -185,7 +185,9 @@ long long weld_impl(Carve& W, const double* d_soup, long long nv, double* d_xyz,
     size_t tmp_bytes = scan_tmp_bytes(nv, st);
     void* tmp = W.take<char>(tmp_bytes);
     CMG_HIP(hipMemsetAsync(tab, 0xff, sizeof(unsigned) * cap, st));
-    const unsigned nb = grid_for((size_t)nv, 256, 8);
+    // 32 waves per CU: the random probes and atomics are latency-bound and these kernels
+    // are small enough (<= 23 VGPRs) for 8 waves per SIMD
+    const unsigned nb = grid_for((size_t)nv, 256, 32);
     weld_insert_kernel<<<nb, 256, 0, st>>>(d_soup, nv, tab, cap - 1, rep);
     CMG_HIP_LAUNCH();
     weld_rep_kernel<<<nb, 256, 0, st>>>(nv, tab, rep, first);
