@@ -41,26 +41,71 @@ namespace {
 constexpr unsigned kEmpty = 0xffffffffu;
 constexpr int kDecodeTris = 256;  // records per workgroup (12,800 B)
 
+// 16-byte vector of dwords that needs only 4-byte alignment (a record block starts 4 bytes
+// past a 16-byte boundary: 84 + 12,800 k)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// One workgroup decodes kDecodeTris records: the block's bytes go to LDS (full blocks:
+// 16-byte loads, all in flight together), each lane converts its record's 9 floats, and
+// the 9 * 256 doubles leave through LDS as coalesced 16-byte stores.
 __global__ __launch_bounds__(256) void stl_decode_kernel(const uint8_t* bytes, long long ntris,
                                                          double* soup) {
-    __shared__ uint32_t lds[kDecodeTris * 50 / 4];
+    __shared__ __align__(16) double lds_d[kDecodeTris * 9];  // 18,432 B; input uses 12,800
+    uint32_t* lds = reinterpret_cast<uint32_t*>(lds_d);
     const long long t0 = (long long)blockIdx.x * kDecodeTris;
     const int nt = (int)min((long long)kDecodeTris, ntris - t0);
     const uint8_t* base = bytes + 84 + 50 * t0;  // 84 + 12,800 * block: 4-byte aligned
     const int nbytes = 50 * nt;
     const uint32_t* w = reinterpret_cast<const uint32_t*>(base);
-    for (int i = threadIdx.x; i < nbytes / 4; i += blockDim.x) lds[i] = w[i];
-    if (threadIdx.x == 0 && (nbytes & 3))  // odd record count: a 2-byte tail
-        lds[nbytes / 4] = reinterpret_cast<const uint16_t*>(base)[nbytes / 2 - 1];
-    __syncthreads();
-    if ((int)threadIdx.x >= nt) return;
-    const uint16_t* h = reinterpret_cast<const uint16_t*>(lds);
-    const int o = (50 * threadIdx.x + 12) / 2;  // vertex floats start 12 B into a record
-    double* out = soup + 9 * (t0 + threadIdx.x);
+    if (nt == kDecodeTris) {
+        constexpr int kVec = kDecodeTris * 50 / 16;  // 800 16-byte words
+        constexpr int kPer = (kVec + 255) / 256;     // 4 per lane
+        const u32x4* wv = reinterpret_cast<const u32x4*>(w);
+        u32x4 r[kPer];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        const uint32_t bits = (uint32_t)h[o + 2 * j] | ((uint32_t)h[o + 2 * j + 1] << 16);
-        out[j] = (double)__uint_as_float(bits);
+        for (int j = 0; j < kPer; ++j) {
+            const int q = j * 256 + (int)threadIdx.x;
+            r[j] = wv[q < kVec ? q : kVec - 1];  // unconditional: the loads issue together
+        }
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int q = j * 256 + (int)threadIdx.x;
+            if (q < kVec) {
+                lds[4 * q] = r[j].x;
+                lds[4 * q + 1] = r[j].y;
+                lds[4 * q + 2] = r[j].z;
+                lds[4 * q + 3] = r[j].w;
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < nbytes / 4; i += blockDim.x) lds[i] = w[i];
+        if (threadIdx.x == 0 && (nbytes & 3))  // odd record count: a 2-byte tail
+            lds[nbytes / 4] = reinterpret_cast<const uint16_t*>(base)[nbytes / 2 - 1];
+    }
+    __syncthreads();
+    float f[9];
+    if ((int)threadIdx.x < nt) {
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(lds);
+        const int o = (50 * threadIdx.x + 12) / 2;  // vertex floats start 12 B into a record
+#pragma unroll
+        for (int j = 0; j < 9; ++j)
+            f[j] = __uint_as_float((uint32_t)h[o + 2 * j] | ((uint32_t)h[o + 2 * j + 1] << 16));
+    }
+    __syncthreads();  // every lane has read its record: the buffer takes the doubles
+    if ((int)threadIdx.x < nt) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) lds_d[9 * threadIdx.x + j] = (double)f[j];
+    }
+    __syncthreads();
+    double* out = soup + 9 * t0;
+    const int nd = 9 * nt;
+    if ((reinterpret_cast<uintptr_t>(out) & 15) == 0) {
+        const double2* src = reinterpret_cast<const double2*>(lds_d);
+        double2* dst = reinterpret_cast<double2*>(out);
+        for (int q = threadIdx.x; q < nd / 2; q += blockDim.x) dst[q] = src[q];
+        if ((nd & 1) && threadIdx.x == 0) out[nd - 1] = lds_d[nd - 1];
+    } else {
+        for (int q = threadIdx.x; q < nd; q += blockDim.x) out[q] = lds_d[q];
     }
 }
 
