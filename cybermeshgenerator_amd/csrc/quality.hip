@@ -202,51 +202,84 @@ constexpr int kTopPer = 32;               // ratios per lane
 constexpr int kChunk = 256 * kTopPer;     // tets per workgroup
 constexpr int kMaxTopK = 64;
 
+// (ra, ia) comes before (rb, ib) in the worst list: larger ratio, then smaller index
+__device__ __forceinline__ bool worse(double ra, long long ia, double rb, long long ib) {
+    return ra > rb || (ra == rb && ia < ib);
+}
+
 __global__ __launch_bounds__(256) void topk_kernel(const double* ratio, long long n, int k,
                                                    double* cand_r, int* cand_i) {
     const long long c0 = (long long)blockIdx.x * kChunk;
+    const int tid = (int)threadIdx.x;
     double r[kTopPer];
 #pragma unroll
     for (int q = 0; q < kTopPer; ++q) {
-        const long long t = c0 + (long long)q * 256 + threadIdx.x;
+        const long long t = c0 + (long long)q * 256 + tid;
         r[q] = t < n ? ratio[t] : -INFINITY;
     }
-    __shared__ double sr[256];
-    __shared__ long long si[256];
-    for (int round = 0; round < k; ++round) {
-        double br = -INFINITY;
-        long long bi = LLONG_MAX;
+    // each lane's best of its own values, rescanned only when that best leaves the chunk
+    double br;
+    long long bi;
+    auto scan = [&]() {
+        br = -INFINITY;
+        bi = LLONG_MAX;
 #pragma unroll
         for (int q = 0; q < kTopPer; ++q) {
-            const long long t = c0 + (long long)q * 256 + threadIdx.x;
-            if (r[q] > br || (r[q] == br && t < bi)) br = r[q], bi = t;
+            const long long t = c0 + (long long)q * 256 + tid;
+            if (worse(r[q], t, br, bi)) br = r[q], bi = t;
         }
-        sr[threadIdx.x] = br;
-        si[threadIdx.x] = bi;
+    };
+    scan();
+    __shared__ double sr[256];
+    __shared__ long long si[256];
+    __shared__ double s_wr;
+    __shared__ long long s_wi;
+    for (int round = 0; round < k; ++round) {
+        // a tree over the lanes' bests, lane t taking lane t + o's when it is worse, for
+        // o = 128, 64 through LDS and 32 .. 1 by shuffles inside the first wavefront
+        sr[tid] = br;
+        si[tid] = bi;
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) {
-                const double orr = sr[threadIdx.x + o];
-                const long long oi = si[threadIdx.x + o];
-                if (orr > sr[threadIdx.x] || (orr == sr[threadIdx.x] && oi < si[threadIdx.x]))
-                    sr[threadIdx.x] = orr, si[threadIdx.x] = oi;
+        double xr = br;
+        long long xi = bi;
+        if (tid < 128) {
+            const double orr = sr[tid + 128];
+            const long long oi = si[tid + 128];
+            if (worse(orr, oi, xr, xi)) xr = orr, xi = oi;
+            sr[tid] = xr;
+            si[tid] = xi;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            const double orr = sr[tid + 64];
+            const long long oi = si[tid + 64];
+            if (worse(orr, oi, xr, xi)) xr = orr, xi = oi;
+            for (int o = 32; o > 0; o >>= 1) {
+                const double dr = __shfl_down(xr, o);
+                const long long di = __shfl_down(xi, o);
+                if (tid < o && worse(dr, di, xr, xi)) xr = dr, xi = di;
             }
-            __syncthreads();
+            if (tid == 0) {
+                s_wr = xr;
+                s_wi = xi;
+            }
         }
-        const double wr = sr[0];
-        const long long wi = si[0];
         __syncthreads();
-        if (threadIdx.x == 0) {
+        const double wr = s_wr;
+        const long long wi = s_wi;
+        if (tid == 0) {
             cand_r[(long long)blockIdx.x * k + round] = wr;
             cand_i[(long long)blockIdx.x * k + round] = wi < n ? (int)wi : -1;
         }
-        // the winner leaves the chunk
-        if (wi < n && (int)((wi - c0) % 256) == (int)threadIdx.x) {
+        // the winner leaves the chunk; its lane rescans
+        if (wi < n && (int)((wi - c0) % 256) == tid) {
             const int q = (int)((wi - c0) / 256);
 #pragma unroll
             for (int u = 0; u < kTopPer; ++u)
                 if (u == q) r[u] = -INFINITY;
+            scan();
         }
+        __syncthreads();  // sr, si and s_wr / s_wi are rewritten next round
     }
 }
 
