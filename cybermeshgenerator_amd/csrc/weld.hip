@@ -130,31 +130,73 @@ __device__ __forceinline__ bool same_point(const double* soup, unsigned a, long 
     return p[0] == q[0] && p[1] == q[1] && p[2] == q[2];
 }
 
+// Global insert of vertex v: returns the slot of v's key. A slot's key never changes once
+// set and its value only decreases, so a plain read that finds it set (even a stale
+// value) names the key correctly: the CAS is needed only where the slot looks empty.
+// At most nv keys in a table of >= 2 nv slots: an empty slot is always reached.
+__device__ __forceinline__ unsigned global_insert(const double* soup, long long v, unsigned h,
+                                                  unsigned* tab, unsigned mask) {
+    unsigned s = h & mask;
+    for (;;) {
+        unsigned cur = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == kEmpty) {
+            cur = atomicCAS(&tab[s], kEmpty, (unsigned)v);
+            if (cur == kEmpty) return s;
+        }
+        if (same_point(soup, cur, v)) {
+            // the slot only decreases: with cur < v it already holds less than v
+            if ((unsigned)v < cur) atomicMin(&tab[s], (unsigned)v);
+            return s;
+        }
+        s = (s + 1) & mask;
+    }
+}
+
+// Each workgroup first merges its 256 consecutive vertices in an LDS table (an STL file
+// repeats a vertex in neighbouring triangles), keeping the smallest local index of each
+// key; only that vertex inserts into the global table, and the others take its slot. The
+// global slot still ends at the key's first occurrence (the minimum over the workgroups'
+// minima). slot[v] = the slot of v's key.
+constexpr int kLocalSlots = 512;
 __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, long long nv,
                                                           unsigned* tab, unsigned mask,
                                                           unsigned* slot) {
-    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x) {
-        unsigned s = hash_key(soup + 3 * v) & mask;
-        // at most nv keys in a table of >= 2 nv slots: an empty slot is always reached.
-        // A slot's key never changes once set and its value only decreases, so a plain
-        // read that finds it set (even a stale value) names the key correctly: the CAS is
-        // needed only where the slot looks empty.
-        for (;;) {
-            unsigned cur = __hip_atomic_load(&tab[s], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-            if (cur == kEmpty) {
-                cur = atomicCAS(&tab[s], kEmpty, (unsigned)v);
-                if (cur == kEmpty) break;
-            }
-            if (same_point(soup, cur, v)) {
-                // the slot only decreases: with cur < v it already holds less than v
-                if ((unsigned)v < cur) atomicMin(&tab[s], (unsigned)v);
-                break;
-            }
-            s = (s + 1) & mask;
+    __shared__ unsigned s_key[kLocalSlots];  // local index of the key's first vertex
+    __shared__ unsigned s_slot[256];         // global slot of each local representative
+    __shared__ double s_xyz[256 * 3];
+    const int tid = (int)threadIdx.x;
+    for (long long base = (long long)blockIdx.x * 256; base < nv;
+         base += (long long)gridDim.x * 256) {
+        const long long v = base + tid;
+        const bool ok = v < nv;
+        s_key[tid] = kEmpty;
+        s_key[tid + 256] = kEmpty;
+        unsigned h = 0;
+        if (ok) {
+            for (int a = 0; a < 3; ++a) s_xyz[3 * tid + a] = soup[3 * v + a];
+            h = hash_key(soup + 3 * v);
         }
-        slot[v] = s;  // the slot of v's key
+        __syncthreads();
+        unsigned ls = 0;
+        if (ok) {
+            ls = h & (kLocalSlots - 1);
+            for (;;) {  // at most 256 keys in 512 slots
+                const unsigned cur = atomicCAS(&s_key[ls], kEmpty, (unsigned)tid);
+                if (cur == kEmpty) break;
+                if (s_xyz[3 * cur] == s_xyz[3 * tid] && s_xyz[3 * cur + 1] == s_xyz[3 * tid + 1] &&
+                    s_xyz[3 * cur + 2] == s_xyz[3 * tid + 2]) {
+                    atomicMin(&s_key[ls], (unsigned)tid);
+                    break;
+                }
+                ls = (ls + 1) & (kLocalSlots - 1);
+            }
+        }
+        __syncthreads();
+        const unsigned rep = ok ? s_key[ls] : 0u;
+        if (ok && rep == (unsigned)tid) s_slot[tid] = global_insert(soup, v, h, tab, mask);
+        __syncthreads();
+        if (ok) slot[v] = s_slot[rep];
+        __syncthreads();  // s_key, s_slot and s_xyz are rewritten by the next round
     }
 }
 
