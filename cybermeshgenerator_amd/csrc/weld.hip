@@ -12,7 +12,8 @@
 //                      linear probing): the first vertex of a key claims a slot with a
 //                      CAS, later ones with an equal key lower it with atomicMin, so the
 //                      slot ends holding the key's first occurrence; each vertex records
-//                      the slot of its key
+//                      the slot of its key. A workgroup merges its 256 vertices in LDS
+//                      first, and only each key's smallest local index probes globally.
 //   weld_rep_kernel    rep = the final value of the vertex's slot (its key's first
 //                      occurrence); flags the vertices that are their own rep
 //   (exclusive scan of the flags: the merged number of each first occurrence)
