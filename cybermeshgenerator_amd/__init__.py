@@ -86,6 +86,8 @@ def _load() -> ctypes.CDLL:
     lib.cmg_insphere_signs.argtypes = [_P, _sz, _P, _sz, _i, _P, _errbuf_t, _sz]
     lib.cmg_orient3d_values.restype = _i
     lib.cmg_orient3d_values.argtypes = [_P, _sz, _P, _sz, _P, _errbuf_t, _sz]
+    lib.cmg_insphere_values.restype = _i
+    lib.cmg_insphere_values.argtypes = [_P, _sz, _P, _sz, _P, _errbuf_t, _sz]
     lib.cmg_voxels_sdf_backward.restype = _i
     lib.cmg_voxels_sdf_backward.argtypes = [_P, _P, _P, _P, _errbuf_t, _sz]
     lib.cmg_voxels_sdf_nearest.restype = _i
@@ -220,6 +222,19 @@ def orient3d_batch(points, quads) -> np.ndarray:
     out = np.empty(q.shape[0], dtype=np.float64)
     err = _err()
     _check(_lib.cmg_orient3d_values(_ptr(pts), pts.shape[0], _ptr(q), q.shape[0],
+                                    _ptr(out), err, len(err)), err)
+    return out
+
+
+def insphere_values(points, quints) -> np.ndarray:
+    """float64 insphere values for index quintuples (a, b, c, d, e): what
+    cmg::robust::insphere returns (robust.cpp:44-50) — the naive FP64 determinant when it
+    is nonzero, else the exact determinant's leading component."""
+    pts = _f64(points).reshape(-1, 3)
+    q = _i32(quints).reshape(-1, 5)
+    out = np.empty(q.shape[0], dtype=np.float64)
+    err = _err()
+    _check(_lib.cmg_insphere_values(_ptr(pts), pts.shape[0], _ptr(q), q.shape[0],
                                     _ptr(out), err, len(err)), err)
     return out
 

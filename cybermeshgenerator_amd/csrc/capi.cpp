@@ -132,6 +132,41 @@ void host_signs(int npts_per, int mode, const double* xyz, size_t npts, const in
                                  std::to_string(c.failed) + " tuples");
 }
 
+// Host-pointer batched values (npts_per 4 = orient3d, 5 = insphere).
+void host_values(int npts_per, const double* xyz, size_t npts, const int* idx, size_t n,
+                 double* out) {
+    require_device();
+    if (n && (!xyz || !idx || !out)) throw cmg::hip::InvalidInput("null pointer");
+    if (n > 0xFFFFFFFFull) throw cmg::hip::InvalidInput("n must be < 2^32");
+    for (size_t i = 0; i < n * (size_t)npts_per; ++i)
+        if (idx[i] < 0 || (size_t)idx[i] >= npts)
+            throw cmg::hip::InvalidInput("point index out of range");
+    g_last_backend = "hip";
+    if (n == 0) return;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 1);
+    size_t wsb = cmg::hip::pred_workspace_bytes(plan, n);
+    size_t b_xyz = al(npts * 3 * sizeof(double)), b_idx = al(n * npts_per * sizeof(int)),
+           b_out = al(n * sizeof(double));
+    cmg::hip::ScratchLease lease;
+    char* base = static_cast<char*>(lease.s->get(b_xyz + b_idx + b_out + wsb));
+    double* d_xyz = reinterpret_cast<double*>(base);
+    int* d_idx = reinterpret_cast<int*>(base + b_xyz);
+    double* d_out = reinterpret_cast<double*>(base + b_xyz + b_idx);
+    cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(base + b_xyz + b_idx + b_out, plan, n);
+    hipStream_t st = nullptr;
+    CMG_HIP(hipMemcpyAsync(d_xyz, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+    CMG_HIP(hipMemcpyAsync(d_idx, idx, n * npts_per * sizeof(int), hipMemcpyHostToDevice, st));
+    cmg::hip::launch_values_indexed(npts_per, d_xyz, d_idx, n, d_out, plan, ws, st);
+    CMG_HIP(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    cmg::hip::PredCounters c{};
+    CMG_HIP(hipMemcpyAsync(&c, ws.ctr, sizeof c, hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    if (c.failed)
+        throw cmg::hip::HipError("exact predicate: expansion capacity exceeded on " +
+                                 std::to_string(c.failed) + " tuples");
+}
+
 cmg_status soa_signs(int npts, const double* soa, size_t n, size_t ld, int mode,
                      signed char* out, void* workspace, size_t workspace_bytes,
                      void* stream, char* errbuf, size_t len) {
@@ -275,34 +310,12 @@ cmg_status cmg_insphere_signs(const double* xyz, size_t npts, const int* quints,
 
 cmg_status cmg_orient3d_values(const double* xyz, size_t npts, const int* quads, size_t n,
                                double* out, char* errbuf, size_t len) {
-    return guarded(errbuf, len, [&] {
-        require_device();
-        if (n && (!xyz || !quads || !out)) throw cmg::hip::InvalidInput("null pointer");
-        if (n > 0xFFFFFFFFull) throw cmg::hip::InvalidInput("n must be < 2^32");
-        for (size_t i = 0; i < 4 * n; ++i)
-            if (quads[i] < 0 || (size_t)quads[i] >= npts)
-                throw cmg::hip::InvalidInput("point index out of range");
-        g_last_backend = "hip";
-        if (n == 0) return;
-        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-        cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 1);
-        size_t wsb = cmg::hip::pred_workspace_bytes(plan, n);
-        size_t b_xyz = al(npts * 3 * sizeof(double)), b_idx = al(n * 4 * sizeof(int)),
-               b_out = al(n * sizeof(double));
-        cmg::hip::ScratchLease lease;
-        char* base = static_cast<char*>(lease.s->get(b_xyz + b_idx + b_out + wsb));
-        double* d_xyz = reinterpret_cast<double*>(base);
-        int* d_idx = reinterpret_cast<int*>(base + b_xyz);
-        double* d_out = reinterpret_cast<double*>(base + b_xyz + b_idx);
-        cmg::hip::PredWorkspace ws =
-            cmg::hip::pred_workspace(base + b_xyz + b_idx + b_out, plan, n);
-        hipStream_t st = nullptr;
-        CMG_HIP(hipMemcpyAsync(d_xyz, xyz, npts * 3 * sizeof(double), hipMemcpyHostToDevice, st));
-        CMG_HIP(hipMemcpyAsync(d_idx, quads, n * 4 * sizeof(int), hipMemcpyHostToDevice, st));
-        cmg::hip::launch_o3_values_indexed(d_xyz, d_idx, n, d_out, plan, ws, st);
-        CMG_HIP(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
-        CMG_HIP(hipStreamSynchronize(st));
-    });
+    return guarded(errbuf, len, [&] { host_values(4, xyz, npts, quads, n, out); });
+}
+
+cmg_status cmg_insphere_values(const double* xyz, size_t npts, const int* quints, size_t n,
+                               double* out, char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] { host_values(5, xyz, npts, quints, n, out); });
 }
 
 cmg_status cmg_voxels_sdf_backward(const cmg_plc* plc, const cmg_voxels* grid,

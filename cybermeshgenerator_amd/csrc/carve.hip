@@ -177,12 +177,14 @@ __global__ __launch_bounds__(256) void tri_check_kernel(const int* tris, long lo
 }
 
 // Packs the triangles' vertices (9 doubles each) and computes every triangle's cell
-// rectangle on each grid, counting the cells it covers.
+// rectangle on each grid, and sums the cells they cover (the list total) in 64 bits.
 template <bool RAYS>
 __global__ __launch_bounds__(256) void rect_kernel(const double* xyz, const int* tris,
                                                    long long ntris, RayParams P, GridSpec g,
-                                                   double* tv, Rect* rect, unsigned* count) {
+                                                   double* tv, Rect* rect,
+                                                   unsigned long long* entries) {
     constexpr int NG = RAYS ? 3 : 1;
+    unsigned long long area = 0;  // list entries of this lane's rectangles, in 64 bits
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ntris;
          t += (long long)gridDim.x * blockDim.x) {
         double V[3][3];
@@ -195,7 +197,6 @@ __global__ __launch_bounds__(256) void rect_kernel(const double* xyz, const int*
         }
         for (int k = 0; k < NG; ++k) {
             Rect R;
-            GridMap M;
             if constexpr (RAYS) {
                 // tetrahedralize_plc.cpp:146-157
                 const int r = P.r;
@@ -206,14 +207,31 @@ __global__ __launch_bounds__(256) void rect_kernel(const double* xyz, const int*
                     R.a0 = min(R.a0, iu), R.a1 = max(R.a1, iu);
                     R.b0 = min(R.b0, iv), R.b1 = max(R.b1, iv);
                 }
-                M = GridMap{k * r * r, r, 1};
             } else {
                 // voxelize.cpp:90-97
                 const int4 xr = occdev::xy_range(V, g);
                 R = Rect{xr.x, xr.y, xr.z, xr.w};
-                M = GridMap{0, 1, g.nx};
             }
             rect[k * ntris + t] = R;
+            if (R.a1 >= R.a0 && R.b1 >= R.b0)
+                area += (unsigned long long)(R.a1 - R.a0 + 1) *
+                        (unsigned long long)(R.b1 - R.b0 + 1);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) area += __shfl_xor(area, o);
+    if (__lane_id() == 0 && area) atomicAdd(entries, area);
+}
+
+// Per-cell list lengths from the rectangles (run only once their total is known to fit).
+template <bool RAYS>
+__global__ __launch_bounds__(256) void count_kernel(long long ntris, RayParams P, GridSpec g,
+                                                    const Rect* rect, unsigned* count) {
+    constexpr int NG = RAYS ? 3 : 1;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < ntris;
+         t += (long long)gridDim.x * blockDim.x) {
+        for (int k = 0; k < NG; ++k) {
+            const Rect R = rect[k * ntris + t];
+            const GridMap M = RAYS ? GridMap{k * P.r * P.r, P.r, 1} : GridMap{0, 1, g.nx};
             for (int a = R.a0; a <= R.a1; ++a)
                 for (int b = R.b0; b <= R.b1; ++b)
                     atomicAdd(&count[M.base + a * M.sa + b * M.sb], 1u);
@@ -600,33 +618,38 @@ CarveEngine::CarveEngine(const double* d_xyz, long long npts, const int* d_tris,
     unsigned* d_count = I.dalloc<unsigned>((size_t)I.ncell);
     I.start = I.dalloc<unsigned>((size_t)I.ncell + 1);
     I.bad = I.dalloc<unsigned long long>(1);
+    unsigned long long* d_entries = I.dalloc<unsigned long long>(1);
     CMG_HIP(hipMemsetAsync(d_count, 0, sizeof(unsigned) * (size_t)I.ncell, st));
     CMG_HIP(hipMemsetAsync(I.bad, 0, sizeof(unsigned long long), st));
+    CMG_HIP(hipMemsetAsync(d_entries, 0, sizeof(unsigned long long), st));
     const unsigned tb = grid_for((size_t)ntris, 256, 8);
     if (rays)
         rect_kernel<true><<<tb, 256, 0, st>>>(d_xyz, d_tris, ntris, I.P, I.g, I.tv, d_rect,
-                                              d_count);
+                                              d_entries);
     else
         rect_kernel<false><<<tb, 256, 0, st>>>(d_xyz, d_tris, ntris, I.P, I.g, I.tv, d_rect,
-                                               d_count);
+                                               d_entries);
     CMG_HIP_LAUNCH();
-    // start = exclusive prefix of the counts; the total (start[ncell]) is the last start
-    // plus the last count. Counts are 32-bit; more than 2^31 entries in all is rejected.
+    // The list total is summed over the rectangles in 64 bits and checked before any
+    // per-cell count, scan or fill: the counts and list offsets are 32-bit, so more than
+    // 2^31 entries in all is rejected.
+    CMG_HIP(hipMemcpyAsync(&I.entries, d_entries, sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    if (I.entries >= (1ULL << 31))
+        throw InvalidInput("carve: cell lists exceed 2^31 entries");
+    if (rays)
+        count_kernel<true><<<tb, 256, 0, st>>>(ntris, I.P, I.g, d_rect, d_count);
+    else
+        count_kernel<false><<<tb, 256, 0, st>>>(ntris, I.P, I.g, d_rect, d_count);
+    CMG_HIP_LAUNCH();
+    // start = exclusive prefix of the counts, start[ncell] = the total
     size_t scan_tmp = 0;
     CMG_HIP(rocprim::exclusive_scan(nullptr, scan_tmp, d_count, I.start, 0u, (size_t)I.ncell,
                                     rocprim::plus<unsigned>(), st));
     void* tmp = I.dalloc<char>(scan_tmp);
     CMG_HIP(rocprim::exclusive_scan(tmp, scan_tmp, d_count, I.start, 0u, (size_t)I.ncell,
                                     rocprim::plus<unsigned>(), st));
-    unsigned last[2];
-    CMG_HIP(hipMemcpyAsync(&last[0], I.start + I.ncell - 1, sizeof(unsigned),
-                           hipMemcpyDeviceToHost, st));
-    CMG_HIP(hipMemcpyAsync(&last[1], d_count + I.ncell - 1, sizeof(unsigned),
-                           hipMemcpyDeviceToHost, st));
-    CMG_HIP(hipStreamSynchronize(st));
-    I.entries = (unsigned long long)last[0] + last[1];
-    if (I.entries >= (1ULL << 31))
-        throw InvalidInput("carve: cell lists exceed 2^31 entries");
     {
         const unsigned tot = (unsigned)I.entries;
         CMG_HIP(hipMemcpyAsync(I.start + I.ncell, &tot, sizeof(unsigned),

@@ -59,9 +59,11 @@ void launch_signs_soa(int npts, int mode, const double* soa, size_t n, size_t ld
 void launch_signs_indexed(int npts, int mode, const double* xyz, const int* idx, size_t n,
                           int8_t* out, const PredPlan& p, const PredWorkspace& ws,
                           hipStream_t stream);
-void launch_o3_values_indexed(const double* xyz, const int* quads, size_t n, double* out,
-                              const PredPlan& p, const PredWorkspace& ws,
-                              hipStream_t stream);
+// Values (npts 4 = orient3d, 5 = insphere): the naive FP64 determinant when nonzero,
+// else the exact determinant's leading component. Plan with pred_plan(n, 1).
+void launch_values_indexed(int npts, const double* xyz, const int* idx, size_t n,
+                           double* out, const PredPlan& p, const PredWorkspace& ws,
+                           hipStream_t stream);
 // Fills ctr->fp32/fp64/exact from the per-wave counts.
 void launch_pred_stats(const PredWorkspace& ws, hipStream_t stream);
 

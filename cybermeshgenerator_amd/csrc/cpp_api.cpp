@@ -131,12 +131,12 @@ double insphere(const Point3& a, const Point3& b, const Point3& c, const Point3&
     const double xyz[15] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z,
                             d.x, d.y, d.z, e.x, e.y, e.z};
     const int q[5] = {0, 1, 2, 3, 4};
-    signed char s = 0;
+    double v = 0.0;
     char err[512] = {0};
-    if (cmg_insphere_signs(xyz, 5, q, 1, CMG_PRED_COMPAT, &s, err, sizeof err) != CMG_OK)
+    if (cmg_insphere_values(xyz, 5, q, 1, &v, err, sizeof err) != CMG_OK)
         throw std::runtime_error(err);
     g_last = backend::Backend::Hip;
-    return static_cast<double>(s);  // the sign; magnitudes are not part of this path
+    return v;
 }
 
 std::vector<double> orient3d_batch(std::span<const Point3> pts,
@@ -295,10 +295,19 @@ std::size_t threshold_for(Op op) noexcept {
     return static_cast<std::size_t>(-1);
 }
 
+// dispatch.cpp:39-63: below the op's threshold the CPU serves the call; at or above it,
+// the device when one is present. The decision is recorded for last_backend().
 bool should_offload(Op op, std::size_t problem_size) noexcept {
-    (void)op;
-    (void)problem_size;
-    return cmg_hip_available() != 0;
+    if (problem_size < threshold_for(op)) {
+        g_last = Backend::Cpu;
+        return false;
+    }
+    if (cmg_hip_available() != 0) {
+        g_last = Backend::Hip;
+        return true;
+    }
+    g_last = Backend::Cpu;
+    return false;
 }
 
 Backend last_backend() noexcept { return g_last; }

@@ -43,7 +43,7 @@ __host__ __device__ __forceinline__ void two_prod(double a, double b, double& x,
 }
 
 // h = e + f. h must hold en + fn; h may not alias e or f.
-__device__ inline int exp_sum(const double* e, int en, const double* f, int fn, double* h) {
+__host__ __device__ inline int exp_sum(const double* e, int en, const double* f, int fn, double* h) {
     int i = 0, j = 0, k = 0;
     double q;
     // smallest-magnitude component first
@@ -63,7 +63,7 @@ __device__ inline int exp_sum(const double* e, int en, const double* f, int fn, 
 }
 
 // h = b * e. h must hold 2*en; h may not alias e.
-__device__ inline int exp_scale(const double* e, int en, double b, double* h) {
+__host__ __device__ inline int exp_scale(const double* e, int en, double b, double* h) {
     int k = 0;
     double q, t;
     two_prod(e[0], b, q, t);
@@ -81,7 +81,7 @@ __device__ inline int exp_scale(const double* e, int en, double b, double* h) {
 }
 
 // In-place Compress: same value, nonadjacent components, usually very few.
-__device__ inline int exp_compress(double* e, int n) {
+__host__ __device__ inline int exp_compress(double* e, int n) {
     if (n <= 1) return n;
     int bottom = n - 1;
     double q = e[bottom];
@@ -106,17 +106,17 @@ __device__ inline int exp_compress(double* e, int n) {
     return top;
 }
 
-__device__ __forceinline__ void exp_negate(double* e, int n) {
+__host__ __device__ __forceinline__ void exp_negate(double* e, int n) {
     for (int i = 0; i < n; ++i) e[i] = -e[i];
 }
 
-__device__ __forceinline__ int exp_sign(const double* e, int n) {
+__host__ __device__ __forceinline__ int exp_sign(const double* e, int n) {
     double t = e[n - 1];
     return t > 0.0 ? 1 : (t < 0.0 ? -1 : 0);
 }
 
 // exact a - b as a 1- or 2-component expansion
-__device__ __forceinline__ int exp_diff(double a, double b, double* h) {
+__host__ __device__ __forceinline__ int exp_diff(double a, double b, double* h) {
     double x, y;
     two_sum(a, -b, x, y);
     if (y != 0.0) {
@@ -129,7 +129,7 @@ __device__ __forceinline__ int exp_diff(double a, double b, double* h) {
 }
 
 // h = e * f, e and f short (<= 2 components each): at most 8 components.
-__device__ inline int exp_mul_small(const double* e, int en, const double* f, int fn,
+__host__ __device__ inline int exp_mul_small(const double* e, int en, const double* f, int fn,
                                     double* h) {
     double a[4], b[4];
     int an = exp_scale(e, en, f[0], a);
@@ -144,7 +144,7 @@ __device__ inline int exp_mul_small(const double* e, int en, const double* f, in
 // h = e * f where f is short (fn components) and e has en <= ECAP components; the
 // result is compressed after every partial sum. Returns -1 if HCAP would be exceeded.
 template <int HCAP>
-__device__ inline int exp_mul(const double* e, int en, const double* f, int fn, double* h,
+__host__ __device__ inline int exp_mul(const double* e, int en, const double* f, int fn, double* h,
                               double* tmp /* HCAP */, double* part /* 2*en */) {
     if (2 * en > HCAP) return -1;
     int hn = exp_scale(e, en, f[0], h);
@@ -161,7 +161,7 @@ __device__ inline int exp_mul(const double* e, int en, const double* f, int fn, 
 }
 
 // 2x2 minor p*q - r*s of 1-2 component entries (<= 16 components, compressed).
-__device__ inline int exp_minor(const double* p, int pn, const double* q, int qn,
+__host__ __device__ inline int exp_minor(const double* p, int pn, const double* q, int qn,
                                 const double* r, int rn, const double* s, int sn,
                                 double* out /* 16 */) {
     double a[8], b[8];
@@ -174,7 +174,7 @@ __device__ inline int exp_minor(const double* p, int pn, const double* q, int qn
 
 // acc += sgn * z * m  (z: 1-2 components, m: <= 16). Returns false on overflow.
 template <int CAP>
-__device__ inline bool exp_accumulate_zm(double* acc, int& an, const double* z, int zn,
+__host__ __device__ inline bool exp_accumulate_zm(double* acc, int& an, const double* z, int zn,
                                          const double* m, int mn, int sgn) {
     double t[64];
     double tmp[64];
@@ -195,7 +195,7 @@ __device__ inline bool exp_accumulate_zm(double* acc, int& an, const double* z, 
 // component (same sign as the determinant, within a relative 2^-52 of its value).
 // Returns false if CAP is too small.
 template <int CAP>
-__device__ inline bool orient3d_exact(const double* pa, const double* pb, const double* pc,
+__host__ __device__ inline bool orient3d_exact(const double* pa, const double* pb, const double* pc,
                                       const double* pd, double& top) {
     double adx[2], ady[2], adz[2], bdx[2], bdy[2], bdz[2], cdx[2], cdy[2], cdz[2];
     int nadx = exp_diff(pa[0], pd[0], adx), nady = exp_diff(pa[1], pd[1], ady),
@@ -893,7 +893,7 @@ __host__ __device__ inline bool insphere_sign_dw(const double* pa, const double*
 // Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small. FAST = false
 // skips the register-only stages (for lanes that already failed them).
 template <int CAP, bool FAST = true>
-__device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
+__host__ __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
                                           const double* pc, const double* pd) {
     int fast;
     if (FAST && orient3d_sign_exact_diffs(pa, pb, pc, pd, fast)) return fast;
@@ -905,14 +905,13 @@ __device__ inline int orient3d_exact_sign(const double* pa, const double* pb,
     return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
 }
 
-// Exact sign of insphere(pa, pb, pc, pd, pe); kOverflow if a capacity is too small.
-// CCAP bounds a 3x3 cofactor, LCAP a lifted product.
+// Exact insphere(pa, pb, pc, pd, pe) as an expansion; `top` receives its largest
+// component (same sign as the determinant, within a relative 2^-52 of its value).
+// Returns false if a capacity is too small: CCAP bounds a 3x3 cofactor, LCAP a lifted
+// product.
 template <int CCAP, int LCAP>
-__device__ inline int insphere_exact_sign(const double* pa, const double* pb,
-                                          const double* pc, const double* pd,
-                                          const double* pe) {
-    int fixed;  // differences within 126 bits: exact in registers
-    if (insphere_sign_fixed_wide(pa, pb, pc, pd, pe, fixed)) return fixed;
+__host__ __device__ inline bool insphere_exact(const double* pa, const double* pb, const double* pc,
+                                      const double* pd, const double* pe, double& top) {
     double ex[4][2], ey[4][2], ez[4][2];
     int nx[4], ny[4], nz[4];
     const double* P[4] = {pa, pb, pc, pd};
@@ -950,7 +949,7 @@ __device__ inline int insphere_exact_sign(const double* pa, const double* pb,
             int mn = exp_minor(ex[E.u], nx[E.u], ey[E.v], ny[E.v], ex[E.v], nx[E.v],
                                ey[E.u], ny[E.u], m);
             if (!exp_accumulate_zm<CCAP>(cof, cn, ez[E.z], nz[E.z], m, mn, E.s))
-                return kOverflow;
+                return false;
         }
         // lift = x^2 + y^2 + z^2 of the term's vertex
         double lift[24];
@@ -966,11 +965,11 @@ __device__ inline int insphere_exact_sign(const double* pa, const double* pb,
         }
         // acc += sgn * lift * cof, one lift component at a time
         for (int j = 0; j < ln; ++j) {
-            if (2 * cn > LCAP) return kOverflow;
+            if (2 * cn > LCAP) return false;
             double part[2 * CCAP];
             int pn = exp_scale(cof, cn, lift[j], part);
             if (T.sgn < 0) exp_negate(part, pn);
-            if (an + pn > LCAP) return kOverflow;
+            if (an + pn > LCAP) return false;
             double sum[LCAP];
             int sn = exp_sum(acc, an, part, pn, sum);
             sn = exp_compress(sum, sn);
@@ -978,7 +977,20 @@ __device__ inline int insphere_exact_sign(const double* pa, const double* pb,
             an = sn;
         }
     }
-    return exp_sign(acc, an);
+    top = acc[an - 1];
+    return true;
+}
+
+// Exact sign of insphere(pa, pb, pc, pd, pe); kOverflow if a capacity is too small.
+template <int CCAP, int LCAP>
+__host__ __device__ inline int insphere_exact_sign(const double* pa, const double* pb,
+                                          const double* pc, const double* pd,
+                                          const double* pe) {
+    int fixed;  // differences within 126 bits: exact in registers
+    if (insphere_sign_fixed_wide(pa, pb, pc, pd, pe, fixed)) return fixed;
+    double top;
+    if (!insphere_exact<CCAP, LCAP>(pa, pb, pc, pd, pe, top)) return kOverflow;
+    return top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
 }
 
 }  // namespace cmg::hip

@@ -184,6 +184,8 @@ LocateStats locate_points(const double* d_xyz, long long npts, const int* d_tets
     if (nq == 0) return {};
     if (ntets < 1) throw InvalidInput("locate: no tetrahedra");
     if (ntets >= (1LL << 31)) throw InvalidInput("locate: more than 2^31 - 1 tetrahedra");
+    // the Morton order carries 32-bit query indices
+    if (nq >= (1LL << 32)) throw InvalidInput("locate: n must be < 2^32 queries");
     if (!d_xyz || !d_tets || !d_nbrs || !d_q || !d_out) throw InvalidInput("locate: null pointer");
     if (reinterpret_cast<uintptr_t>(d_tets) % 16 || reinterpret_cast<uintptr_t>(d_nbrs) % 16)
         throw InvalidInput("locate: tets and nbrs must be 16-byte aligned");

@@ -608,12 +608,13 @@ __global__ __launch_bounds__(64) void exact_kernel(Src src, int8_t* __restrict__
     }
 }
 
-// orient3d values (cmg::robust::orient3d_batch): the naive FP64 determinant when it is
-// nonzero, else the exact determinant's leading expansion component (same sign).
-template <class Src>
-__global__ __launch_bounds__(256) void o3_values_kernel(Src src, size_t n,
-                                                        double* __restrict__ out,
-                                                        PredWorkspace ws, unsigned iters) {
+// Predicate values (cmg::robust::orient3d_batch, cmg::robust::insphere): the naive FP64
+// determinant when it is nonzero, else the exact determinant's leading expansion
+// component (same sign).
+template <int NPTS, class Src>
+__global__ __launch_bounds__(256) void values_kernel(Src src, size_t n,
+                                                     double* __restrict__ out,
+                                                     PredWorkspace ws, unsigned iters) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned wave = (unsigned)(tid / 64);
@@ -623,9 +624,9 @@ __global__ __launch_bounds__(256) void o3_values_kernel(Src src, size_t n,
         const size_t i = tid + (size_t)it * stride;
         bool u = false;
         if (i < n) {
-            double v[12];
+            double v[NPTS * 3];
             src.load(i, v);
-            double det = o3_naive(v, nullptr);
+            double det = NPTS == 4 ? o3_naive(v, nullptr) : is_naive(v, nullptr);
             u = !(det > 0.0 || det < -0.0);
             out[i] = det;
         }
@@ -639,19 +640,22 @@ __global__ __launch_bounds__(256) void o3_values_kernel(Src src, size_t n,
     }
 }
 
-template <class Src>
-__global__ __launch_bounds__(64) void o3_values_exact_kernel(Src src,
-                                                             double* __restrict__ out,
-                                                             PredWorkspace ws) {
+template <int NPTS, class Src>
+__global__ __launch_bounds__(64) void values_exact_kernel(Src src,
+                                                          double* __restrict__ out,
+                                                          PredWorkspace ws) {
     const uint32_t total = ws.qpref[ws.nwaves];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
         {
             const uint32_t i = queued(ws, k);
-            double v[12];
+            double v[NPTS * 3];
             src.load(i, v);
             double top = 0.0;
-            if (!orient3d_exact<320>(v, v + 3, v + 6, v + 9, top)) {
+            const bool ok =
+                NPTS == 4 ? orient3d_exact<320>(v, v + 3, v + 6, v + 9, top)
+                          : insphere_exact<200, 1024>(v, v + 3, v + 6, v + 9, v + 12, top);
+            if (!ok) {
                 atomicAdd(&ws.ctr->failed, 1u);
                 top = 0.0;
             }
@@ -821,21 +825,34 @@ void launch_signs_indexed(int npts, int mode, const double* xyz, const int* idx,
 #undef CMG_FILTER_IDX
 }
 
-void launch_o3_values_indexed(const double* xyz, const int* quads, size_t n, double* out,
-                              const PredPlan& p, const PredWorkspace& ws,
-                              hipStream_t stream) {
+void launch_values_indexed(int npts, const double* xyz, const int* idx, size_t n,
+                           double* out, const PredPlan& p, const PredWorkspace& ws,
+                           hipStream_t stream) {
     zero_counters_kernel<<<1, 64, 0, stream>>>(ws.ctr);
     CMG_HIP_LAUNCH();
     if (n == 0) return;
-    IndexedSource<4> src{xyz, quads};
-    o3_values_kernel<IndexedSource<4>>
-        <<<p.grid, p.block, 0, stream>>>(src, n, out, ws, p.iters);
-    CMG_HIP_LAUNCH();
-    queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
-    CMG_HIP_LAUNCH();
     const unsigned g = (unsigned)cu_count() * 8;
-    o3_values_exact_kernel<IndexedSource<4>><<<g, 64, 0, stream>>>(src, out, ws);
-    CMG_HIP_LAUNCH();
+    if (npts == 4) {
+        IndexedSource<4> src{xyz, idx};
+        values_kernel<4, IndexedSource<4>><<<p.grid, p.block, 0, stream>>>(src, n, out, ws,
+                                                                            p.iters);
+        CMG_HIP_LAUNCH();
+        queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
+        CMG_HIP_LAUNCH();
+        values_exact_kernel<4, IndexedSource<4>><<<g, 64, 0, stream>>>(src, out, ws);
+        CMG_HIP_LAUNCH();
+    } else {
+        IndexedSource<5> src{xyz, idx};
+        values_kernel<5, IndexedSource<5>><<<p.grid, p.block, 0, stream>>>(src, n, out, ws,
+                                                                            p.iters);
+        CMG_HIP_LAUNCH();
+        queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
+        CMG_HIP_LAUNCH();
+        // the wide-capacity insphere expansion spills heavily: a small grid
+        const unsigned gw = cu_count() / 4 > 0 ? cu_count() / 4 : 1;
+        values_exact_kernel<5, IndexedSource<5>><<<gw, 64, 0, stream>>>(src, out, ws);
+        CMG_HIP_LAUNCH();
+    }
 }
 
 }  // namespace cmg::hip

@@ -21,7 +21,8 @@
 // Keys compare as doubles (-0 == +0, as the map's operator< treats them) and hash on
 // bits with -0 folded to +0; the point written is the first occurrence's own value.
 // A NaN coordinate makes its vertex unequal to every vertex (the reference's map has no
-// defined behaviour for NaN keys).
+// defined behaviour for NaN keys): such a vertex skips the hash tables and is its own
+// representative.
 //
 // Bound: HBM and atomic latency; the table and soup accesses of the insert and rep
 // kernels are random. Work buffers come from the library's per-device scratch (reused
@@ -169,13 +170,18 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, lo
     for (long long base = (long long)blockIdx.x * 256; base < nv;
          base += (long long)gridDim.x * 256) {
         const long long v = base + tid;
-        const bool ok = v < nv;
+        const bool in = v < nv;
         s_key[tid] = kEmpty;
         s_key[tid + 256] = kEmpty;
         unsigned h = 0;
-        if (ok) {
+        bool ok = false;  // a vertex that takes part in the merge
+        if (in) {
             for (int a = 0; a < 3; ++a) s_xyz[3 * tid + a] = soup[3 * v + a];
-            h = hash_key(soup + 3 * v);
+            // a vertex with a NaN coordinate equals no vertex: it stays out of both
+            // tables (where equal NaN bit patterns would all probe one cluster)
+            ok = !(__builtin_isnan(s_xyz[3 * tid]) || __builtin_isnan(s_xyz[3 * tid + 1]) ||
+                   __builtin_isnan(s_xyz[3 * tid + 2]));
+            if (ok) h = hash_key(soup + 3 * v);
         }
         __syncthreads();
         unsigned ls = 0;
@@ -196,19 +202,21 @@ __global__ __launch_bounds__(256) void weld_insert_kernel(const double* soup, lo
         const unsigned rep = ok ? s_key[ls] : 0u;
         if (ok && rep == (unsigned)tid) s_slot[tid] = global_insert(soup, v, h, tab, mask);
         __syncthreads();
-        if (ok) slot[v] = s_slot[rep];
+        if (in) slot[v] = ok ? s_slot[rep] : kEmpty;  // kEmpty: its own representative
         __syncthreads();  // s_key, s_slot and s_xyz are rewritten by the next round
     }
 }
 
 // After every insert: each slot holds the first occurrence of its key, so a vertex's
 // representative is the final value of the slot the insert recorded for it (rep holds the
-// slot on entry and the representative on exit).
+// slot on entry and the representative on exit; a vertex with a NaN coordinate recorded
+// no slot and is its own representative).
 __global__ __launch_bounds__(256) void weld_rep_kernel(long long nv, const unsigned* tab,
                                                        unsigned* rep, unsigned* first) {
     for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < nv;
          v += (long long)gridDim.x * blockDim.x) {
-        const unsigned r = tab[rep[v]];
+        const unsigned s = rep[v];
+        const unsigned r = s == kEmpty ? (unsigned)v : tab[s];
         rep[v] = r;
         first[v] = r == (unsigned)v;
     }

@@ -368,6 +368,33 @@ def tuples_insphere(n: int, seed: int = 22, degenerate: float = 0.10):
     return np.ascontiguousarray(soa), kind
 
 
+def tuples_wide_spread(n: int, npts: int, seed: int = 23):
+    """(3*npts, n) float64 SoA tuples whose coordinates carry random 53-bit significands
+    and independent random exponents in [-300, 300) (orient3d) or [-150, 150) (insphere),
+    the ranges in which expansion arithmetic is exact: coordinate differences span far
+    more than 126 bits, so the fixed-point stages decline and the expansion passes decide.
+    Every tuple repeats a point (orient3d: b = a; insphere: e = a), so its determinant is
+    exactly 0; in the second half one coordinate of the repeated copy is then moved by
+    +-1 ulp (tiny nonzero determinants)."""
+    lim = 300 if npts == 4 else 150
+    rng = np.random.default_rng(seed)
+    q = 3 * npts
+
+    def spread(shape):
+        m = rng.integers(2 ** 52, 2 ** 53, size=shape).astype(np.float64)
+        sg = np.where(rng.random(shape) < 0.5, -1.0, 1.0)
+        return sg * np.ldexp(m, rng.integers(-lim, lim, size=shape) - 53)
+
+    t = spread((n, q))
+    dst = 3 if npts == 4 else 12          # the repeated copy: b (orient3d) or e (insphere)
+    t[:, dst:dst + 3] = t[:, 0:3]
+    rows = np.arange(n // 2, n)
+    col = dst + rng.integers(0, 3, size=rows.size)
+    up = rng.random(rows.size) < 0.5
+    t[rows, col] = np.nextafter(t[rows, col], np.where(up, np.inf, -np.inf))
+    return np.ascontiguousarray(t.T)
+
+
 def binary_stl_bytes(nu: int = 2048, nv: int = 1024, seed: int = 6) -> bytes:
     """A binary STL file (80-byte header, uint32 count, 50-byte records) of a closed torus
     (R = 0.3, r = 0.12, centred at 0.5) tessellated nu x nv (2 * nu * nv triangles, every

@@ -131,7 +131,9 @@ expected<std::vector<Point3>, MeshError> sdf_backward(const PLC& plc, const Voxe
 namespace robust {
 
 // robust.hpp:22-46. Values and signs follow the shipped predicates (zero static
-// filter: the naive FP64 determinant when nonzero, the exact sign when it is 0).
+// filter): orient3d, insphere and orient3d_batch return the naive FP64 determinant,
+// bit-identical, when it is nonzero, and otherwise the exact determinant's leading
+// expansion component (the exact sign, 0 on a true zero).
 void ensure_initialized();
 double orient3d(const Point3& a, const Point3& b, const Point3& c, const Point3& d);
 double insphere(const Point3& a, const Point3& b, const Point3& c, const Point3& d,
@@ -181,9 +183,10 @@ namespace backend {
 enum class Backend { Cpu, Cuda, OpenCL, Metal, Hip };
 enum class Op { OrientFilter, HilbertSort, PointLocation, QualityScan, Voxelize };
 
-// True when a HIP device is present (this library has no CPU kernels, so it serves
-// every size; the reference's per-op thresholds, dispatch.cpp:23-30, are kept for
-// callers that dispatch between the two).
+// dispatch.cpp:39-63: false below threshold_for(op) (dispatch.cpp:23-30) and when no HIP
+// device is present, true otherwise; records the decision for last_backend(). The entry
+// points of this library run on the GPU at every size (it has no CPU kernels); the
+// decision is for callers that dispatch between a CPU path and this one.
 bool should_offload(Op op, std::size_t problem_size) noexcept;
 std::size_t threshold_for(Op op) noexcept;
 Backend last_backend() noexcept;

@@ -100,6 +100,13 @@ cmg_status cmg_orient3d_signs(const double* xyz, size_t npts, const int* quads, 
 cmg_status cmg_orient3d_values(const double* xyz, size_t npts, const int* quads, size_t n,
                                double* out, char* errbuf, size_t errbuf_len);
 
+/* Batched insphere values: cmg::robust::insphere (robust.cpp:44-50) over index
+ * quintuples (a,b,c,d,e). The naive FP64 lifted determinant in the reference's operation
+ * order (predicates.cpp:4063-4106), bit-identical, when nonzero; when it is exactly 0 the
+ * exact determinant's leading expansion component (exact sign). */
+cmg_status cmg_insphere_values(const double* xyz, size_t npts, const int* quints, size_t n,
+                               double* out, char* errbuf, size_t errbuf_len);
+
 /* Batched insphere signs over index quintuples (a,b,c,d,e) into AoS points; the batched
  * form of cmg::robust::insphere (robust.cpp:44-50). */
 cmg_status cmg_insphere_signs(const double* xyz, size_t npts, const int* quints, size_t n,
@@ -389,7 +396,8 @@ cmg_status cmg_hip_quality_report(const double* xyz, size_t npts, const int* tet
  * (device) = the tet found, identical to the reference's walk from the same start;
  * steps[i] (device, optional) = the walk length, -1 where brute force answered;
  * *brute (optional) = how many walks did. Synchronises `stream`; a walk that meets an
- * index out of range fails the call with CMG_ERR_INVALID_INPUT. */
+ * index out of range fails the call with CMG_ERR_INVALID_INPUT, and so does nq >= 2^32
+ * (queries are ordered by 32-bit index). */
 cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const int* nbrs,
                           size_t ntets, const double* q, size_t nq, const int* start,
                           int start_tet, int* out, int* steps, uint64_t* brute,

@@ -6,6 +6,9 @@
 // differences): decided (0/1), sign. orient3d tuples are pa pb pc pd (12 doubles); the
 // stages are the exact-difference sum, the double-word bound and fixed point with 62- and
 // 126-bit differences. Unused slots read 0.
+// Modes orient3d-exp and insphere-exp replay the exact pass instead: fixed point with 62-
+// and 126-bit differences, then the expansion arithmetic with the first pass's small
+// capacities and with the wide re-run's capacities (decided 0 = capacity exceeded).
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -17,7 +20,8 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "usage: %s insphere|orient3d in.bin out.bin\n", argv[0]);
         return 2;
     }
-    const bool ins = std::strcmp(argv[1], "insphere") == 0;
+    const bool ins = std::strncmp(argv[1], "insphere", 8) == 0;
+    const bool exp = std::strstr(argv[1], "-exp") != nullptr;
     const int per = ins ? 15 : 12;
     std::FILE* f = std::fopen(argv[2], "rb");
     if (!f) return 2;
@@ -31,7 +35,24 @@ int main(int argc, char** argv) {
         const double* t = v.data() + per * i;
         int s[4] = {0, 0, 0, 0};
         bool ok[4] = {false, false, false, false};
-        if (ins) {
+        if (exp) {
+            double top = 0.0;
+            if (ins) {
+                ok[0] = cmg::hip::insphere_sign_fixed(t, t + 3, t + 6, t + 9, t + 12, s[0]);
+                ok[1] = cmg::hip::insphere_sign_fixed_wide(t, t + 3, t + 6, t + 9, t + 12, s[1]);
+                ok[2] = cmg::hip::insphere_exact<40, 96>(t, t + 3, t + 6, t + 9, t + 12, top);
+                s[2] = top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
+                ok[3] = cmg::hip::insphere_exact<200, 1024>(t, t + 3, t + 6, t + 9, t + 12, top);
+                s[3] = top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
+            } else {
+                ok[0] = cmg::hip::orient3d_sign_fixed(t, t + 3, t + 6, t + 9, s[0]);
+                ok[1] = cmg::hip::orient3d_sign_fixed_wide(t, t + 3, t + 6, t + 9, s[1]);
+                ok[2] = cmg::hip::orient3d_exact<40>(t, t + 3, t + 6, t + 9, top);
+                s[2] = top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
+                ok[3] = cmg::hip::orient3d_exact<320>(t, t + 3, t + 6, t + 9, top);
+                s[3] = top > 0.0 ? 1 : (top < 0.0 ? -1 : 0);
+            }
+        } else if (ins) {
             ok[0] = cmg::hip::insphere_sign_fixed(t, t + 3, t + 6, t + 9, t + 12, s[0]);
             ok[1] = cmg::hip::insphere_sign_dw(t, t + 3, t + 6, t + 9, t + 12, s[1]);
             ok[2] = cmg::hip::insphere_sign_fixed_wide(t, t + 3, t + 6, t + 9, t + 12, s[2]);

@@ -105,6 +105,14 @@ int main() {
     CHECK(o != 0.0 && (o > 0) != (robust::orient3d(pa, pb, pd, pc) > 0));
     CHECK((robust::insphere(pa, pb, pc, pd, {0.25, 0.25, 0.25}) > 0) !=
           (robust::insphere(pa, pb, pc, pd, {100, 100, 100}) > 0));
+    // robust.cpp:44-50 returns the determinant's value. For this tetrahedron (circumcentre
+    // (1/2,1/2,1/2), r^2 = 3/4) insphere = orient3d(a,b,c,d) * (r^2 - |e - centre|^2),
+    // and every operation of the naive formula is exact on these dyadic inputs.
+    CHECK(o == -1.0);
+    CHECK(robust::insphere(pa, pb, pc, pd, {0.25, 0.25, 0.25}) == -0.5625);
+    CHECK(robust::insphere(pa, pb, pc, pd, {0.5, 0.5, 0.5}) == -0.75);
+    CHECK(robust::insphere(pa, pb, pc, pd, {100, 100, 100}) == 29700.0);
+    CHECK(robust::insphere(pa, pb, pc, pd, {1, 1, 0}) == 0.0);  // on the sphere
     // test_foundation.cpp:94-106 — batch equals per-item
     std::vector<Point3> pts{{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, -1}};
     std::vector<Tetrahedron> cands{{0, 1, 2, 3}, {0, 1, 2, 4}};
@@ -190,7 +198,15 @@ int main() {
         for (int h : q.dihedral_histogram) hist += h;
         CHECK(hist == 6 && q.worst_tets.size() == 1 && q.worst_tets[0] == 0);
     }
-    CHECK(backend::should_offload(backend::Op::OrientFilter, 10));
+    // dispatch.cpp:39-63: below an op's threshold the CPU serves it, and last_backend
+    // records the decision
+    CHECK(!backend::should_offload(backend::Op::OrientFilter, 10));
+    CHECK(backend::last_backend() == backend::Backend::Cpu);
+    CHECK(backend::should_offload(backend::Op::OrientFilter, std::size_t(1) << 16));
+    CHECK(backend::last_backend() == backend::Backend::Hip);
+    CHECK(!backend::should_offload(backend::Op::QualityScan, (std::size_t(1) << 17) - 1));
+    CHECK(backend::last_backend() == backend::Backend::Cpu);
+    CHECK(backend::should_offload(backend::Op::PointLocation, std::size_t(1) << 20));
     std::printf("C++ API: %s (%d failures)\n", failures ? "FAIL" : "OK", failures);
     return failures ? 1 : 0;
 }

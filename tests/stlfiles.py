@@ -2,6 +2,8 @@
 and write), and seeded surfaces with shared vertices."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 
@@ -63,3 +65,24 @@ def write_obj(path, points, polygons, extra: str = "") -> None:
             f.write("v {!r} {!r} {!r}\n".format(*(float(x) for x in p)))
         for poly in polygons:
             f.write("f " + " ".join(str(int(v) + 1) for v in poly) + "\n")
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# examples/bunny/README.md:70-72 (cybermesh.voxelize(read_plc(bunny), resolution=48),
+# default pad 1, examples/bunny/run_bunny.py:43,194,297-298)
+BUNNY_RES48_CELLS = 24667
+BUNNY_RES48_VOLUME = 280129
+
+
+def bunny_stl(directory) -> str:
+    """Decompresses the reference's bunny surface (tests/golden) into `directory` and
+    returns the .stl path."""
+    import lzma
+    path = os.path.join(str(directory), "Stanford_Bunny_sample.stl")
+    if not os.path.exists(path):
+        with lzma.open(os.path.join(GOLDEN, "stanford_bunny_sample.stl.xz")) as f:
+            data = f.read()
+        with open(path, "wb") as g:
+            g.write(data)
+    return path

@@ -193,3 +193,99 @@ def test_backward_and_nearest_need_a_forward(gpu):
     b.backward(g, graph=False)
     b.nearest()
     torch.cuda.synchronize()
+
+
+def test_c5_full_batch_against_the_oracle(gpu):
+    """C5 at full size: 64 uneven meshes (2,085,816 triangles, 202,578,542 cells), SDF
+    forward + backward as one captured hipGraph (captured, then replayed), against the
+    oracle mesh by mesh:
+      * signs of every cell equal the oracle's occupancy (voxelize.cpp:331-358);
+      * every cell's distance is, within the FP32 margin, the FP64 distance to the
+        triangle the engine reports as nearest;
+      * three z-slabs per mesh against the oracle's FP64 search: bit-identical within one
+        cell of the surface, within the FP32 bound elsewhere, and the reported triangle
+        a nearest one within twice the margin;
+      * the adjoint on six meshes, checked as at C4: the FP64 adjoint of the same
+        triangles, the FP64 adjoint, and FP64 central differences."""
+    import torch
+    from cybermeshgenerator_amd.batch import SdfBatch
+    cases = W.batch_c5(seed=5)
+    print(f"C5: {len(cases)} meshes", flush=True)
+    b = SdfBatch([(c.points, c.tris) for c in cases], [c.resolution for c in cases],
+                 [c.pad for c in cases])
+    assert b.cells == 202578542
+    gsp = W.upstream_grad(b.cells, seed=50)
+    gs = torch.from_numpy(gsp).cuda()
+    sdf = torch.empty(b.cells, dtype=torch.float32, device="cuda")
+    grad = torch.empty((b.points.shape[0], 3), dtype=torch.float32, device="cuda")
+    b.step(gs, sdf=sdf, grad=grad, graph=True)          # capture (and run)
+    sdf.fill_(7.0)
+    grad.fill_(7.0)
+    b.step(gs, sdf=sdf, grad=grad, graph=True)          # replay
+    torch.cuda.synchronize()
+    assert b.status()["overflow"] == 0
+    near = b.nearest().cpu().numpy()
+    sdf_h = sdf.cpu().numpy()
+    gparts = b.mesh_points(grad.cpu().numpy())
+    del sdf, grad
+    worst_far, worst_gap, flips, ncell = 0.0, 0.0, 0, 0
+    for m, c in enumerate(cases):
+        g = oracle.make_grid(c.points, c.resolution, c.pad)
+        gr = b.grids[m]
+        assert tuple(gr.dims) == g.dims and gr.ncell == g.ncell
+        sl = slice(gr.offset, gr.offset + gr.ncell)
+        got, nm = sdf_h[sl], near[sl]
+        occ = oracle.occupancy(c.points, c.tris, g)
+        assert np.array_equal(np.signbit(got), occ == 1), m
+        assert (nm >= 0).all() and (nm < c.tris.shape[0]).all(), m
+        margin = _margin(g)
+        d_used = np.sqrt(oracle.sdf_dist2_to(c.points, c.tris, g, nm))
+        assert np.abs(d_used - np.abs(got.astype(np.float64))).max() <= 3 * margin, m
+        nx, ny, nz = g.dims
+        for k in (nz // 4, nz // 2, (3 * nz) // 4):
+            first, count = k * nx * ny, nx * ny
+            want, best, arg = oracle.sdf_bvh(c.points, c.tris, g, occ, first, count)
+            r = slice(first, first + count)
+            near_cells = np.sqrt(best[r]) <= g.spacing
+            assert np.array_equal(got[r][near_cells], want[r][near_cells]), (m, k)
+            far = np.abs(got[r].astype(np.float64) - want[r]).max()
+            assert far <= 4e-6 * max(g.dims) * g.spacing, (m, k, far)
+            gap = d_used[r] - np.sqrt(best[r])
+            assert gap.min() >= 0 and gap.max() <= 2 * margin, (m, k)
+            worst_far = max(worst_far, far / g.spacing)
+            worst_gap = max(worst_gap, gap.max() / g.spacing)
+            flips += int((nm[r] != arg[r]).sum())
+            ncell += count
+    print(f"C5 slabs: {ncell} cells, max FP32 error {worst_far:.2e} cells, max nearest gap "
+          f"{worst_gap:.2e} cells, other nearest triangle on {flips / ncell:.2%}")
+    order = np.argsort([bg.ncell for bg in b.grids])
+    picks = list(order[:4]) + [order[len(order) // 3], order[len(order) // 2]]
+    for m in picks:
+        c = cases[m]
+        g = oracle.make_grid(c.points, c.resolution, c.pad)
+        gr = b.grids[m]
+        sl = slice(gr.offset, gr.offset + gr.ncell)
+        occ = oracle.occupancy(c.points, c.tris, g)
+        _, best, arg = oracle.sdf_bvh(c.points, c.tris, g, occ)
+        gap = np.sqrt(oracle.sdf_dist2_to(c.points, c.tris, g, near[sl])) - np.sqrt(best)
+        assert 0 <= gap.min() and gap.max() <= 2 * _margin(g), m
+        same = oracle.sdf_grad(c.points, c.tris, g, occ, gsp[sl], arg=near[sl])
+        scale = np.abs(same).max()
+        rel_same = np.linalg.norm(gparts[m] - same) / np.linalg.norm(same)
+        max_same = np.abs(gparts[m] - same).max() / scale
+        ref = oracle.sdf_grad(c.points, c.tris, g, occ, gsp[sl])
+        rel = np.linalg.norm(gparts[m] - ref) / np.linalg.norm(ref)
+        rng = np.random.default_rng(m)
+        nv = c.points.shape[0] - 2
+        fd_err = []
+        for v in rng.choice(nv, min(24, nv), replace=False):
+            for axis in range(3):
+                fd, ties = oracle.sdf_fd(c.points, c.tris, g, occ, gsp[sl], int(v), axis,
+                                         h=1e-5, base=(best, arg), with_ties=True)
+                fd_err.append(max(0.0, abs(fd - gparts[m][v, axis]) - ties) / scale)
+        print(f"C5 mesh {m} ({c.name}, {g.ncell} cells): vs FP64 adjoint of the same "
+              f"triangles rel L2 {rel_same:.2e} max {max_same:.2e}; vs FP64 adjoint rel L2 "
+              f"{rel:.2e}; vs FD max {max(fd_err):.2e}", flush=True)
+        assert rel_same < 1e-4 and max_same < 1e-3, m
+        assert rel < 2e-2, m
+        assert max(fd_err) < 1e-2, m

@@ -160,3 +160,38 @@ def test_empty_query_and_full_size(gpu):
     got = cg.points_inside(torch.tensor(q, device="cuda")).cpu().numpy()
     want = oracle.Carve(pts, tris).points(q)
     assert np.array_equal(got, want)
+
+
+def _cylinder(n=2000, radius=0.5, height=0.05):
+    """A closed cylinder whose two caps are fan-triangulated from one rim vertex: every
+    cap triangle spans from that vertex to a far rim edge."""
+    ang = 2 * np.pi * np.arange(n) / n
+    ring = np.stack([radius * np.cos(ang), radius * np.sin(ang)], 1)
+    pts = np.concatenate([np.c_[ring, np.zeros(n)], np.c_[ring, np.full(n, height)]])
+    tris = []
+    for i in range(2, n):
+        tris.append((0, i, i - 1))                  # bottom cap, facing -z
+        tris.append((n, n + i - 1, n + i))          # top cap, facing +z
+    for i in range(n):
+        j = (i + 1) % n
+        tris += [(i, j, n + j), (i, n + j, n + i)]
+    return pts, np.array(tris, np.int32)
+
+
+def test_cell_lists_beyond_2_32_entries_rejected(gpu):
+    """The parity grid's cell-list total is summed over the triangles' rectangles in 64
+    bits and refused at 2^31 or more before any list is counted, scanned or filled: here
+    the caps' 3,996 fan triangles cover more than 2^32 (column, triangle) entries on a
+    4090 x 4090 grid, a total a 32-bit sum would wrap."""
+    pts, tris = _cylinder()
+    res = 4090
+    g = oracle.make_grid(pts, res, 1)
+    assert g.dims[0] * g.dims[1] <= 1 << 24
+    cx = np.floor((pts[:, 0] - g.corner[0]) / g.spacing).astype(np.int64)
+    cy = np.floor((pts[:, 1] - g.corner[1]) / g.spacing).astype(np.int64)
+    w = cx[tris].max(1) - cx[tris].min(1) + 1
+    h = cy[tris].max(1) - cy[tris].min(1) + 1
+    assert int((w * h).sum()) > 1 << 32
+    with pytest.raises(RuntimeError, match="exceed 2\\^31 entries"):
+        _grid(pts, tris, "parity", res)
+    _grid(pts, tris, "parity", 64)     # the same surface on a coarser grid

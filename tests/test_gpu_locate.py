@@ -46,3 +46,26 @@ def test_step_limit_fallback_and_errors(gpu):
         _run(pts, tets, bad, np.array([[0.9, 0.9, 0.9]]), 0)
     with pytest.raises(RuntimeError, match="start tet out of range"):
         _run(pts, tets, nbrs, np.array([[0.5, 0.5, 0.5]]), len(tets))
+
+
+def test_query_count_limit(gpu):
+    """nq >= 2^32 is rejected before any query is read (the Morton order carries 32-bit
+    query indices); the buffers here are far smaller than nq and are never touched."""
+    import ctypes
+    import torch
+    from cybermeshgenerator_amd import locate
+    locate._bind()      # argument types of cmg_hip_locate (size_t counts)
+    pts, tets, nbrs = lattice_mesh(2)
+    P = torch.tensor(pts, dtype=torch.float64, device="cuda")
+    T = torch.tensor(tets, dtype=torch.int32, device="cuda")
+    N = torch.tensor(nbrs, dtype=torch.int32, device="cuda")
+    q = torch.zeros((4, 3), dtype=torch.float64, device="cuda")
+    out = torch.zeros(4, dtype=torch.int32, device="cuda")
+    err = ctypes.create_string_buffer(128)
+    v = ctypes.c_void_p
+    st = gpu.lib().cmg_hip_locate(v(P.data_ptr()), P.shape[0], v(T.data_ptr()),
+                                  v(N.data_ptr()), T.shape[0], v(q.data_ptr()), 1 << 32,
+                                  None, 0, v(out.data_ptr()), None, None,
+                                  v(torch.cuda.current_stream().cuda_stream), err, 128)
+    assert st == 1 and b"2^32" in err.value
+    assert int(out.abs().sum()) == 0

@@ -97,3 +97,25 @@ def test_weld_nan_vertices_never_merge(gpu):
     assert np.array_equal(wi.cpu().numpy().ravel(), np.array(idx))
     assert same_bits(wp.cpu().numpy(), np.array(pts))
     assert int(np.isnan(wp.cpu().numpy()).any(1).sum()) == int(np.isnan(soup).any(1).sum())
+
+
+def test_bunny_published_occupancy(gpu, tmp_path):
+    """The reference's published bunny occupancy (examples/bunny/README.md:70-72: 24,667
+    cells, volume 280,129 at resolution 48, pad 1) through the drop-in path: cmg_read_plc
+    (binary STL decoded and merged on the GPU), then cmg_plc_voxelize on the GPU,
+    bit-identical to the oracle's grid."""
+    from cybermeshgenerator_amd import files
+    from tests.stlfiles import BUNNY_RES48_CELLS, BUNNY_RES48_VOLUME, bunny_stl
+    path = bunny_stl(tmp_path)
+    plc = files.read_plc(path)
+    pts, tris, _ = oracle.read_stl(path)
+    assert same_bits(plc.points, pts) and np.array_equal(plc.triangles, tris)
+    vg = gpu.voxelize(plc, resolution=48, mode="occupancy", pad=1, flat=True)
+    g = oracle.make_grid(pts, 48, 1)
+    want = oracle.occupancy(pts, tris, g)
+    assert np.array_equal(vg.grid, want)
+    assert int(vg.grid.sum()) == BUNNY_RES48_CELLS
+    assert round(float(vg.grid.sum()) * vg.spacing ** 3) == BUNNY_RES48_VOLUME
+    # the reference's (nx, ny, nz) view of the same cells
+    v3 = gpu.voxelize(plc, resolution=48, mode="occupancy")
+    assert v3.grid.shape == (50, 42, 50) and int(v3.grid.sum()) == BUNNY_RES48_CELLS

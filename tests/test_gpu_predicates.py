@@ -176,3 +176,68 @@ def test_fixed_point_stages_on_device(gpu):
         want = batch(soa, oracle.EXACT)
         assert (want == 0).sum() >= 200
         assert (got == want).all(), (npts, np.nonzero(got != want)[0][:10])
+
+
+def test_insphere_values(gpu):
+    """cmg::robust::insphere returns the determinant's value (robust.cpp:44-50): the naive
+    FP64 lifted determinant in the reference's operation order (predicates.cpp:4063-4106)
+    bit for bit when it is nonzero, else a value with the exact sign."""
+    rng = np.random.default_rng(404)
+    pts = np.round(rng.uniform(-1, 1, (400, 3)) * 16) / 16     # many exact cosphericities
+    quints = rng.integers(0, 400, size=(20000, 5)).astype(np.int32)
+    vals = gpu.insphere_values(pts, quints)
+    naive = np.array([oracle.insphere_naive(*pts[q]) for q in quints])
+    nz = naive != 0
+    assert nz.sum() > 1000 and (~nz).sum() > 100
+    assert np.array_equal(vals[nz], naive[nz])
+    sg = np.sign(vals[~nz]).astype(np.int8)
+    want = np.array([oracle.insphere_sign(*pts[q], oracle.EXACT) for q in quints[~nz]])
+    assert np.array_equal(sg, want)
+    # the closed form insphere = orient3d(a,b,c,d) * (r^2 - |e - centre|^2) on the unit
+    # corner tetrahedron (circumcentre (1/2,1/2,1/2), r^2 = 3/4), exact on dyadic inputs
+    tet = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    es = np.array([[0.25, 0.25, 0.25], [0.5, 0.5, 0.5], [100, 100, 100], [1, 1, 0]])
+    p = np.concatenate([tet, es])
+    q = np.array([[0, 1, 2, 3, 4 + k] for k in range(4)], np.int32)
+    assert list(gpu.insphere_values(p, q)) == [-0.5625, -0.75, 29700.0, 0.0]
+
+
+@pytest.mark.parametrize("npts", [4, 5])
+def test_expansion_and_wide_passes_on_device(gpu, npts):
+    """Tuples beyond the fixed-point tiers (coordinate exponents over +-300 for orient3d,
+    +-150 for insphere, a repeated point: exact zeros, then 1-ulp moves) are decided by the
+    device expansion passes (predicates.cpp:1609-1777, 3237-3488 restated) with the
+    oracle's sign in both modes; for orient3d some lanes outgrow the first pass and are
+    re-run with the wide capacity."""
+    import torch
+    n = 6000
+    soa = W.tuples_wide_spread(n, npts, seed=60 + npts)
+    d = _dev(soa)
+    ws = torch.empty(gpu.pred_workspace_size(n, npts), dtype=torch.uint8, device="cuda")
+    batch = oracle.orient3d_batch if npts == 4 else oracle.insphere_batch
+    for mode, om in (("exact", oracle.EXACT), ("compat", oracle.COMPAT)):
+        got = gpu.signs_soa(d, mode, workspace=ws)
+        st = gpu.pred_stats(n, npts, workspace=ws)
+        got = got.cpu().numpy()
+        want = batch(soa, om)
+        assert (got == want).all(), (mode, np.nonzero(got != want)[0][:10])
+        assert st["failed"] == 0, st
+        expansion = st["exact"] - st["exact_inline"]
+        print(npts, mode, st)
+        assert expansion >= n // 3, (mode, st)               # past the fixed-point tiers
+        if npts == 4:
+            assert st["wide"] > 0, (mode, st)
+    assert (want == 0).sum() >= n // 2
+
+
+def test_full_size_insphere_batch_compat(gpu):
+    """The 2^22 insphere batch in compat mode (what cmg::robust::insphere returns as
+    shipped: the naive FP64 sign when nonzero, else the exact sign)."""
+    n = 1 << 22
+    soa, kind = W.tuples_insphere(n, seed=22)
+    got = gpu.signs_soa(_dev(soa), "compat").cpu().numpy()
+    want = oracle.insphere_batch(soa, oracle.COMPAT)
+    assert (got == want).all(), int((got != want).sum())
+    # compat and exact differ on some near-cospherical tuples (wrong-signed naive values)
+    deg = np.ascontiguousarray(soa[:, kind > 0])
+    assert (want[kind > 0] != oracle.insphere_batch(deg, oracle.EXACT)).any()

@@ -271,3 +271,20 @@ def test_invalid_input_rejected():
     pts, tris = subdivided_cube(2)
     with pytest.raises(ValueError):
         oracle.voxelize(pts, tris, resolution=0)
+
+
+def test_bunny_published_occupancy(tmp_path):
+    """The reference's own published output for this path: cybermesh.read_plc of
+    examples/bunny/Stanford_Bunny_sample.stl, then voxelize(resolution=48) with the
+    default pad 1, gives 24,667 occupied cells and volume 280,129
+    (examples/bunny/README.md:70-72; examples/bunny/run_bunny.py:43,194,297-298)."""
+    from tests.stlfiles import BUNNY_RES48_CELLS, BUNNY_RES48_VOLUME, bunny_stl
+    pts, tris, binary = oracle.read_stl(bunny_stl(tmp_path))
+    assert binary and tris.shape == (112402, 3)
+    g = oracle.make_grid(pts, 48, 1)
+    occ, stats = oracle.occupancy(pts, tris, g, with_stats=True)
+    assert int(occ.sum()) == BUNNY_RES48_CELLS
+    assert round(float(occ.sum()) * g.spacing ** 3) == BUNNY_RES48_VOLUME
+    # the same answer through the oracle's voxelize driver (validation + fan triangles)
+    g2, occ2 = oracle.voxelize(pts, tris, resolution=48, pad=1)
+    assert g2.dims == (50, 42, 50) and np.array_equal(occ2, occ)

@@ -200,3 +200,39 @@ def test_orient3d_stages_against_oracle(tmp_path):
     want = oracle.orient3d_batch(np.ascontiguousarray(t.T), oracle.EXACT).astype(np.int64)
     assert np.array_equal(sign[ok[:, 3], 3], want[ok[:, 3]])
     assert ok[:, 3].mean() > 0.95 and (want[ok[:, 3]] == 0).any()
+
+
+def _run_exp(mode, tuples, tmp_path):
+    """Host replay of the exact pass (register_stages <mode>-exp): decided flags and
+    signs of fixed point (62-, 126-bit differences), then the expansions with the first
+    pass's capacities and with the wide re-run's."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/native/register_stages not built (run the package build)")
+    per = 12 if mode == "orient3d" else 15
+    t = np.ascontiguousarray(tuples, dtype=np.float64).reshape(-1, per)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    t.tofile(src)
+    subprocess.run([BIN, mode + "-exp", str(src), str(dst)], check=True)
+    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 4, 2).astype(np.int64)
+    return out[:, :, 0] > 0, out[:, :, 1]
+
+
+@pytest.mark.parametrize("npts", [4, 5])
+def test_expansion_passes_on_wide_spread_tuples(npts, tmp_path):
+    """predicates.cpp:1609-1777 (orient3dexact) and :3237-3488 (insphereexact), restated as
+    the device's expansion passes and run on the host: coordinates with independent
+    exponents over +-300 (orient3d) / +-150 (insphere) and a repeated point (exact zeros,
+    then 1-ulp moves). The fixed-point stages decline every tuple; the expansions decide
+    every one with the oracle's sign; for orient3d some outgrow the first pass's capacity
+    and are decided by the wide re-run."""
+    soa = W.tuples_wide_spread(3000, npts, seed=40 + npts)
+    ok, sign = _run_exp("orient3d" if npts == 4 else "insphere", soa.T, tmp_path)
+    want = (oracle.orient3d_batch(soa, oracle.EXACT) if npts == 4
+            else oracle.insphere_batch(soa, oracle.EXACT)).astype(np.int64)
+    assert not ok[:, 0].any() and not ok[:, 1].any()      # beyond the fixed-point tiers
+    assert ok[:, 3].all()                                 # the wide capacities suffice
+    for k in (2, 3):
+        assert np.array_equal(sign[ok[:, k], k], want[ok[:, k]]), k
+    assert (want == 0).sum() >= 1500 and (want > 0).any() and (want < 0).any()
+    if npts == 4:
+        assert 0.02 < (~ok[:, 2]).mean() < 0.5            # some lanes need the wide pass
