@@ -11,6 +11,7 @@
 #include <cctype>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,38 @@ cmg_status soa_signs(int npts, const double* soa, size_t n, size_t ld, int mode,
             CMG_HIP(hipStreamSynchronize(st));
         }
     });
+}
+
+// ---- fixed workspace of the no-allocation orient3d entry point ------------------------
+//
+// cmg_hip_orient3d_signs_soa_noalloc runs the same kernels as cmg_hip_orient3d_signs_soa
+// with its workspace in this file-scope device array instead of a caller or library
+// allocation: every GPU the library runs on holds one copy (16 MiB) until the process
+// ends, and every call on that GPU shares it. In host code `scratch` only names the
+// symbol; its device address on each GPU comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr size_t kNoAllocMaxTuples = size_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// This GPU's address of `scratch` (looked up on the first call on each device).
+void* noalloc_scratch() {
+    int dev = 0;
+    CMG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw cmg::hip::HipError("device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw cmg::hip::HipError(std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
 }
 
 }  // namespace
@@ -391,6 +424,28 @@ cmg_status cmg_hip_orient3d_signs_soa(const double* soa, size_t n, size_t ld, in
                                       size_t len) {
     return soa_signs(4, soa, n, ld, mode, out, workspace, workspace_bytes, stream, errbuf,
                      len);
+}
+
+cmg_status cmg_hip_orient3d_signs_soa_noalloc(const double* soa, size_t n, size_t ld,
+                                              int mode, signed char* out, void* stream,
+                                              char* errbuf, size_t len) {
+    return guarded(errbuf, len, [&] {
+        require_device();
+        if (mode != cmg::hip::kModeExact && mode != cmg::hip::kModeCompat)
+            throw cmg::hip::InvalidInput("mode must be CMG_PRED_EXACT or CMG_PRED_COMPAT");
+        if (n && (!soa || !out)) throw cmg::hip::InvalidInput("null pointer");
+        if (n && ld < n) throw cmg::hip::InvalidInput("ld must be >= n");
+        if (n > kNoAllocMaxTuples)
+            throw cmg::hip::InvalidInput(
+                "n must be <= 2^20 tuples (the size of the library's fixed workspace)");
+        const cmg::hip::PredPlan plan = cmg::hip::pred_plan(n, 2);
+        if (cmg::hip::pred_workspace_bytes(plan, n) > kNoAllocScratchBytes)
+            throw cmg::hip::InvalidInput("batch needs more than the 16 MiB fixed workspace");
+        g_last_backend = "hip";
+        const cmg::hip::PredWorkspace ws = cmg::hip::pred_workspace(noalloc_scratch(), plan, n);
+        cmg::hip::launch_signs_soa(4, mode, soa, n, ld, reinterpret_cast<int8_t*>(out), plan,
+                                   ws, as_stream(stream));
+    });
 }
 
 cmg_status cmg_hip_insphere_signs_soa(const double* soa, size_t n, size_t ld, int mode,

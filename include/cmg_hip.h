@@ -179,6 +179,17 @@ cmg_status cmg_hip_insphere_signs_soa(const double* soa, size_t n, size_t ld, in
                                       size_t workspace_bytes, void* stream, char* errbuf,
                                       size_t errbuf_len);
 
+/* No counterpart in the reference: cmg_hip_orient3d_signs_soa for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same kernels and results; the workspace is a fixed 16 MiB
+ * device array inside the library, one per GPU, held until the process ends, so the call
+ * performs no allocation of any kind. n <= 2^20 tuples (larger n: CMG_ERR_INVALID_INPUT).
+ * Stream-ordered, returns at once. Every call on one GPU shares that workspace: calls
+ * that overlap on one GPU (on different streams, or from different threads) race. */
+cmg_status cmg_hip_orient3d_signs_soa_noalloc(const double* soa, size_t n, size_t ld,
+                                              int mode, signed char* out, void* stream,
+                                              char* errbuf, size_t errbuf_len);
+
 /* Reads the per-stage counts of the last call that used `workspace` (synchronises
  * `stream`). */
 cmg_status cmg_hip_pred_stats(void* workspace, size_t n, int npts, void* stream,
