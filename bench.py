@@ -209,6 +209,74 @@ def cpu_baseline_predicates(soa_np, gpu_signs):
             "seconds": dt}
 
 
+def harness_threads() -> int:
+    """Host threads for the harness-level parallel CPU baselines: the job's CPU share
+    (OMP_NUM_THREADS when set), else every core."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_harness_predicates(soa_np, gpu_signs):
+    """C2 with the CPU oracle over batch slices on `harness_threads()` threads (a harness
+    around the reference's single-threaded predicate; its calls are independent)."""
+    import oracle
+    nt = harness_threads()
+    t = time.perf_counter()
+    want = oracle.orient3d_batch(soa_np, oracle.EXACT, nthreads=nt)
+    dt = time.perf_counter() - t
+    if not (want == gpu_signs).all():
+        raise SystemExit("bench: GPU signs differ from the CPU oracle")
+    n = soa_np.shape[1]
+    return {"value": n / dt, "unit": "signs/s", "cores": nt, "kind": "port",
+            "harness": True, "seconds": dt,
+            "sample": f"the full {n}-tuple exact orient3d batch split over {nt} threads "
+                      f"(harness-level parallel run; {cpu_model()})"}
+
+
+def cpu_harness_c5(W, cases, budget_s=20.0):
+    """C5 on the CPU: the reference's SDF search (3-D buckets + Chebyshev shells, restated
+    in the oracle) over an evenly strided sample of every mesh's cells, one mesh per task
+    on `harness_threads()` threads (a harness: the reference is single-threaded and each
+    mesh is independent). Every task rebuilds its mesh's occupancy and buckets; a pass
+    over one cell per mesh times those builds, and the sample is sized so the search
+    takes about budget_s seconds."""
+    import concurrent.futures as cf
+    import oracle
+    nt = harness_threads()
+    grids = [oracle.make_grid(c.points, c.resolution, c.pad) for c in cases]
+    total = sum(g.ncell for g in grids)
+
+    def task(m, frac):
+        c, g = cases[m], grids[m]
+        occ = oracle.occupancy(c.points, c.tris, g, nthreads=1)
+        n = max(1, int(g.ncell * frac))
+        stride = max(1, g.ncell // n)
+        oracle.sdf_shell(c.points, c.tris, g, occ, first=stride // 2, count=n, stride=stride)
+        return n
+
+    def run_all(frac):
+        t = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=nt) as ex:
+            n = sum(ex.map(lambda m: task(m, frac), range(len(cases))))
+        return n, time.perf_counter() - t
+
+    _, t_build = run_all(0.0)                 # one cell per mesh: the builds
+    n_probe, t_probe = run_all(4e-6)
+    per_cell = max(t_probe - t_build, 1e-3) / n_probe
+    frac = min(0.01, budget_s / per_cell / total)
+    done, dt = run_all(frac)
+    return {"value": done / dt, "unit": "voxels/s (forward SDF only)", "cores": nt,
+            "kind": "port", "harness": True, "seconds": dt, "cells_timed": done,
+            "build_seconds": t_build,
+            "search_only_voxels_per_s": done / max(dt - t_build, 1e-9),
+            "sample": f"{done} of the {total} C5 cells ({frac:.2e} of each mesh's, evenly "
+                      f"strided), reference shell search, one mesh per task on {nt} "
+                      f"threads, occupancy and bucket builds included ({t_build:.1f} s of "
+                      f"the {dt:.1f} s; {cpu_model()})"}
+
+
 def cpu_sdf_sample(W, budget_s=20.0):
     """The reference's own SDF search (3-D buckets + Chebyshev shells, restated in the
     oracle) on one core, over cells spread evenly through the C3 grid (256^3 torus),
@@ -239,29 +307,45 @@ def cpu_sdf_sample(W, budget_s=20.0):
                       f"{t_build:.2f} s excluded ({cpu_model()})"}
 
 
-# FP32 operations per lane of the voxel passes' three kinds of work (FMA = 2): an
-# oriented-box lower bound (tree node) ~26, a bounding-disk lower bound (triangle) ~30, a
+# FP32 operations per lane of the forward voxel pass's three kinds of work (FMA = 2): an
+# oriented-box lower bound (tree node) ~26, an oriented-box lower bound (triangle) ~30, a
 # point-triangle distance ~80.
 FLOP_BOX, FLOP_DISK, FLOP_TRI = 26, 30, 80
 FP32_PEAK_TFLOPS = 157.3
+# Algorithmic FP32 work of the SDF (the survey's count): 48 flops per point-triangle pair
+# evaluated, plus 20 per cell in the backward pass.
+ALG_FLOP_PAIR, ALG_FLOP_BWD_CELL = 48, 20
 
 
-def voxel_work(counters, cells, vpl, kernel_ms):
-    """Work of the forward + backward voxel passes from the engine's counters (each
-    counted once per wavefront of 64 lanes x vpl voxels) and the FP32 rate it implies."""
+def voxel_work(counters, cells, vpl, fwd_ms, bwd_ms):
+    """Work of the forward voxel pass from the engine's counters (each counted once per
+    wavefront of 64 lanes x vpl voxels), the FP32 rate it implies, and the SDF roofline
+    by the algorithmic count (48 flops per evaluated point-triangle pair + 20 per cell in
+    the backward) over the forward + backward voxel kernels' time."""
     lanes = 64 * vpl
     box = counters["node_tests"] * lanes
     disk = counters["tri_tests"] * lanes
     tri = counters["tri_evals"] * lanes
     flops = box * FLOP_BOX + disk * FLOP_DISK + tri * FLOP_TRI
-    out = {"node_tests_per_voxel": box / (2 * cells),
-           "triangle_bound_tests_per_voxel": disk / (2 * cells),
-           "triangle_distances_per_voxel": tri / (2 * cells),
+    out = {"node_tests_per_voxel": box / cells,
+           "triangle_bound_tests_per_voxel": disk / cells,
+           "triangle_distances_per_voxel": tri / cells,
            "counted_fp32_gflop_per_step": flops / 1e9}
-    if kernel_ms > 0:
-        tf = flops / (kernel_ms / 1e3) / 1e12
-        out["voxel_kernels_fp32_tflops"] = tf
-        out["voxel_kernels_fp32_frac_of_peak"] = tf / FP32_PEAK_TFLOPS
+    if fwd_ms > 0:
+        tf = flops / (fwd_ms / 1e3) / 1e12
+        out["fwd_voxel_kernel_fp32_tflops"] = tf
+        out["fwd_voxel_kernel_fp32_frac_of_peak"] = tf / FP32_PEAK_TFLOPS
+    alg = tri * ALG_FLOP_PAIR + cells * ALG_FLOP_BWD_CELL
+    ms = fwd_ms + bwd_ms
+    if ms > 0:
+        a = alg / (ms / 1e3) / 1e12
+        out["roofline"] = {"bound": "fp32", "achieved": a, "peak": FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": a / FP32_PEAK_TFLOPS,
+                           "algorithmic_flop_per_step": alg,
+                           "count": "48 x point-triangle pairs evaluated (engine counter) "
+                                    "+ 20 x cells (backward)",
+                           "kernels": "vox_kernel (forward) + adjoint_kernel (backward)",
+                           "kernel_ms": ms}
     return out
 
 
@@ -315,7 +399,7 @@ def bench_sdf(cm, W, steps, warmup, variant=None, full=True):
                "batched_voxels_per_s": cells / (ms_b / 1e3), "fwd_voxel_kernel_ms": fwd_ms,
                "bwd_voxel_kernel_ms": bwd_ms, "occupancy_kernel_ms": occ_ms,
                "variant": batch.variant_info()}
-        out.update(voxel_work(st, cells, batch.vpl(), fwd_ms + bwd_ms))
+        out.update(voxel_work(st, cells, batch.vpl(), fwd_ms, bwd_ms))
         batch.close()
         return out
     # plain: one engine per mesh, one mesh at a time
@@ -349,7 +433,7 @@ def bench_sdf(cm, W, steps, warmup, variant=None, full=True):
            "occupancy_kernel_ms": occ_ms,
            "variant": batch.variant_info(), "autotune_ms": tune["ms"],
            "overflow": st["overflow"]}
-    out.update(voxel_work(st, cells, batch.vpl(), fwd_ms + bwd_ms))
+    out.update(voxel_work(st, cells, batch.vpl(), fwd_ms, bwd_ms))
     batch.close()
     # single-mesh C3 and C4 forward
     for name, case in (("c3_torus_256", W.torus_c3(seed=3)), ("c4_harmonic_128", W.harmonic_c4(seed=4))):
@@ -643,6 +727,8 @@ def main():
             if not args.no_cpu:
                 log("[bench] CPU reference SDF search sample (1 thread)")
                 extra["sdf"]["cpu_baseline"] = cpu_sdf_sample(W)
+                log("[bench] CPU reference SDF search over C5, harness threads")
+                extra["sdf"]["cpu_harness"] = cpu_harness_c5(W, W.batch_c5(seed=5))
         except Exception as e:  # reported, never hides the predicate line
             log(f"[bench] sdf workloads failed: {e!r}")
             extra["sdf"] = {"error": repr(e)}
@@ -674,6 +760,8 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU oracle baseline (1 thread)")
         cpu = cpu_baseline_predicates(pr["soa_np"], pr["signs"])
+        log("[bench] CPU oracle, harness threads")
+        extra["orient3d"]["cpu_harness"] = cpu_harness_predicates(pr["soa_np"], pr["signs"])
 
     line = {
         "metric": "SDF voxels/s fwd+bwd batched vs plain GPU path; orient3d signs/s, exact signs",

@@ -33,9 +33,11 @@
 //   occupancy    per (mesh, 8x8 column tile): exact column parity (below); tiles where
 //                a naive determinant is exactly 0 are redone with exact predicates
 //   tree         level 1 from the triangles, then each level from the one below
-//   voxels       per (mesh, voxel leaf): FP32 nearest, FP64 near the surface, signed
-// A backward step reuses the forward's occupancy and tree and runs the adjoint voxel
-// pass (closest features recomputed) after zeroing the gradient.
+//   voxels       per (mesh, voxel leaf): FP32 nearest, FP64 near the surface, signed;
+//                records per cell (4 bytes) the triangle the distance comes from
+// A backward step reuses the forward's occupancy and per-cell triangles: after zeroing
+// the gradient, the adjoint pass recomputes each cell's closest point on its triangle
+// (the closest feature and barycentric weights are not stored) and scatters.
 //
 // Compiled with -ffp-contract=off like the single-mesh kernels it shares code with.
 #include <hip/hip_runtime.h>
@@ -127,7 +129,10 @@ struct Ctx {  // kernel arguments shared by every phase
     unsigned long long* masks;
     int* redo;                  // tiles (global items) for the exact occupancy pass
     uint8_t* occ;
-    float* best;                // per cell: the forward's FP32 squared distance
+    int* cell_tri;              // per cell, blocked (64 cells of a 4x4x4 block in a row):
+                                // the tree slot of the triangle the cell's distance comes
+                                // from: s >= 0 (FP32 nearest), -2 - s (FP64 near the
+                                // surface), -1 (none)
     float* sdf;
     const float* gsdf;
     float* grad;
@@ -212,19 +217,14 @@ __global__ __launch_bounds__(256) void reset_kernel(Ctx C) {
     }
 }
 
-__global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n,
-                                                        BatchCounters* ctr) {
+__global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n) {
     const long long stride = (long long)gridDim.x * blockDim.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->queue[2] = 0;  // backward queue
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         g[i] = 0.0f;
 }
 
-__global__ void reset_queue_kernel(BatchCounters* ctr, int a, int b) {
-    if (threadIdx.x == 0) {
-        ctr->queue[a] = 0;
-        ctr->queue[b] = 0;
-    }
+__global__ void reset_voxel_queue_kernel(BatchCounters* ctr) {
+    if (threadIdx.x == 0) ctr->queue[1] = 0;
 }
 
 // Morton code (10 bits per axis) of each triangle's centroid within its mesh's grid.
@@ -843,17 +843,25 @@ __global__ __launch_bounds__(256) void node_merge_kernel(Ctx C, int level, int t
 
 // ---- voxel pass ---------------------------------------------------------------------------
 
-// One voxel leaf per item: 4x4x(4 VPL) voxels, VPL per lane, 4 cells apart along z. D = 0 reads a tree leaf's triangle
-// records with scalar loads (SGPR operands); D = 8 stages the leaf's 8 records through LDS
-// as one 1 KiB tile. BWD: the adjoint (and, when C.nearest is set, the nearest triangle
-// of every cell) instead of the distance.
-template <bool BWD, int VPL, int D>
+// Blocked index of cell (i, j, k) in a mesh's cell_tri array: the 64 cells of each
+// 4x4x4 block are contiguous (one wavefront's cells in the 4x4x4 variant).
+__device__ __forceinline__ size_t blocked_cell(const MeshDesc& M, int i, int j, int k) {
+    const size_t blk = (size_t)(i >> 2) +
+                       (size_t)M.L0.bx * ((size_t)(j >> 2) + (size_t)M.L0.by * (size_t)(k >> 2));
+    return blk * 64 + (size_t)((i & 3) + 4 * (j & 3) + 16 * (k & 3));
+}
+
+// One voxel leaf per item: 4x4x(4 VPL) voxels, VPL per lane, 4 cells apart along z. D = 0
+// reads a tree leaf's triangle records with scalar loads (SGPR operands); D = 8 stages the
+// leaf's 8 records through LDS as one 1 KiB tile. Writes the signed distance and, for the
+// backward pass, the triangle each cell's distance comes from.
+template <int VPL, int D>
 __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
     __shared__ TriRec s_rec[D > 0 ? D : 1];
     const long long* off = phase_off(C, VPL == 1 ? kPhL0 : (VPL == 2 ? kPhL0b : kPhL0c));
     const long long total = off[C.nm];
     const int lane = threadIdx.x;
-    unsigned long long* q = &C.ctr->queue[BWD ? 2 : 1];
+    unsigned long long* q = &C.ctr->queue[1];
     unsigned long long nvis = 0, ntest = 0, neval = 0;
     while (true) {
         const long long item = next_item(q);
@@ -1007,23 +1015,14 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             return s0;
         };
 
-        // ---- FP32 nearest triangle. The backward starts from the forward's final bound
-        // (the same triangles, the same FP32 distances): whatever it prunes has a
-        // distance above the forward's minimum, so it finds the same triangle with a
-        // shorter walk.
+        // ---- FP32 nearest triangle
         float best[VPL], b2[VPL];
         int arg[VPL];
-        float* const cell_best = C.best + M.cell_base;
 #pragma unroll
         for (int v = 0; v < VPL; ++v) {
             best[v] = FLT_MAX;
             b2[v] = valid[v] ? FLT_MAX : -1.0f;
             arg[v] = -1;
-            if (BWD && valid[v]) {
-                const float bb =
-                    bound_root(cell_best[((size_t)k[v] * g.ny + j) * g.nx + i]) + eps;
-                b2[v] = bb * bb;
-            }
         }
         walk(b2, [&](int loc, int cnt) {
             const long long s0 = leaf_rec(loc, cnt);
@@ -1126,61 +1125,150 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
             });
         }
         const uint8_t* occ = C.occ + M.cell_base;
-        if (!BWD) {
-            float* out = C.sdf + M.cell_base;
+        float* out = C.sdf + M.cell_base;
+        int* const tri_out = C.cell_tri + M.blk_base;
 #pragma unroll
-            for (int v = 0; v < VPL; ++v) {
-                if (!valid[v]) continue;
-                const size_t idx = ((size_t)k[v] * g.ny + j) * g.nx + i;
-                cell_best[idx] = best[v];
-                if (near[v]) {
-                    const double d = sqrt(best64[v]);
-                    out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
-                } else {
-                    const float d32 = sqrtf(best[v]) * M.iscf;  // back to world units (exact)
-                    out[idx] = occ[idx] ? -d32 : d32;
-                }
+        for (int v = 0; v < VPL; ++v) {
+            if (!valid[v]) continue;
+            const size_t idx = ((size_t)k[v] * g.ny + j) * g.nx + i;
+            tri_out[blocked_cell(M, i, j, k[v])] = near[v] ? -2 - arg64[v] : arg[v];
+            if (near[v]) {
+                const double d = sqrt(best64[v]);
+                out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
+            } else {
+                const float d32 = sqrtf(best[v]) * M.iscf;  // back to world units (exact)
+                out[idx] = occ[idx] ? -d32 : d32;
             }
-            continue;
         }
-        // ---- adjoint: d = |p - q| with q the closest point, dd/dv_m = -w_m (p - q) / d
-        // (one body per voxel of the lane, each with compile-time indices)
-        auto adjoint = [&](auto vc) {
-            constexpr int v = decltype(vc)::value;
+    }
+    if (lane == 0) {
+        if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
+        if (ntest) atomicAdd(&C.ctr->tri_tests, ntest);
+        if (neval) atomicAdd(&C.ctr->tri_evals, neval);
+    }
+}
+
+// ---- adjoint ------------------------------------------------------------------------------
+//
+// One 4x4x4 block per wavefront and item. The forward recorded, per cell, the triangle its
+// distance comes from; the closest point on it (barycentric weights: Ericson's regions in
+// FP32, or the reference's FP64 routine within one cell of the surface) is recomputed here
+// and the cell's contribution d(dist)/d(vertex) = -w_m (p - q) / d, times the upstream
+// gradient and the sign, is summed per triangle: over the wavefront (one group of lanes
+// per distinct triangle), then over the workgroup in an LDS table keyed by triangle, and
+// only then added to the vertices with global atomics. A workgroup takes a contiguous
+// range of items (neighbouring blocks share their nearest triangles: far from a coarse
+// mesh whole regions share one), so the same vertices are not hammered by every
+// wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
+constexpr int kAdjSlots = 1024;   // LDS table entries per workgroup (power of two)
+constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
+constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
+
+// Adds the 9 partial derivatives of triangle slot `tri` (mesh M) to its vertices.
+__device__ __forceinline__ void add_tri_grad(const Ctx& C, const MeshDesc& M, int tri,
+                                             const float* v9) {
+    const int* tv = C.tris + 3 * (size_t)C.perm[tri];
+    float* gbase = C.grad + 3 * M.vbase;
+#pragma unroll
+    for (int vtx = 0; vtx < 3; ++vtx) {
+        float* gp = gbase + 3 * (size_t)tv[vtx];
+        atomicAdd(gp + 0, v9[3 * vtx + 0]);
+        atomicAdd(gp + 1, v9[3 * vtx + 1]);
+        atomicAdd(gp + 2, v9[3 * vtx + 2]);
+    }
+}
+
+// Adds v9 into the workgroup table entry of triangle slot `tri` (or, when the table is
+// crowded, to the vertices directly); a newly taken entry is listed in used[].
+__device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int tri,
+                                          const float* v9, int* key, float* val, int* used,
+                                          int* nused) {
+    const unsigned hsh = (unsigned)tri * 0x9E3779B1u;
+    for (int pr = 0; pr < kAdjProbe; ++pr) {
+        const int e = (int)((hsh >> 22) + (unsigned)pr) & (kAdjSlots - 1);
+        const int cur = atomicCAS(&key[e], -1, tri);
+        if (cur == -1) used[atomicAdd(nused, 1)] = e;
+        if (cur == -1 || cur == tri) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) atomicAdd(&val[e * 9 + q], v9[q]);
+            return;
+        }
+    }
+    add_tri_grad(C, M, tri, v9);
+}
+
+// A workgroup (4 wavefronts) takes ranges of per_range consecutive blocks; after each
+// range the table entries taken are added to the vertices and cleared.
+__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, long long per_range) {
+    __shared__ int s_key[kAdjSlots];
+    __shared__ float s_val[kAdjSlots * 9];
+    __shared__ int s_used[kAdjSlots];  // entries taken since the last flush
+    __shared__ int s_nused;
+    const long long* off = phase_off(C, kPhL0);
+    const long long total = off[C.nm];
+    const int lane = threadIdx.x & 63;
+    const bool scatter = C.grad != nullptr;
+    for (int e = threadIdx.x; e < kAdjSlots; e += blockDim.x) s_key[e] = -1;
+    for (int e = threadIdx.x; e < kAdjSlots * 9; e += blockDim.x) s_val[e] = 0.0f;
+    if (threadIdx.x == 0) s_nused = 0;
+    __syncthreads();
+    const long long nrange = (total + per_range - 1) / per_range;
+    for (long long r = blockIdx.x; r < nrange; r += gridDim.x) {
+        const long long first = r * per_range;
+        const long long last = first + per_range < total ? first + per_range : total;
+        for (long long item = first + (threadIdx.x >> 6); item < last;
+             item += blockDim.x / 64) {
+            const int pos = find_pos(off, C.nm, item);
+            const MeshDesc& M = C.mesh[C.order[pos]];
+            const GridSpec& g = M.g;
+            const int b = (int)(item - off[pos]);
+            const int ix = b % M.L0.bx, iy = (b / M.L0.bx) % M.L0.by,
+                      iz = b / (M.L0.bx * M.L0.by);
+            const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3),
+                      k = iz * kLeaf + (lane >> 4);
+            const bool valid = i < g.nx && j < g.ny && k < g.nz;
+            const int code = valid ? C.cell_tri[M.blk_base + (size_t)b * 64 + lane] : -1;
+            const bool near = code <= -2;
+            const int tri = near ? -2 - code : code;
+            const size_t idx = valid ? ((size_t)k * g.ny + j) * g.nx + i : 0;
+            if (valid && C.nearest)
+                C.nearest[M.cell_base + idx] = tri >= 0 ? (int)(C.perm[tri] - M.tbase) : -1;
             float c[9];
             bool act = false;
-            const size_t idx = valid[v] ? ((size_t)k[v] * g.ny + j) * g.nx + i : 0;
-            const int tri = near[v] ? arg64[v] : arg[v];
-            if (valid[v] && C.nearest)
-                C.nearest[M.cell_base + idx] = tri >= 0 ? (int)(C.perm[tri] - M.tbase) : -1;
-            if (valid[v] && tri >= 0 && C.grad) {
+            if (valid && tri >= 0 && scatter) {
                 const float gv = C.gsdf[M.cell_base + idx];
-                const float sg = occ[idx] ? -1.0f : 1.0f;
-                if (near[v]) {  // FP64 closest point near the surface
+                const float sg = C.occ[M.cell_base + idx] ? -1.0f : 1.0f;
+                if (near) {  // FP64 closest point near the surface
+                    const double p64[3] = {g.corner[0] + (i + 0.5) * g.spacing,
+                                           g.corner[1] + (j + 0.5) * g.spacing,
+                                           g.corner[2] + (k + 0.5) * g.spacing};
                     const TriRec64& T64 = C.rec64[tri];
                     double w[3];
-                    const double d2 = point_tri_dist2_ref(p64[v], T64.a, T64.b, T64.c, w);
+                    const double d2 = point_tri_dist2_ref(p64, T64.a, T64.b, T64.c, w);
                     double e[3];
                     for (int a = 0; a < 3; ++a)
-                        e[a] = p64[v][a] - (w[0] * T64.a[a] + w[1] * T64.b[a] + w[2] * T64.c[a]);
+                        e[a] = p64[a] - (w[0] * T64.a[a] + w[1] * T64.b[a] + w[2] * T64.c[a]);
                     const double d = sqrt(d2);
                     if (d > M.dzero && gv != 0.0f) {
                         const double coef = -(double)gv * sg / d;
 #pragma unroll
                         for (int m = 0; m < 3; ++m)
 #pragma unroll
-                            for (int a = 0; a < 3; ++a) c[3 * m + a] = (float)(coef * w[m] * e[a]);
+                            for (int a = 0; a < 3; ++a)
+                                c[3 * m + a] = (float)(coef * w[m] * e[a]);
                         act = true;
                     }
                 } else {
+                    const float h = (float)g.spacing * M.scf;  // cell size, search units
+                    const float px = (i + 0.5f) * h, py = (j + 0.5f) * h, pz = (k + 0.5f) * h;
                     const TriRec T = C.rec[tri];
                     float vv, ww;
-                    tri_bary(T, px, py, pz[v], vv, ww);
+                    tri_bary(T, px, py, pz, vv, ww);
                     const float u = 1.0f - vv - ww;
                     const float qx = T.ax + vv * T.abx + ww * T.acx;
                     const float qy = T.ay + vv * T.aby + ww * T.acy;
                     const float qz = T.az + vv * T.abz + ww * T.acz;
-                    const float ex = px - qx, ey = py - qy, ez = pz[v] - qz;
+                    const float ex = px - qx, ey = py - qy, ez = pz - qz;
                     const float d = sqrtf(ex * ex + ey * ey + ez * ez);
                     if (d > 0.0f && gv != 0.0f) {
                         const float coef = -gv * sg / d;
@@ -1191,38 +1279,37 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                     }
                 }
             }
-            // wavefront-aggregated scatter: one group of lanes per distinct triangle
+            // large groups of lanes on one triangle (far from the surface) are summed
+            // across the wavefront first; the rest go into the table lane by lane
             while (true) {
                 const unsigned long long mask = __ballot(act);
                 if (mask == 0) break;
                 const int leader = __ffsll((long long)mask) - 1;
-                const int lt = __shfl(tri, leader);
+                const int lt = __builtin_amdgcn_readlane(tri, leader);
                 const bool match = act && tri == lt;
+                if (__popcll(__ballot(match)) < kAdjGroup) break;
                 float sum[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
-                if (lane == leader) {
-                    const int* tv = C.tris + 3 * (size_t)C.perm[lt];
-                    float* gbase = C.grad + 3 * M.vbase;
-#pragma unroll
-                    for (int vtx = 0; vtx < 3; ++vtx) {
-                        float* gp = gbase + 3 * (size_t)tv[vtx];
-                        atomicAdd(gp + 0, sum[3 * vtx + 0]);
-                        atomicAdd(gp + 1, sum[3 * vtx + 1]);
-                        atomicAdd(gp + 2, sum[3 * vtx + 2]);
-                    }
-                }
+                if (lane == leader) table_add(C, M, lt, sum, s_key, s_val, s_used, &s_nused);
                 act = act && !match;
             }
-        };
-        [&]<int... V>(std::integer_sequence<int, V...>) {
-            (adjoint(std::integral_constant<int, V>{}), ...);
-        }(std::make_integer_sequence<int, VPL>{});
-    }
-    if (lane == 0) {
-        if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
-        if (ntest) atomicAdd(&C.ctr->tri_tests, ntest);
-        if (neval) atomicAdd(&C.ctr->tri_evals, neval);
+            if (act) table_add(C, M, tri, c, s_key, s_val, s_used, &s_nused);
+        }
+        if (!scatter) continue;
+        __syncthreads();
+        const int nu = s_nused;
+        for (int u = threadIdx.x; u < nu; u += blockDim.x) {
+            const int e = s_used[u];
+            const int t = s_key[e];
+            add_tri_grad(C, C.mesh[mesh_of_tri(C, t)], t, &s_val[e * 9]);
+            s_key[e] = -1;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) s_val[e * 9 + q] = 0.0f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_nused = 0;
+        __syncthreads();
     }
 }
 
@@ -1323,7 +1410,7 @@ struct BatchEngine::Impl {
     }
 
     int vpl = 1, depth = 0;  // voxel-pass variant (voxels per lane, LDS tile depth)
-    bool have_forward = false;  // a forward was issued: tree, occupancy and bounds exist
+    bool have_forward = false;  // a forward was issued: occupancy and cell triangles exist
 
     void launch_forward(hipStream_t st);
     void launch_backward(hipStream_t st);
@@ -1340,19 +1427,31 @@ struct BatchEngine::Impl {
         return (unsigned)cu_count() * (unsigned)std::max(per_cu, 1);
     }
 
-    template <bool BWD>
     void launch_vox(hipStream_t st) {
 #define CMG_VOX(V, DD)                                                 \
     if (vpl == V && depth == DD) {                                    \
         static unsigned grid = 0;                                      \
-        if (!grid) grid = resident_grid(vox_kernel<BWD, V, DD>);       \
-        vox_kernel<BWD, V, DD><<<grid, 64, 0, st>>>(ctx);              \
+        if (!grid) grid = resident_grid(vox_kernel<V, DD>);            \
+        vox_kernel<V, DD><<<grid, 64, 0, st>>>(ctx);                   \
         CMG_HIP_LAUNCH();                                              \
         return;                                                        \
     }
         CMG_VOX(1, 0) CMG_VOX(1, 8) CMG_VOX(2, 0) CMG_VOX(2, 8) CMG_VOX(4, 0) CMG_VOX(4, 8)
 #undef CMG_VOX
         throw HipError("batch: unknown voxel-pass variant");
+    }
+
+    // Persistent workgroups over ranges of 8 consecutive blocks (at most 512 distinct
+    // triangles per range: the table stays at most half full).
+    void launch_adjoint(hipStream_t st) {
+        static unsigned grid = 0;
+        if (!grid) {
+            int per_cu = 0;
+            CMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel, 256, 0));
+            grid = (unsigned)cu_count() * (unsigned)std::max(per_cu, 1);
+        }
+        adjoint_kernel<<<grid, 256, 0, st>>>(ctx, 8);
+        CMG_HIP_LAUNCH();
     }
 };
 
@@ -1400,17 +1499,17 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     launch_tree(st);
     {
         TimedLaunch timed("batch_sdf", st);
-        launch_vox<false>(st);
+        launch_vox(st);
     }
 }
 
 void BatchEngine::Impl::launch_backward(hipStream_t st) {
     Ctx& C = ctx;
-    zero_grad_kernel<<<grid256, 256, 0, st>>>(C.grad, 3 * total_pts, C.ctr);
+    zero_grad_kernel<<<grid256, 256, 0, st>>>(C.grad, 3 * total_pts);
     CMG_HIP_LAUNCH();
     {
         TimedLaunch timed("batch_sdf_backward", st);
-        launch_vox<true>(st);
+        launch_adjoint(st);
     }
 }
 
@@ -1449,7 +1548,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     CMG_HIP(hipMemcpyAsync(lohi.data(), d_lohi, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost, st));
     CMG_HIP(hipStreamSynchronize(st));
     I.h_mesh.resize(nm);
-    long long cells = 0, tiles = 0, words = 0;
+    long long cells = 0, tiles = 0, words = 0, blocked = 0;
     std::vector<double> cost(nm);
     std::vector<long long> nodes_per_level(kMaxLevels + 1, 0);
     I.h_lvl_off.assign((size_t)(kMaxLevels + 1) * (nm + 1), 0);
@@ -1469,6 +1568,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.tbase = tri_start[m];
         M.ntris = (int)(tri_start[m + 1] - tri_start[m]);
         M.cell_base = cells;
+        M.blk_base = blocked;
         OccPlan op = occupancy_plan(g);
         M.words = op.words;
         M.ntx = op.ntx;
@@ -1478,6 +1578,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         M.L0 = make_level(g, kLeaf);
         M.L0b = make_level(g, kLeaf, 2 * kLeaf);
         M.L0c = make_level(g, kLeaf, 4 * kLeaf);
+        blocked += 64LL * M.L0.bx * M.L0.by * M.L0.bz;
         M.sc = search_scale(g);
         M.scf = (float)M.sc;
         M.iscf = (float)(1.0 / M.sc);
@@ -1573,7 +1674,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.masks = I.dalloc<unsigned long long>(words);
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
-    C.best = I.dalloc<float>(cells);
+    C.cell_tri = I.dalloc<int>(blocked);
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
     I.grid256 = (unsigned)cu_count() * 4;
@@ -1633,8 +1734,9 @@ void BatchEngine::run(int flags, const double* d_xyz, float* d_sdf, const float*
         throw InvalidInput("batch: backward needs a forward");
     if (fwd && !d_sdf) throw InvalidInput("batch: forward needs an sdf buffer");
     if (bwd && (!d_gsdf || !d_grad)) throw InvalidInput("batch: backward needs grad buffers");
-    // an occupancy-only pass re-reads the points but leaves the tree and the bounds of
-    // the last forward as they were: a backward needs a new forward after it
+    // an occupancy-only pass rewrites the occupancy from the current points but leaves
+    // the last forward's per-cell triangles as they were: a backward needs a new forward
+    // after it
     if (fwd) I.have_forward = true;
     else if (occ_only) I.have_forward = false;
     I.ctx.sdf = d_sdf;
@@ -1689,9 +1791,7 @@ void BatchEngine::nearest(int* d_tri, hipStream_t st) {
     I.ctx.gsdf = nullptr;
     I.ctx.grad = nullptr;
     I.ctx.nearest = d_tri;
-    reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 2, 2);
-    CMG_HIP_LAUNCH();
-    I.launch_vox<true>(st);
+    I.launch_adjoint(st);
     I.ctx = saved;
 }
 
@@ -1715,11 +1815,11 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
             I.depth = d;
             float total = 0;
             for (int rep = 0; rep < 3; ++rep) {
-                reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 1, 2);
-                zero_grad_kernel<<<I.grid256, 256, 0, st>>>(I.ctx.grad, 3 * I.total_pts, I.ctx.ctr);
+                reset_voxel_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr);
+                zero_grad_kernel<<<I.grid256, 256, 0, st>>>(I.ctx.grad, 3 * I.total_pts);
                 CMG_HIP(hipEventRecord(e0, st));
-                I.launch_vox<false>(st);
-                I.launch_vox<true>(st);
+                I.launch_vox(st);
+                I.launch_adjoint(st);
                 CMG_HIP(hipEventRecord(e1, st));
                 CMG_HIP(hipEventSynchronize(e1));
                 float ms = 0;

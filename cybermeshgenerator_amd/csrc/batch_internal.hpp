@@ -42,6 +42,8 @@ struct MeshDesc {
     int ntris;
     int nlev;             // root level (>= 1)
     long long cell_base;  // first cell in the concatenated outputs
+    long long blk_base;   // first entry of the mesh's blocked per-cell arrays (64 per
+                          // 4x4x4 block of L0)
     long long mask_base;  // first occupancy bit-mask word
     int words, ntx, nty;  // occupancy: mask words per column, column tiles per axis
     int tile_base;        // first column tile in the global tile arrays
@@ -57,8 +59,8 @@ struct MeshDesc {
 enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };
 
 struct BatchCounters {
-    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 backward,
-                                   // 3 tiles redone with exact predicates
+    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 3 tiles
+                                   // redone with exact predicates (2 unused)
     unsigned long long tile_used;  // tile-list pool cursor
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
     // work of the voxel passes, counted per wavefront: cylinder tests of tree nodes,
@@ -75,7 +77,7 @@ enum : int { kBatchForward = 1, kBatchBackward = 2, kBatchGraph = 4, kBatchOccup
 // through LDS as one 1 KiB tile).
 struct BatchVariant {
     int vpl, depth;
-    float ms;  // forward + backward voxel passes, when measured by autotune
+    float ms;  // forward voxel pass + adjoint pass, when measured by autotune
 };
 
 class BatchEngine {
