@@ -223,8 +223,8 @@ __global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n) {
         g[i] = 0.0f;
 }
 
-__global__ void reset_voxel_queue_kernel(BatchCounters* ctr) {
-    if (threadIdx.x == 0) ctr->queue[1] = 0;
+__global__ void reset_queue_kernel(BatchCounters* ctr, int q) {
+    if (threadIdx.x == 0) ctr->queue[q] = 0;
 }
 
 // Morton code (10 bits per axis) of each triangle's centroid within its mesh's grid.
@@ -1160,7 +1160,7 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
 // range of items (neighbouring blocks share their nearest triangles: far from a coarse
 // mesh whole regions share one), so the same vertices are not hammered by every
 // wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
-constexpr int kAdjSlots = 1024;   // LDS table entries per workgroup (power of two)
+constexpr int kAdjSlots = 512;    // LDS table entries per workgroup (power of two)
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
 constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
 
@@ -1197,9 +1197,12 @@ __device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int t
     add_tri_grad(C, M, tri, v9);
 }
 
-// A workgroup (4 wavefronts) takes ranges of per_range consecutive blocks; after each
-// range the table entries taken are added to the vertices and cleared.
-__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, long long per_range) {
+// Workgroups (4 wavefronts) take chunks of kAdjChunk neighbouring blocks from a work
+// queue and walk each chunk four blocks at a time (one per wavefront); the table is added
+// to the vertices and cleared whenever it is a quarter full (an iteration takes at most
+// 256 entries, so the table never passes three quarters) and at the end.
+constexpr int kAdjChunk = 16;
+__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
     __shared__ int s_key[kAdjSlots];
     __shared__ float s_val[kAdjSlots * 9];
     __shared__ int s_used[kAdjSlots];  // entries taken since the last flush
@@ -1207,17 +1210,22 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, long long per_range
     const long long* off = phase_off(C, kPhL0);
     const long long total = off[C.nm];
     const int lane = threadIdx.x & 63;
+    const int nwave = blockDim.x / 64;
     const bool scatter = C.grad != nullptr;
     for (int e = threadIdx.x; e < kAdjSlots; e += blockDim.x) s_key[e] = -1;
     for (int e = threadIdx.x; e < kAdjSlots * 9; e += blockDim.x) s_val[e] = 0.0f;
     if (threadIdx.x == 0) s_nused = 0;
     __syncthreads();
-    const long long nrange = (total + per_range - 1) / per_range;
-    for (long long r = blockIdx.x; r < nrange; r += gridDim.x) {
-        const long long first = r * per_range;
-        const long long last = first + per_range < total ? first + per_range : total;
-        for (long long item = first + (threadIdx.x >> 6); item < last;
-             item += blockDim.x / 64) {
+    long long base = 0, last = 0;
+    while (true) {
+        if (base >= last) {  // next chunk (workgroup-uniform)
+            base = next_item(&C.ctr->queue[2]) * kAdjChunk;
+            last = base + kAdjChunk < total ? base + kAdjChunk : total;
+        }
+        const bool done = base >= total;
+        const long long item = base + (threadIdx.x >> 6);
+        base += nwave;
+        if (!done && item < last) {
             const int pos = find_pos(off, C.nm, item);
             const MeshDesc& M = C.mesh[C.order[pos]];
             const GridSpec& g = M.g;
@@ -1296,9 +1304,16 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, long long per_range
             }
             if (act) table_add(C, M, tri, c, s_key, s_val, s_used, &s_nused);
         }
-        if (!scatter) continue;
+        if (!scatter) {
+            if (done) break;
+            continue;
+        }
         __syncthreads();
         const int nu = s_nused;
+        // every thread has read the count before any wavefront can add to it again: the
+        // flush decision (and the barriers inside it) stays uniform
+        __syncthreads();
+        if (nu <= kAdjSlots / 4 && !done) continue;
         for (int u = threadIdx.x; u < nu; u += blockDim.x) {
             const int e = s_used[u];
             const int t = s_key[e];
@@ -1310,6 +1325,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, long long per_range
         __syncthreads();
         if (threadIdx.x == 0) s_nused = 0;
         __syncthreads();
+        if (done) break;
     }
 }
 
@@ -1441,8 +1457,7 @@ struct BatchEngine::Impl {
         throw HipError("batch: unknown voxel-pass variant");
     }
 
-    // Persistent workgroups over ranges of 8 consecutive blocks (at most 512 distinct
-    // triangles per range: the table stays at most half full).
+    // Persistent workgroups, as many as fit on the device at once.
     void launch_adjoint(hipStream_t st) {
         static unsigned grid = 0;
         if (!grid) {
@@ -1450,7 +1465,9 @@ struct BatchEngine::Impl {
             CMG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel, 256, 0));
             grid = (unsigned)cu_count() * (unsigned)std::max(per_cu, 1);
         }
-        adjoint_kernel<<<grid, 256, 0, st>>>(ctx, 8);
+        reset_queue_kernel<<<1, 64, 0, st>>>(ctx.ctr, 2);
+        CMG_HIP_LAUNCH();
+        adjoint_kernel<<<grid, 256, 0, st>>>(ctx);
         CMG_HIP_LAUNCH();
     }
 };
@@ -1815,7 +1832,7 @@ std::vector<BatchVariant> BatchEngine::autotune(const float* d_gsdf, float* d_sd
             I.depth = d;
             float total = 0;
             for (int rep = 0; rep < 3; ++rep) {
-                reset_voxel_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr);
+                reset_queue_kernel<<<1, 64, 0, st>>>(I.ctx.ctr, 1);
                 zero_grad_kernel<<<I.grid256, 256, 0, st>>>(I.ctx.grad, 3 * I.total_pts);
                 CMG_HIP(hipEventRecord(e0, st));
                 I.launch_vox(st);

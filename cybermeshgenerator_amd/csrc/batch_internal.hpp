@@ -59,8 +59,8 @@ struct MeshDesc {
 enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };
 
 struct BatchCounters {
-    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 3 tiles
-                                   // redone with exact predicates (2 unused)
+    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 adjoint,
+                                   // 3 tiles redone with exact predicates
     unsigned long long tile_used;  // tile-list pool cursor
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
     // work of the voxel passes, counted per wavefront: cylinder tests of tree nodes,
