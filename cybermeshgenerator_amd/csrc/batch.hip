@@ -133,6 +133,7 @@ struct Ctx {  // kernel arguments shared by every phase
                                 // the tree slot of the triangle the cell's distance comes
                                 // from: s >= 0 (FP32 nearest), -2 - s (FP64 near the
                                 // surface), -1 (none)
+    const long long* cell_start;  // [nm + 1] first cell of each mesh id (concatenated)
     float* sdf;
     const float* gsdf;
     float* grad;
@@ -855,8 +856,11 @@ __device__ __forceinline__ size_t blocked_cell(const MeshDesc& M, int i, int j, 
 // reads a tree leaf's triangle records with scalar loads (SGPR operands); D = 8 stages the
 // leaf's 8 records through LDS as one 1 KiB tile. Writes the signed distance and, for the
 // backward pass, the triangle each cell's distance comes from.
+// The register budget keeps at least 5 waves per SIMD for the 4x4x4 variant (at most 96
+// VGPRs), 4 and 3 for the taller ones. (A 6-wave budget spills and measured slower.)
 template <int VPL, int D>
-__global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox_kernel(Ctx C) {
     __shared__ TriRec s_rec[D > 0 ? D : 1];
     const long long* off = phase_off(C, VPL == 1 ? kPhL0 : (VPL == 2 ? kPhL0b : kPhL0c));
     const long long total = off[C.nm];
@@ -1124,27 +1128,60 @@ __global__ __launch_bounds__(64) void vox_kernel(Ctx C) {
                 }
             });
         }
-        const uint8_t* occ = C.occ + M.cell_base;
-        float* out = C.sdf + M.cell_base;
+        // the cell's source triangle; sdf_kernel turns it into the signed distance
         int* const tri_out = C.cell_tri + M.blk_base;
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-            if (!valid[v]) continue;
-            const size_t idx = ((size_t)k[v] * g.ny + j) * g.nx + i;
-            tri_out[blocked_cell(M, i, j, k[v])] = near[v] ? -2 - arg64[v] : arg[v];
-            if (near[v]) {
-                const double d = sqrt(best64[v]);
-                out[idx] = (float)(occ[idx] ? -d : d);  // voxelize.cpp:354-355
-            } else {
-                const float d32 = sqrtf(best[v]) * M.iscf;  // back to world units (exact)
-                out[idx] = occ[idx] ? -d32 : d32;
-            }
-        }
+        for (int v = 0; v < VPL; ++v)
+            if (valid[v]) tri_out[blocked_cell(M, i, j, k[v])] = near[v] ? -2 - arg64[v] : arg[v];
     }
     if (lane == 0) {
         if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
         if (ntest) atomicAdd(&C.ctr->tri_tests, ntest);
         if (neval) atomicAdd(&C.ctr->tri_evals, neval);
+    }
+}
+
+// ---- signed distance from the source triangles -------------------------------------------
+//
+// One cell per thread in flat order (idx = (k ny + j) nx + i per mesh, meshes back to back),
+// so every store instruction writes consecutive floats and the distances leave as whole
+// lines. The distance to the recorded triangle is recomputed with exactly the forward's
+// arithmetic: tri_d2 at the same FP32 point (the FP32 minimum), or the reference's FP64
+// routine near the surface, rounded once (voxelize.cpp:354-355); the sign is the
+// occupancy's.
+__device__ __forceinline__ float cell_sdf(const Ctx& C, const MeshDesc& M, int i, int j,
+                                          int k, uint8_t inside) {
+    const GridSpec& g = M.g;
+    const int code = C.cell_tri[M.blk_base + blocked_cell(M, i, j, k)];
+    if (code <= -2) {
+        const double p64[3] = {g.corner[0] + (i + 0.5) * g.spacing,
+                               g.corner[1] + (j + 0.5) * g.spacing,
+                               g.corner[2] + (k + 0.5) * g.spacing};
+        const TriRec64& T64 = C.rec64[-2 - code];
+        const double d = sqrt(point_tri_dist2_ref(p64, T64.a, T64.b, T64.c));
+        return (float)(inside ? -d : d);
+    }
+    float d2 = FLT_MAX;
+    if (code >= 0) {
+        const float h = (float)g.spacing * M.scf;  // the cell size in search units (exact)
+        d2 = tri_d2(C.rec[code], (i + 0.5f) * h, (j + 0.5f) * h, (k + 0.5f) * h);
+    }
+    const float d32 = sqrtf(d2) * M.iscf;  // back to world units (exact)
+    return inside ? -d32 : d32;
+}
+
+__global__ __launch_bounds__(256) void sdf_kernel(Ctx C, long long total_cells) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    int m = 0;  // the mesh of the thread's current cell (cells only increase)
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < total_cells;
+         c += stride) {
+        while (c >= C.cell_start[m + 1]) ++m;
+        const MeshDesc& M = C.mesh[m];
+        const long long local = c - M.cell_base;
+        const int i = (int)(local % M.g.nx);
+        const int j = (int)((local / M.g.nx) % M.g.ny);
+        const int k = (int)(local / ((long long)M.g.nx * M.g.ny));
+        C.sdf[c] = cell_sdf(C, M, i, j, k, C.occ[c]);
     }
 }
 
@@ -1392,7 +1429,7 @@ struct BatchEngine::Impl {
     std::vector<int> h_order;
     std::vector<long long> h_item_off;
     std::vector<int> h_lvl_off;  // [kMaxLevels + 1][nm + 1]
-    std::vector<long long> h_tri_start, h_pt_start;
+    std::vector<long long> h_tri_start, h_pt_start, h_cell_start;
     long long total_tris = 0, total_pts = 0, total_cells = 0, total_tiles = 0;
     long long total_cols_words = 0;
     int max_lev = 1;
@@ -1517,6 +1554,8 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     {
         TimedLaunch timed("batch_sdf", st);
         launch_vox(st);
+        sdf_kernel<<<grid256 * 4, 256, 0, st>>>(ctx, total_cells);
+        CMG_HIP_LAUNCH();
     }
 }
 
@@ -1692,6 +1731,15 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
     C.cell_tri = I.dalloc<int>(blocked);
+    I.h_cell_start.resize(nm + 1);
+    for (int m = 0; m < nm; ++m) I.h_cell_start[m] = I.h_mesh[m].cell_base;
+    I.h_cell_start[nm] = cells;
+    {
+        long long* d_cs = I.dalloc<long long>(nm + 1);
+        CMG_HIP(hipMemcpyAsync(d_cs, I.h_cell_start.data(), sizeof(long long) * (nm + 1),
+                               hipMemcpyHostToDevice, st));
+        C.cell_start = d_cs;
+    }
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
     I.grid256 = (unsigned)cu_count() * 4;
