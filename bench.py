@@ -16,8 +16,8 @@ steps. The JSON line also carries:
   extra         insphere (2^22 tuples) signs/s, per-stage lane fractions, the SDF
                 workloads (forward+backward voxels/s, batched vs plain launches), the
                 PLC-carve classification (points/s and tets/s, both modes), binary
-                STL input decoded and merged on the GPU, the mesh quality report and
-                batched point location
+                STL input decoded and merged on the GPU, the mesh quality report,
+                batched point location and Delaunay cavities
 All other output goes to stderr; stdout holds exactly one JSON line.
 """
 from __future__ import annotations
@@ -576,6 +576,51 @@ def bench_locate(cm, W, steps, warmup, cpu=True):
                                "kind": "port", "sample": "first 32,768 queries",
                                "identical": bool(np.array_equal(
                                    last["r"][0][:m].cpu().numpy(), want))}
+    res["cavity"] = bench_cavity(W, steps, warmup, cpu)
+    return res
+
+
+def bench_cavity(W, steps, warmup, cpu=True):
+    """Delaunay cavities (the insertion's flood, incremental.cpp:136-151): 2^20 points
+    uniform in the unit cube, located in a Delaunay tetrahedralization of 100,000 uniform
+    points (scipy's Qhull), then every point's cavity from its tet, one batched call."""
+    import torch
+    from cybermeshgenerator_amd.locate import cavity, locate
+    pts_np, tets_np, nbrs_np = W.delaunay_tets(100_000, seed=12)
+    n = 1 << 20
+    q_np = np.random.default_rng(13).uniform(0.001, 0.999, (n, 3))
+    pts, tets, nbrs, q = (torch.tensor(a, device="cuda") for a in (pts_np, tets_np, nbrs_np,
+                                                                 q_np))
+    loc, _, _ = locate(pts, tets, nbrs, q, 0)
+    cnt = torch.empty(n, dtype=torch.int32, device="cuda")
+    cav = torch.full((n, 128), -1, dtype=torch.int32, device="cuda")
+    last = {}
+
+    def step():
+        last["r"] = cavity(pts, tets, nbrs, q, loc, cap=128, count=cnt, out=cav)
+
+    ms = time_steps(step, steps, warmup, torch.cuda.synchronize)
+    count = last["r"][0]
+    res = {"mesh": "Delaunay of 100,000 uniform points", "tets": len(tets_np), "queries": n,
+           "ms_per_batch": ms, "cavities_per_s": n / (ms / 1e3),
+           "mean_cavity_tets": float(count.float().mean().item()),
+           "overflowed": int((count < 0).sum().item())}
+    log(f"[bench] cavity: {n / (ms / 1e3) / 1e6:.1f} M cavities/s ({ms:.2f} ms per 2^20, "
+        f"{res['mean_cavity_tets']:.1f} tets each)")
+    if cpu:
+        import oracle
+        m = 1 << 15
+        loc_np = loc[:m].cpu().numpy()
+        t0 = time.perf_counter()
+        want, want_cav = oracle.cavity(pts_np, tets_np, nbrs_np, q_np[:m], loc_np, cap=128,
+                                       nthreads=1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m / dt, "unit": "cavities/s", "cores": 1,
+                               "kind": "port", "sample": "first 32,768 queries",
+                               "identical": bool(np.array_equal(count[:m].cpu().numpy(), want)
+                                                 and np.array_equal(
+                                                     last["r"][1][:m].cpu().numpy(),
+                                                     want_cav))}
     return res
 
 

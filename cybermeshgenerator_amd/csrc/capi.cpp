@@ -821,4 +821,19 @@ cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const
     });
 }
 
+cmg_status cmg_hip_cavity(const double* xyz, size_t npts, const int* tets, const int* nbrs,
+                          size_t ntets, const double* q, size_t nq, const int* located,
+                          int cap, int* count, int* cavity, uint64_t* bad, void* stream,
+                          char* errbuf, size_t len) {
+    if (bad) *bad = 0;
+    return guarded(errbuf, len, [&] {
+        require_device();
+        g_last_backend = "hip";
+        const long long b = cmg::hip::cavities(xyz, (long long)npts, tets, nbrs,
+                                               (long long)ntets, q, (long long)nq, located,
+                                               cap, count, cavity, as_stream(stream));
+        if (bad) *bad = (uint64_t)b;
+    });
+}
+
 }  // extern "C"

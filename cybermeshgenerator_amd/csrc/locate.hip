@@ -24,8 +24,10 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "expansion.hpp"
 #include "locate_internal.hpp"
 #include "occ_device.hpp"
+#include "pred_naive.hpp"
 
 namespace cmg::hip {
 namespace {
@@ -174,7 +176,144 @@ __global__ __launch_bounds__(256) void locate_kernel(const double* xyz, long lon
     }
 }
 
+// ---- Delaunay cavities ----------------------------------------------------------------
+
+// Exact insphere sign for the rare tuples whose naive determinant is exactly 0. FULL =
+// false tries the register-only stages (fixed point with 62- and 126-bit differences, the
+// double-word bound) and returns kUndecided when they decline (differences spread over
+// more than 126 bits); FULL = true then uses expansions. Out of line, so that neither
+// weighs on the common path.
+constexpr int kUndecided = 3;
+template <bool FULL>
+__device__ __noinline__ int is_exact_slow(const double* v) {
+    int s;
+    if (insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
+        insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s) ||
+        insphere_sign_fixed_wide(v, v + 3, v + 6, v + 9, v + 12, s))
+        return s;
+    if (!FULL) return kUndecided;
+    s = insphere_exact_sign<40, 96>(v, v + 3, v + 6, v + 9, v + 12);
+    if (s == kOverflow) s = insphere_exact_sign<200, 1024>(v, v + 3, v + 6, v + 9, v + 12);
+    return s == kOverflow ? 0 : s;
+}
+
+// The sign of cmg::robust::insphere(a, b, c, d, e) as shipped: the naive FP64 determinant
+// (predicates.cpp:4063-4106) when nonzero, else the exact sign (or kUndecided, FULL =
+// false only).
+template <bool FULL>
+__device__ __forceinline__ int is_compat(const double* v) {
+    const double det = is_naive(v, nullptr);
+    if (det > 0.0) return 1;
+    if (det < 0.0) return -1;
+    return is_exact_slow<FULL>(v);
+}
+
+// The Delaunay cavity of each query point p (incremental.cpp:136-151, the flood the
+// insertion performs before retriangulating): from its located tet t0, breadth first over
+// neighbours k = 0..3, every tet not yet in the cavity whose circumsphere contains p
+// (robust::insphere(v0, v1, v2, v3, p) > 0) joins it. The cavity of query i is written in
+// that discovery order to cav[i*cap ...]; count[i] = its size, or -1 when it has more than
+// cap tets (the first cap are written). ctr[0] counts queries with a tet or neighbour index
+// out of range (count -1 as well); a query whose located tet is -1 gets count 0. Pass
+// FULL = false leaves count -2 on the queries whose insphere tests need expansions
+// (counted in ctr[1]); pass FULL = true redoes exactly those.
+template <bool FULL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cavity_kernel(
+    const double* xyz, long long npts,
+                                                     const int* tets, const int* nbrs,
+                                                     long long ntets, const double* q,
+                                                     long long nq, const int* located,
+                                                     int cap, int* count, int* cav,
+                                                     unsigned long long* ctr) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nq;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (FULL && count[i] != -2) continue;
+        const int t0 = located[i];
+        if (t0 < 0) {
+            count[i] = 0;
+            continue;
+        }
+        int* list = cav + (size_t)i * cap;
+        bool bad = t0 >= ntets, full = false, undecided = false;
+        int n = 0;
+        if (!bad) list[n++] = t0;
+        double v[15];
+        v[12] = q[3 * i];
+        v[13] = q[3 * i + 1];
+        v[14] = q[3 * i + 2];
+        for (int h = 0; h < n && !bad && !full && !undecided; ++h) {
+            const int4 nb = *reinterpret_cast<const int4*>(nbrs + 4 * (long long)list[h]);
+            const int nk[4] = {nb.x, nb.y, nb.z, nb.w};
+            for (int k = 0; k < 4 && !bad && !full && !undecided; ++k) {
+                const int m = nk[k];
+                if (m < 0) continue;
+                if (m >= ntets) {
+                    bad = true;
+                    break;
+                }
+                bool in = false;
+                for (int e = 0; e < n && !in; ++e) in = list[e] == m;
+                if (in) continue;
+                const int4 tv = *reinterpret_cast<const int4*>(tets + 4 * (long long)m);
+                if (!tet_ok(tv, npts)) {
+                    bad = true;
+                    break;
+                }
+                const int id[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v[3 * a + c] = xyz[3 * (long long)id[a] + c];
+                const int sg = is_compat<FULL>(v);
+                if (sg == kUndecided) {
+                    undecided = true;
+                } else if (sg > 0) {
+                    if (n == cap) full = true;
+                    else list[n++] = m;
+                }
+            }
+        }
+        if (bad) atomicAdd(&ctr[0], 1ull);
+        if (undecided && !bad) atomicAdd(&ctr[1], 1ull);
+        count[i] = bad || full ? -1 : (undecided ? -2 : n);
+    }
+}
+
 }  // namespace
+
+long long cavities(const double* d_xyz, long long npts, const int* d_tets, const int* d_nbrs,
+                   long long ntets, const double* d_q, long long nq, const int* d_located,
+                   int cap, int* d_count, int* d_cav, hipStream_t st) {
+    if (nq < 0 || ntets < 0 || npts < 0) throw InvalidInput("cavity: negative count");
+    if (cap < 1) throw InvalidInput("cavity: cap must be >= 1");
+    if (nq == 0) return 0;
+    if (ntets < 1) throw InvalidInput("cavity: no tetrahedra");
+    if (ntets >= (1LL << 31)) throw InvalidInput("cavity: more than 2^31 - 1 tetrahedra");
+    if (!d_xyz || !d_tets || !d_nbrs || !d_q || !d_located || !d_count || !d_cav)
+        throw InvalidInput("cavity: null pointer");
+    if (reinterpret_cast<uintptr_t>(d_tets) % 16 || reinterpret_cast<uintptr_t>(d_nbrs) % 16)
+        throw InvalidInput("cavity: tets and nbrs must be 16-byte aligned");
+    ScratchLease lease;
+    auto* ctr = static_cast<unsigned long long*>(lease.s->get(256));
+    CMG_HIP(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned long long), st));
+    {
+        TimedLaunch tl("cavity", st);
+        cavity_kernel<false><<<grid_for((size_t)nq, 256, 8), 256, 0, st>>>(
+            d_xyz, npts, d_tets, d_nbrs, ntets, d_q, nq, d_located, cap, d_count, d_cav, ctr);
+        CMG_HIP_LAUNCH();
+    }
+    unsigned long long c[2] = {0, 0};
+    CMG_HIP(hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, st));
+    CMG_HIP(hipStreamSynchronize(st));
+    if (c[1]) {  // queries whose insphere tests need expansions: redone with them
+        cavity_kernel<true><<<grid_for((size_t)nq, 256, 2), 256, 0, st>>>(
+            d_xyz, npts, d_tets, d_nbrs, ntets, d_q, nq, d_located, cap, d_count, d_cav, ctr);
+        CMG_HIP_LAUNCH();
+        CMG_HIP(hipMemcpyAsync(c, ctr, sizeof c, hipMemcpyDeviceToHost, st));
+        CMG_HIP(hipStreamSynchronize(st));
+    }
+    return (long long)c[0];
+}
 
 LocateStats locate_points(const double* d_xyz, long long npts, const int* d_tets,
                           const int* d_nbrs, long long ntets, const double* d_q, long long nq,

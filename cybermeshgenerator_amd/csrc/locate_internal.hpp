@@ -19,4 +19,14 @@ LocateStats locate_points(const double* d_xyz, long long npts, const int* d_tets
                           const int* d_start, int start_tet, int* d_out, int* d_steps,
                           hipStream_t st);
 
+// Delaunay cavities of query points against a fixed mesh (incremental.cpp:136-151): for
+// query i located in d_located[i], the tets reached breadth first from it through
+// neighbours whose circumsphere contains the point (compat insphere > 0), in discovery
+// order, into d_cav[i*cap ...]; d_count[i] = the cavity size, -1 when it exceeds cap or an
+// index is out of range, 0 when d_located[i] < 0. Returns the number of queries that met
+// an index out of range. Synchronises st.
+long long cavities(const double* d_xyz, long long npts, const int* d_tets, const int* d_nbrs,
+                   long long ntets, const double* d_q, long long nq, const int* d_located,
+                   int cap, int* d_count, int* d_cav, hipStream_t st);
+
 }  // namespace cmg::hip

@@ -468,3 +468,20 @@ def tet_neighbors(tets: np.ndarray) -> np.ndarray:
     nbrs[owner[b], slot[b]] = owner[a]
     return nbrs
 
+
+
+def delaunay_tets(n: int, seed: int = 8, lattice: int = 0):
+    """A Delaunay tetrahedralization (scipy's Qhull) of n seeded points uniform in the
+    unit cube, or, with lattice > 0, drawn from a lattice of `lattice` cells per axis
+    (many exactly cospherical tuples). Returns (points, tets positively oriented, nbrs)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    if lattice:
+        g = rng.integers(0, lattice + 1, size=(4 * n, 3))
+        pts = np.unique(g, axis=0)[:n].astype(np.float64) / lattice
+        pts = pts[rng.permutation(len(pts))]
+    else:
+        pts = rng.uniform(0.0, 1.0, (n, 3))
+    tets = Delaunay(pts).simplices.astype(np.int32)
+    tets = positive_tets(pts, tets)
+    return pts, tets, tet_neighbors(tets)

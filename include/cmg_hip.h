@@ -14,7 +14,8 @@
  *  5. The PLC carve's point-in-domain classification (device and host pointers).
  *  6. Surface files (STL, OBJ) to PLCs and to device arrays.
  *  7. The mesh quality report (device pointers).
- *  8. Batched point location in a tetrahedral mesh (device pointers).
+ *  8. Batched point location and Delaunay cavities in a tetrahedral mesh (device
+ *     pointers).
  *
  * Every entry point runs on the GPU. There is no CPU fallback: with no usable HIP
  * device a call returns CMG_ERR_INTERNAL and fills errbuf. Inputs must be finite.
@@ -413,6 +414,22 @@ cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const
                           size_t ntets, const double* q, size_t nq, const int* start,
                           int start_tet, int* out, int* steps, uint64_t* brute,
                           void* stream, char* errbuf, size_t errbuf_len);
+
+/* Delaunay cavities of query points in a fixed mesh: the flood Triangulation::insert
+ * performs before it retriangulates (src/delaunay/incremental.cpp:136-151; conflict at
+ * :132-141), as one batched call. For query i located in tet located[i] (device, e.g. from
+ * cmg_hip_locate), breadth first from that tet through neighbours k = 0..3, every tet not
+ * yet in the cavity whose circumsphere contains q[i] (the compat insphere,
+ * robust::insphere(v0, v1, v2, v3, q) > 0) joins it. cavity[i*cap ...] (device, nq * cap
+ * ints) receives the cavity in that discovery order, count[i] (device) its size: -1 when
+ * it has more than cap tets (the first cap are written) or meets an index out of range,
+ * 0 when located[i] < 0. tets (positively oriented) and nbrs as for cmg_hip_locate, both
+ * 16-byte aligned. *bad (optional) = queries that met an index out of range. Synchronises
+ * `stream`. */
+cmg_status cmg_hip_cavity(const double* xyz, size_t npts, const int* tets, const int* nbrs,
+                          size_t ntets, const double* q, size_t nq, const int* located,
+                          int cap, int* count, int* cavity, uint64_t* bad, void* stream,
+                          char* errbuf, size_t errbuf_len);
 
 #ifdef __cplusplus
 } /* extern "C" */

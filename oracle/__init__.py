@@ -106,6 +106,7 @@ def _load() -> ctypes.CDLL:
     lib.orc_locate.restype = i32
     lib.orc_locate.argtypes = [P, P, P, sz, P, i32, P]
     lib.orc_locate_batch.argtypes = [P, P, P, sz, P, sz, P, i32, P, P, i32]
+    lib.orc_cavity_batch.argtypes = [P, P, P, P, sz, P, i32, P, P, i32]
     return lib
 
 
@@ -588,3 +589,19 @@ def locate(pts, tets, nbrs, q, start=0, nthreads=0):
         lib().orc_locate_batch(_p(pts), _p(tets), _p(nbrs), tets.shape[0], _p(q), q.shape[0],
                                _p(st), 0, _p(out), _p(steps), nthreads)
     return out, steps
+
+
+def cavity(pts, tets, nbrs, q, located, cap=256, nthreads=0):
+    """(count (nq,), cavities (nq, cap)) as Triangulation::insert floods them
+    (incremental.cpp:136-151): count -1 past cap, 0 where located is -1; row i holds the
+    cavity of q[i] in discovery order."""
+    pts = _f64(pts).reshape(-1, 3)
+    tets = _i32(tets).reshape(-1, 4)
+    nbrs = _i32(nbrs).reshape(-1, 4)
+    q = _f64(q).reshape(-1, 3)
+    located = _i32(located).ravel()
+    count = np.zeros(q.shape[0], dtype=np.int32)
+    out = np.full((q.shape[0], cap), -1, dtype=np.int32)
+    lib().orc_cavity_batch(_p(pts), _p(tets), _p(nbrs), _p(q), q.shape[0], _p(located), cap,
+                           _p(count), _p(out), nthreads)
+    return count, out

@@ -273,3 +273,41 @@ void orc_locate_batch(const double* pts, const int* tets, const int* nbrs, size_
         out[i] = orc_locate(pts, tets, nbrs, ntets, q + 3 * i, start ? start[i] : start_tet,
                             steps ? steps + i : 0);
 }
+
+/* Delaunay cavity of point q (src/delaunay/incremental.cpp:136-151, the flood before the
+ * retriangulation, unweighted): from tet t0, breadth first, each neighbour m (k = 0..3 in
+ * order) not yet in the cavity joins it when robust::insphere(v0, v1, v2, v3, q) > 0 (the
+ * compat sign). Writes the cavity in discovery order to out[0..cap) and returns its size,
+ * or -1 when it has more than cap tets. */
+int orc_cavity(const double* pts, const int* tets, const int* nbrs, const double* q, int t0,
+               int cap, int* out) {
+    if (t0 < 0) return 0;
+    int n = 0;
+    out[n++] = t0;
+    for (int h = 0; h < n; ++h) {
+        const int* nb = nbrs + 4 * (size_t)out[h];
+        for (int k = 0; k < 4; ++k) {
+            const int m = nb[k];
+            if (m < 0) continue;
+            int in = 0;
+            for (int e = 0; e < n && !in; ++e) in = out[e] == m;
+            if (in) continue;
+            const int* v = tets + 4 * (size_t)m;
+            if (orc_insphere_sign(pts + 3 * (size_t)v[0], pts + 3 * (size_t)v[1],
+                                  pts + 3 * (size_t)v[2], pts + 3 * (size_t)v[3], q,
+                                  ORC_COMPAT) > 0) {
+                if (n == cap) return -1;
+                out[n++] = m;
+            }
+        }
+    }
+    return n;
+}
+
+void orc_cavity_batch(const double* pts, const int* tets, const int* nbrs, const double* q,
+                      size_t nq, const int* located, int cap, int* count, int* out,
+                      int nthreads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
+    for (size_t i = 0; i < nq; ++i)
+        count[i] = orc_cavity(pts, tets, nbrs, q + 3 * i, located[i], cap, out + i * (size_t)cap);
+}

@@ -1,7 +1,8 @@
 """Parity of the batched GPU point location (cmg_hip_locate) with the CPU oracle of
 src/delaunay/incremental.cpp:321-358: same tet and same walk length for every query,
 from a common start and from per-query starts, inside and outside the hull, and through
-the step-limit fallback."""
+the step-limit fallback; and of the batched Delaunay cavities (cmg_hip_cavity) with the
+oracle's flood (:136-151): same tets in the same discovery order."""
 import numpy as np
 import pytest
 
@@ -69,3 +70,50 @@ def test_query_count_limit(gpu):
                                   v(torch.cuda.current_stream().cuda_stream), err, 128)
     assert st == 1 and b"2^32" in err.value
     assert int(out.abs().sum()) == 0
+
+
+@pytest.mark.parametrize("lattice", [0, 24])
+def test_cavities_match_the_oracle(gpu, lattice):
+    """Queries located on the GPU in a Delaunay mesh (uniform points, or lattice points
+    with many exactly cospherical tuples and queries on circumspheres, where the compat
+    insphere falls back to its exact sign), then their cavities: counts and tets in
+    discovery order identical to the oracle's; a small cap marks the larger cavities."""
+    from cybermeshgenerator_amd import workloads as W
+    from cybermeshgenerator_amd.locate import cavity, locate
+    # lattice: 3000 of the 25^3 lattice points, queries on the lattice (about a quarter
+    # of the boundary tests have the query exactly on a circumsphere)
+    pts, tets, nbrs = W.delaunay_tets(3000 if lattice else 20000, seed=41, lattice=lattice)
+    rng = np.random.default_rng(42)
+    if lattice:
+        q = rng.integers(1, lattice, size=(1 << 14, 3)).astype(np.float64) / lattice
+    else:
+        q = rng.uniform(0.02, 0.98, (1 << 14, 3))
+    print(f"cavity: {len(tets)} tets, {len(q)} queries, lattice {lattice}", flush=True)
+    loc, _, _ = locate(pts, tets, nbrs, q, 0)
+    loc_np = loc.cpu().numpy()
+    assert np.array_equal(loc_np, oracle.locate(pts, tets, nbrs, q, 0)[0])
+    count, cav, bad = cavity(pts, tets, nbrs, q, loc, cap=256)
+    want_count, want_cav = oracle.cavity(pts, tets, nbrs, q, loc_np, cap=256)
+    count = count.cpu().numpy()
+    assert bad == 0 and (count > 0).all() and count.max() > 12
+    assert np.array_equal(count, want_count)
+    assert np.array_equal(cav.cpu().numpy(), want_cav)
+    print(f"cavity sizes: mean {count.mean():.1f}, max {count.max()}", flush=True)
+    small, _, _ = cavity(pts, tets, nbrs, q, loc, cap=12)
+    assert np.array_equal(small.cpu().numpy(), np.where(want_count > 12, -1, want_count))
+
+
+def test_cavity_edge_cases(gpu):
+    """A query located at -1 gets an empty cavity; a neighbour index out of range fails
+    that query (count -1) and is counted, without a read through it."""
+    from cybermeshgenerator_amd import workloads as W
+    from cybermeshgenerator_amd.locate import cavity
+    pts, tets, nbrs = W.delaunay_tets(500, seed=43)
+    q = np.array([[0.5, 0.5, 0.5], [0.4, 0.6, 0.5]])
+    loc = oracle.locate(pts, tets, nbrs, q, 0)[0]
+    count, _, bad = cavity(pts, tets, nbrs, q, np.array([-1, loc[1]], np.int32))
+    assert count.cpu().numpy()[0] == 0 and count.cpu().numpy()[1] > 0 and bad == 0
+    broken = nbrs.copy()
+    broken[broken >= 0] = len(tets) + 7
+    count, _, bad = cavity(pts, tets, broken, q, loc.astype(np.int32))
+    assert (count.cpu().numpy() == -1).all() and bad == 2

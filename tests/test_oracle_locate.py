@@ -80,3 +80,76 @@ def test_step_limit_falls_back_to_brute_force():
     inside_b = np.array([[0.6, 0.6, 0.6]])
     got, steps = oracle.locate(pts, tets, nbrs, inside_b, 0)
     assert got[0] == 1 and steps[0] >= 0
+
+
+def _conflict_component(pts, tets, nbrs, q, t0):
+    """Brute force: the compat insphere sign of q against every tet, then the tets
+    connected to t0 through conflicting neighbours (a set; t0 itself needs no test)."""
+    soa = np.concatenate([pts[tets].reshape(len(tets), 12),
+                          np.tile(q, (len(tets), 1))], 1).T.copy()
+    conflict = oracle.insphere_batch(soa, oracle.COMPAT) > 0
+    seen = {t0}
+    stack = [t0]
+    while stack:
+        t = stack.pop()
+        for m in nbrs[t]:
+            if m >= 0 and m not in seen and conflict[m]:
+                seen.add(m)
+                stack.append(m)
+    return seen
+
+
+def _cavity_case(lattice):
+    pts, tets, nbrs = W.delaunay_tets(3000, seed=31, lattice=lattice)
+    rng = np.random.default_rng(32)
+    if lattice:
+        q = rng.integers(1, lattice, size=(120, 3)).astype(np.float64) / lattice
+    else:
+        q = rng.uniform(0.05, 0.95, (120, 3))
+    loc, _ = oracle.locate(pts, tets, nbrs, q, 0)
+    return pts, tets, nbrs, q, loc
+
+
+def test_cavity_equals_the_conflicting_component():
+    """orc_cavity (incremental.cpp:136-151) against brute force: the set of tets connected
+    to the located tet through neighbours whose circumsphere contains the point, the
+    insphere signs computed for every tet; uniform points and lattice points (exactly
+    cospherical tuples and points on circumspheres)."""
+    for lattice in (0, 24):
+        pts, tets, nbrs, q, loc = _cavity_case(lattice)
+        count, cav = oracle.cavity(pts, tets, nbrs, q, loc, cap=512)
+        assert (count > 0).all()
+        for i in range(len(q)):
+            got = cav[i, :count[i]]
+            assert got[0] == loc[i] and len(set(got.tolist())) == count[i]
+            assert set(got.tolist()) == _conflict_component(pts, tets, nbrs, q[i], loc[i])
+
+
+def test_cavity_is_star_shaped_from_the_point():
+    """Bowyer-Watson: in a Delaunay triangulation the cavity of an inserted point is star-
+    shaped from it, so every boundary face of the cavity makes a positively oriented tet
+    with the point (orientation of the face with q in place of the opposite vertex, by
+    exact arithmetic); on uniform points, where no point lies on a boundary face."""
+    pts, tets, nbrs, q, loc = _cavity_case(0)
+    count, cav = oracle.cavity(pts, tets, nbrs, q, loc, cap=512)
+    faces = 0
+    for i in range(len(q)):
+        inside = set(cav[i, :count[i]].tolist())
+        for t in inside:
+            for k in range(4):
+                if nbrs[t, k] in inside:
+                    continue
+                v = pts[tets[t]].copy()
+                v[k] = q[i]
+                assert oracle.orient3d_sign(*v, oracle.EXACT) > 0
+                faces += 1
+    assert faces > 10 * len(q)
+
+
+def test_cavity_cap():
+    pts, tets, nbrs, q, loc = _cavity_case(0)
+    full, _ = oracle.cavity(pts, tets, nbrs, q, loc, cap=512)
+    small, cav = oracle.cavity(pts, tets, nbrs, q, loc, cap=8)
+    assert ((small == -1) == (full > 8)).all() and (small[full <= 8] == full[full <= 8]).all()
+    none, _ = oracle.cavity(pts, tets, nbrs, q[:3], np.full(3, -1, np.int32))
+    assert (none == 0).all()
