@@ -1170,11 +1170,20 @@ __device__ __forceinline__ float cell_sdf(const Ctx& C, const MeshDesc& M, int i
     return inside ? -d32 : d32;
 }
 
+// A workgroup takes chunks of 256 consecutive cells. Chunks go to the XCDs in runs of
+// kSdfRun (workgroup w runs on XCD w % 8: dispatch is round robin), so the neighbouring
+// rows and slices of a run, which share their triangle records, are read through one L2.
+// (gridDim.x is a multiple of 8; any other dispatch only loses that locality.)
+constexpr long long kSdfRun = 64;
 __global__ __launch_bounds__(256) void sdf_kernel(Ctx C, long long total_cells) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
+    const long long nchunk = (total_cells + 255) / 256;
     int m = 0;  // the mesh of the thread's current cell (cells only increase)
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < total_cells;
-         c += stride) {
+    for (long long s = blockIdx.x >> 3;; s += per_xcd) {
+        const long long chunk = ((s / kSdfRun) * 8 + xcd) * kSdfRun + s % kSdfRun;
+        if (chunk >= nchunk) break;  // chunks only grow with s
+        const long long c = chunk * 256 + threadIdx.x;
+        if (c >= total_cells) break;
         while (c >= C.cell_start[m + 1]) ++m;
         const MeshDesc& M = C.mesh[m];
         const long long local = c - M.cell_base;
