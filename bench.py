@@ -581,7 +581,7 @@ def bench_locate(cm, W, steps, warmup, cpu=True):
 
 
 def bench_cavity(W, steps, warmup, cpu=True):
-    """Delaunay cavities (the insertion's flood, incremental.cpp:136-151): 2^20 points
+    """Delaunay cavities (the insertion's flood, incremental.cpp:148-161): 2^20 points
     uniform in the unit cube, located in a Delaunay tetrahedralization of 100,000 uniform
     points (scipy's Qhull), then every point's cavity from its tet, one batched call."""
     import torch

@@ -296,13 +296,14 @@ class PLC:
 
     def __init__(self):
         self._handle = _lib.cmg_plc_create()
+        self._destroy = _lib.cmg_plc_destroy  # usable at interpreter exit
         self._points = np.zeros((0, 3))
         self._facets: list[np.ndarray] = []
 
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h:
-            _lib.cmg_plc_destroy(h)
+            self._destroy(h)
             self._handle = None
 
     def add_points(self, points) -> "PLC":

@@ -110,6 +110,9 @@ class SdfBatch:
             err, len(err))
         _check(st, err)
         self._h = h.value
+        # held by the object: module globals may already be gone when it is collected at
+        # interpreter exit
+        self._destroy = lib().cmg_voxelize_batch_destroy
         self.n = n
         self.cells = int(lib().cmg_voxelize_batch_cells(self._h))
         self.grids = []
@@ -126,7 +129,7 @@ class SdfBatch:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().cmg_voxelize_batch_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def __del__(self):

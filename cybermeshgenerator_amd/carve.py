@@ -87,11 +87,12 @@ class CarveGrid:
                                     len(err))
         _check(st, err)
         self._h = h
+        self._destroy = lib().cmg_carve_destroy  # usable at interpreter exit
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
         if h:
-            lib().cmg_carve_destroy(h)
+            self._destroy(h)
 
     def points_inside(self, q, out=None, stream=None):
         """uint8 (N,): 1 where q[i] (float64 (N, 3) CUDA tensor) is inside."""

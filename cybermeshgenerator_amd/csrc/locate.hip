@@ -208,7 +208,7 @@ __device__ __forceinline__ int is_compat(const double* v) {
     return is_exact_slow<FULL>(v);
 }
 
-// The Delaunay cavity of each query point p (incremental.cpp:136-151, the flood the
+// The Delaunay cavity of each query point p (incremental.cpp:148-161, the flood the
 // insertion performs before retriangulating): from its located tet t0, breadth first over
 // neighbours k = 0..3, every tet not yet in the cavity whose circumsphere contains p
 // (robust::insphere(v0, v1, v2, v3, p) > 0) joins it. The cavity of query i is written in

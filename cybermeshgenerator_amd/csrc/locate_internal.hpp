@@ -19,7 +19,7 @@ LocateStats locate_points(const double* d_xyz, long long npts, const int* d_tets
                           const int* d_start, int start_tet, int* d_out, int* d_steps,
                           hipStream_t st);
 
-// Delaunay cavities of query points against a fixed mesh (incremental.cpp:136-151): for
+// Delaunay cavities of query points against a fixed mesh (incremental.cpp:148-161): for
 // query i located in d_located[i], the tets reached breadth first from it through
 // neighbours whose circumsphere contains the point (compat insphere > 0), in discovery
 // order, into d_cav[i*cap ...]; d_count[i] = the cavity size, -1 when it exceeds cap or an

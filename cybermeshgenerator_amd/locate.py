@@ -1,7 +1,7 @@
 """Batched point location in a tetrahedral mesh (cmg_hip_locate): one visibility walk per
 query with the rules of the Delaunay insertion's locate
 (src/delaunay/incremental.cpp:321-358); and the Delaunay cavity of each located query
-(cmg_hip_cavity, the insertion's flood, :136-151).
+(cmg_hip_cavity, the insertion's flood, :148-161).
 
 points float64 (V, 3), tets int32 (T, 4) positively oriented, nbrs int32 (T, 4) (the tet
 across the face opposite each vertex, -1 on the hull), queries float64 (N, 3): numpy
@@ -68,7 +68,7 @@ def locate(points, tets, nbrs, queries, start=0, stream=None):
 
 
 def cavity(points, tets, nbrs, queries, located, cap=64, stream=None, count=None, out=None):
-    """Delaunay cavity of every query from its located tet (incremental.cpp:136-151):
+    """Delaunay cavity of every query from its located tet (incremental.cpp:148-161):
     returns (count (N,) int32 CUDA tensor: the cavity size, -1 past cap or on a bad index,
     0 where located is -1; cavities (N, cap) int32 CUDA tensor, row i in discovery order
     and -1 after it unless `out` (N, cap) is given, whose entries past each cavity are left

@@ -416,7 +416,7 @@ cmg_status cmg_hip_locate(const double* xyz, size_t npts, const int* tets, const
                           void* stream, char* errbuf, size_t errbuf_len);
 
 /* Delaunay cavities of query points in a fixed mesh: the flood Triangulation::insert
- * performs before it retriangulates (src/delaunay/incremental.cpp:136-151; conflict at
+ * performs before it retriangulates (src/delaunay/incremental.cpp:148-161; conflict at
  * :132-141), as one batched call. For query i located in tet located[i] (device, e.g. from
  * cmg_hip_locate), breadth first from that tet through neighbours k = 0..3, every tet not
  * yet in the cavity whose circumsphere contains q[i] (the compat insphere,

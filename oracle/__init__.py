@@ -593,7 +593,7 @@ def locate(pts, tets, nbrs, q, start=0, nthreads=0):
 
 def cavity(pts, tets, nbrs, q, located, cap=256, nthreads=0):
     """(count (nq,), cavities (nq, cap)) as Triangulation::insert floods them
-    (incremental.cpp:136-151): count -1 past cap, 0 where located is -1; row i holds the
+    (incremental.cpp:148-161): count -1 past cap, 0 where located is -1; row i holds the
     cavity of q[i] in discovery order."""
     pts = _f64(pts).reshape(-1, 3)
     tets = _i32(tets).reshape(-1, 4)

@@ -241,7 +241,7 @@ void orc_locate_batch(const double* pts, const int* tets, const int* nbrs, size_
                       const double* q, size_t nq, const int* start, int start_tet,
                       int* out, int* steps, int nthreads);
 
-/* Delaunay cavity of q from its located tet t0 (src/delaunay/incremental.cpp:136-151):
+/* Delaunay cavity of q from its located tet t0 (src/delaunay/incremental.cpp:148-161):
  * breadth first through neighbours k = 0..3, tets whose circumsphere contains q
  * (compat insphere > 0) join, in discovery order; returns the size (-1 past cap; 0 for
  * t0 < 0). */

@@ -274,7 +274,7 @@ void orc_locate_batch(const double* pts, const int* tets, const int* nbrs, size_
                             steps ? steps + i : 0);
 }
 
-/* Delaunay cavity of point q (src/delaunay/incremental.cpp:136-151, the flood before the
+/* Delaunay cavity of point q (src/delaunay/incremental.cpp:148-161, the flood before the
  * retriangulation, unweighted): from tet t0, breadth first, each neighbour m (k = 0..3 in
  * order) not yet in the cavity joins it when robust::insphere(v0, v1, v2, v3, q) > 0 (the
  * compat sign). Writes the cavity in discovery order to out[0..cap) and returns its size,

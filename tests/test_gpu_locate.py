@@ -2,7 +2,7 @@
 src/delaunay/incremental.cpp:321-358: same tet and same walk length for every query,
 from a common start and from per-query starts, inside and outside the hull, and through
 the step-limit fallback; and of the batched Delaunay cavities (cmg_hip_cavity) with the
-oracle's flood (:136-151): same tets in the same discovery order."""
+oracle's flood (:148-161): same tets in the same discovery order."""
 import numpy as np
 import pytest
 

@@ -111,7 +111,7 @@ def _cavity_case(lattice):
 
 
 def test_cavity_equals_the_conflicting_component():
-    """orc_cavity (incremental.cpp:136-151) against brute force: the set of tets connected
+    """orc_cavity (incremental.cpp:148-161) against brute force: the set of tets connected
     to the located tet through neighbours whose circumsphere contains the point, the
     insphere signs computed for every tet; uniform points and lattice points (exactly
     cospherical tuples and points on circumspheres)."""
