@@ -180,9 +180,28 @@ __device__ __forceinline__ float node_lb2(const NodeRec& N, float px, float py, 
 // true when the predicate holds on any lane of the wavefront
 __device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0; }
 
+// One step of a DPP max reduction: x = max(x, x of the lane `ctrl` selects), on the rows
+// row_mask enables (the other lanes keep x).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max(float x) {
+    const int xi = __builtin_bit_cast(int, x);
+    const int y = __builtin_amdgcn_update_dpp(xi, xi, CTRL, ROW_MASK, 0xF, false);
+    return fmaxf(x, __builtin_bit_cast(float, y));
+}
+
+// Largest x over the wavefront (every lane active), as a wave-uniform value: a max within
+// quads (quad_perm [1,0,3,2], [2,3,0,1]), within rows of 16 (row_half_mirror,
+// row_mirror), then across rows (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2
+// and 3), so lane 63 holds the maximum. Uniform by construction (readlane), so a branch
+// on it keeps the code after it scalar.
 __device__ __forceinline__ float wave_max(float x) {
-    for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
-    return x;
+    x = dpp_max<0xB1, 0xF>(x);
+    x = dpp_max<0x4E, 0xF>(x);
+    x = dpp_max<0x141, 0xF>(x);
+    x = dpp_max<0x140, 0xF>(x);
+    x = dpp_max<0x142, 0xA>(x);
+    x = dpp_max<0x143, 0xC>(x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
 }
 
 // Squared lower bound from the oriented box N to every point of the axis-aligned block
