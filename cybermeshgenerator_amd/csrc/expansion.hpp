@@ -473,43 +473,58 @@ __host__ __device__ __forceinline__ long long scaled_int(double x, int s) {
     return (bits >> 63) ? -q : q;
 }
 
-// Exact insphere sign in fixed point; false when a difference is subnormal or the
-// differences span more than 62 bits (the caller then uses expansion arithmetic).
-__host__ __device__ inline bool insphere_sign_fixed(const double* pa, const double* pb, const double* pc,
-                                           const double* pd, const double* pe, int& sign) {
+// The twelve differences p - pe (vertex v = a, b, c, d; axis a at 3v + a) as Two-Sum
+// pairs x + y, and the bit range of their nonzero parts: lowest set bit 2^lo, magnitudes
+// below 2^hi (hi < lo when every difference is zero). False when a part is subnormal.
+struct InsDiffs {
+    double x[12], y[12];
+    int lo, hi;
+};
+
+__host__ __device__ __forceinline__ bool insphere_diffs(const double* pa, const double* pb,
+                                                        const double* pc, const double* pd,
+                                                        const double* pe, InsDiffs& d) {
     const double* P[4] = {pa, pb, pc, pd};
-    double dx[12], dy[12];
-    int lo = 1 << 20, hi = -(1 << 20);
+    d.lo = 1 << 20;
+    d.hi = -(1 << 20);
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             double x, y;
             two_sum(P[v][a], -pe[a], x, y);
-            dx[3 * v + a] = x;
-            dy[3 * v + a] = y;
+            d.x[3 * v + a] = x;
+            d.y[3 * v + a] = y;
             const double xy[2] = {x, y};
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 if (xy[q] == 0.0) continue;
                 int l, h;
                 if (!bit_range(xy[q], l, h)) return false;
-                lo = l < lo ? l : lo;
-                hi = h > hi ? h : hi;
+                d.lo = l < d.lo ? l : d.lo;
+                d.hi = h > d.hi ? h : d.hi;
             }
         }
-    if (hi < lo) {  // every difference is zero
-        sign = 0;
-        return true;
-    }
-    if (hi - lo > 62) return false;
-    // Stay where every product of five differences and its rounding-error terms is a
-    // normal double (components within [2^-150, 2^150]): there the expansion arithmetic
-    // is exact too, so both stages give the same sign; outside, leave the lane to it.
-    if (hi > 150 || lo < -150) return false;
+    return true;
+}
+
+// Whether the fixed-point stages may evaluate d with differences of at most `bits` bits.
+// Besides the width, stay where every product of five differences and its rounding-error
+// terms is a normal double (components within [2^-150, 2^150]): there the expansion
+// arithmetic is exact too, so every stage gives the same sign; outside, leave the lane to
+// the expansions.
+__host__ __device__ __forceinline__ bool insphere_fixed_fits(const InsDiffs& d, int bits) {
+    return d.hi - d.lo <= bits && d.hi <= 150 && d.lo >= -150;
+}
+
+// Exact insphere sign from differences of at most 62 bits: the determinant is an integer
+// polynomial in the 62-bit integers D = (x + y) / 2^lo, evaluated in two's complement
+// multi-limb integers: lifts (unsigned, < 2^126), xy minors (signed 128-bit), cofactors
+// (192-bit), lifted products and their sum (320-bit).
+__host__ __device__ inline int insphere_fixed62_sign(const InsDiffs& d) {
     long long D[12];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) D[i] = scaled_int(dx[i], lo) + scaled_int(dy[i], lo);
+    for (int i = 0; i < 12; ++i) D[i] = scaled_int(d.x[i], d.lo) + scaled_int(d.y[i], d.lo);
     // D[3v + a]: vertex v (a, b, c, d) minus e, axis a
     u128 lift[4];
 #pragma unroll
@@ -548,11 +563,108 @@ __host__ __device__ inline bool insphere_sign_fixed(const double* pa, const doub
     t = mul_lift_cof(lift[A], bcd);
     limbs_neg(t.w);
     limbs_add(det.w, t.w);
-    if ((det.w[0] | det.w[1] | det.w[2] | det.w[3] | det.w[4]) == 0) {
-        sign = 0;
-    } else {
-        sign = (long long)det.w[4] < 0 ? -1 : 1;
+    if ((det.w[0] | det.w[1] | det.w[2] | det.w[3] | det.w[4]) == 0) return 0;
+    return (long long)det.w[4] < 0 ? -1 : 1;
+}
+
+// |a| * b for |a| < 2^31 and b < 2^64 as an unsigned 128-bit integer, from two 32 x 32-bit
+// products
+__host__ __device__ __forceinline__ u128 mul_u32_u64(unsigned a, unsigned long long b) {
+    const unsigned long long p0 = (unsigned long long)a * (unsigned)b;
+    const unsigned long long p1 = (unsigned long long)a * (unsigned)(b >> 32);
+    return ((u128)p1 << 32) + p0;
+}
+
+// Exact insphere sign from differences of at most 31 bits (D = (x + y) / 2^lo, |D| < 2^31):
+// lifts fit 64 bits unsigned (3 squares below 2^62), xy minors 64 bits signed (two
+// products below 2^62), cofactors 128 bits signed (below 2^96), and the lifted products
+// and their sum 192 bits; every product is built from 32 x 32-bit ones (84 in all, against
+// about 290 for the 62-bit evaluation).
+__host__ __device__ inline int insphere_fixed31_sign(const InsDiffs& d) {
+    int D[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        D[i] = (int)(scaled_int(d.x[i], d.lo) + scaled_int(d.y[i], d.lo));
+    unsigned long long lift[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        unsigned long long l = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const unsigned m = (unsigned)(D[3 * v + a] < 0 ? -D[3 * v + a] : D[3 * v + a]);
+            l += (unsigned long long)m * m;
+        }
+        lift[v] = l;
     }
+    auto minor = [&](int u, int v) -> long long {  // uex * vey - vex * uey
+        return (long long)D[3 * u] * D[3 * v + 1] - (long long)D[3 * v] * D[3 * u + 1];
+    };
+    constexpr int A = 0, B = 1, C = 2, Dv = 3;
+    const long long ab = minor(A, B), bc = minor(B, C), cd = minor(C, Dv), da = minor(Dv, A),
+                    ac = minor(A, C), bd = minor(B, Dv);
+    auto zm = [&](int v, long long m) -> __int128 {  // z_v * m, |z| < 2^31, |m| < 2^63
+        const int zv = D[3 * v + 2];
+        const unsigned az = (unsigned)(zv < 0 ? -zv : zv);
+        const unsigned long long am = m < 0 ? 0ull - (unsigned long long)m : (unsigned long long)m;
+        const __int128 p = (__int128)mul_u32_u64(az, am);
+        return ((zv < 0) != (m < 0)) ? -p : p;
+    };
+    const __int128 abc = zm(A, bc) - zm(B, ac) + zm(C, ab);
+    const __int128 bcd = zm(B, cd) - zm(C, bd) + zm(Dv, bc);
+    const __int128 cda = zm(C, da) + zm(Dv, ac) + zm(A, cd);
+    const __int128 dab = zm(Dv, ab) + zm(A, bd) + zm(B, da);
+    // det = (dlift*abc - clift*dab) + (blift*cda - alift*bcd), each lifted product below
+    // 2^160 in magnitude: |cof| (< 2^96) times the 64-bit lift in 192-bit limbs
+    unsigned long long det[3] = {0, 0, 0};
+    auto add_lifted = [&](unsigned long long lift_v, __int128 c, bool negate) {
+        const bool neg = (c < 0) != negate;
+        const u128 ac = c < 0 ? (u128)0 - (u128)c : (u128)c;
+        const unsigned long long c0 = (unsigned long long)ac, c1 = (unsigned long long)(ac >> 64);
+        const u128 p0 = (u128)lift_v * c0;                // < 2^128
+        const u128 p1 = mul_u32_u64((unsigned)c1, lift_v);  // c1 < 2^32
+        unsigned long long w[3];
+        w[0] = (unsigned long long)p0;
+        const u128 mid = (p0 >> 64) + (unsigned long long)p1;
+        w[1] = (unsigned long long)mid;
+        w[2] = (unsigned long long)(p1 >> 64) + (unsigned long long)(mid >> 64);
+        limbs_neg(w, neg);
+        limbs_add(det, w);
+    };
+    add_lifted(lift[Dv], abc, false);
+    add_lifted(lift[C], dab, true);
+    add_lifted(lift[B], cda, false);
+    add_lifted(lift[A], bcd, true);
+    if ((det[0] | det[1] | det[2]) == 0) return 0;
+    return (long long)det[2] < 0 ? -1 : 1;
+}
+
+// Exact insphere sign in fixed point with differences of at most 62 bits; false when a
+// difference is subnormal or they span more (the caller then uses expansion arithmetic).
+__host__ __device__ inline bool insphere_sign_fixed(const double* pa, const double* pb, const double* pc,
+                                           const double* pd, const double* pe, int& sign) {
+    InsDiffs d;
+    if (!insphere_diffs(pa, pb, pc, pd, pe, d)) return false;
+    if (d.hi < d.lo) {  // every difference is zero
+        sign = 0;
+        return true;
+    }
+    if (!insphere_fixed_fits(d, 62)) return false;
+    sign = insphere_fixed62_sign(d);
+    return true;
+}
+
+// The same with differences of at most 31 bits (the cheaper evaluation).
+__host__ __device__ inline bool insphere_sign_fixed31(const double* pa, const double* pb,
+                                                      const double* pc, const double* pd,
+                                                      const double* pe, int& sign) {
+    InsDiffs d;
+    if (!insphere_diffs(pa, pb, pc, pd, pe, d)) return false;
+    if (d.hi < d.lo) {
+        sign = 0;
+        return true;
+    }
+    if (!insphere_fixed_fits(d, 31)) return false;
+    sign = insphere_fixed31_sign(d);
     return true;
 }
 
@@ -888,6 +1000,29 @@ __host__ __device__ inline bool insphere_sign_dw(const double* pa, const double*
         return true;
     }
     return false;
+}
+
+// The two cheapest register-only exact insphere stages for one lane: fixed point with
+// 31-bit differences (exact zeros and small determinants of moderately spread tuples),
+// then the double-word bound (nonzero determinants of any spread within range). False
+// when both decline; the caller then tries fixed point with 62- and 126-bit differences.
+__host__ __device__ inline bool insphere_sign_inline(const double* pa, const double* pb,
+                                                     const double* pc, const double* pd,
+                                                     const double* pe, int& sign) {
+    {
+        InsDiffs d;
+        if (insphere_diffs(pa, pb, pc, pd, pe, d)) {
+            if (d.hi < d.lo) {
+                sign = 0;
+                return true;
+            }
+            if (insphere_fixed_fits(d, 31)) {
+                sign = insphere_fixed31_sign(d);
+                return true;
+            }
+        }
+    }
+    return insphere_sign_dw(pa, pb, pc, pd, pe, sign);
 }
 
 // Exact sign of orient3d(pa, pb, pc, pd); kOverflow if CAP is too small. FAST = false

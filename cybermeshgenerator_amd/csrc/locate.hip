@@ -179,16 +179,16 @@ __global__ __launch_bounds__(256) void locate_kernel(const double* xyz, long lon
 // ---- Delaunay cavities ----------------------------------------------------------------
 
 // Exact insphere sign for the rare tuples whose naive determinant is exactly 0. FULL =
-// false tries the register-only stages (fixed point with 62- and 126-bit differences, the
-// double-word bound) and returns kUndecided when they decline (differences spread over
-// more than 126 bits); FULL = true then uses expansions. Out of line, so that neither
-// weighs on the common path.
+// false tries the register-only stages (fixed point with 31-bit differences, the
+// double-word bound, fixed point with 62- and 126-bit differences) and returns kUndecided
+// when they decline (differences spread over more than 126 bits); FULL = true then uses
+// expansions. Out of line, so that neither weighs on the common path.
 constexpr int kUndecided = 3;
 template <bool FULL>
 __device__ __noinline__ int is_exact_slow(const double* v) {
     int s;
-    if (insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
-        insphere_sign_dw(v, v + 3, v + 6, v + 9, v + 12, s) ||
+    if (insphere_sign_inline(v, v + 3, v + 6, v + 9, v + 12, s) ||
+        insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
         insphere_sign_fixed_wide(v, v + 3, v + 6, v + 9, v + 12, s))
         return s;
     if (!FULL) return kUndecided;

@@ -132,8 +132,20 @@ constexpr double kEps = 0x1p-53;
 constexpr double kO3ErrA = (7.0 + 56.0 * kEps) * kEps;     // predicates.cpp:619
 constexpr double kIspErrA = (16.0 + 224.0 * kEps) * kEps;  // predicates.cpp:625
 
+// The FP64 semi-static stage of exact mode: the naive determinant and its error bound.
+template <int NPTS>
+__device__ __forceinline__ int decide_fp64(const double* v) {
+    double perm;
+    const double det = NPTS == 4 ? o3_naive(v, &perm) : is_naive(v, &perm);
+    const double eb = (NPTS == 4 ? kO3ErrA : kIspErrA) * perm;
+    if (det > eb || -det > eb) return sgn(det);
+    return kUndecided;
+}
+
 // Returns the sign, or kUndecided; `stage` records which stage decided (1 fp32, 2 fp64).
-template <int NPTS, int MODE>
+// FP32_ONLY (exact mode) stops after the FP32 stage: the caller runs the FP64 stage later,
+// on full wavefronts of the lanes the FP32 stage left.
+template <int NPTS, int MODE, bool FP32_ONLY = false>
 __device__ __forceinline__ int decide(const double* v, int& stage) {
     if (MODE == kModeExact) {
         int s = NPTS == 4 ? o3_fp32(v) : is_fp32(v);
@@ -141,6 +153,7 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
             stage = 1;
             return s;
         }
+        if (FP32_ONLY) return kUndecided;
         double perm;
         double det = NPTS == 4 ? o3_naive(v, &perm) : is_naive(v, &perm);
         double eb = (NPTS == 4 ? kO3ErrA : kIspErrA) * perm;
@@ -160,7 +173,9 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
 // differences are exact, else double-word arithmetic with a rigorous bound; the lanes it
 // cannot decide (in practice exact zeros with inexact differences) are queued for the
 // exact pass, whose fixed-point stages resolve them. insphere: fixed point with 62-bit
-// differences, else the double-word bound; the rest is queued for the 126-bit stage.
+// differences, else the double-word bound; the rest is queued for the 126-bit stage. (A
+// cheaper first stage for narrow tuples does not pay here: a wavefront runs every stage
+// any of its lanes needs, and the drained lanes mix narrow and wide tuples.)
 template <int NPTS>
 __device__ __forceinline__ bool decide_inline_exact(const double* v, int& s) {
     if constexpr (NPTS == 4) {
@@ -292,6 +307,12 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                                                          PredWorkspace ws, unsigned iters) {
     constexpr int NPL = NPTS * 3;
     constexpr bool STAGE = true;
+    // insphere, exact mode: about 10 % of the lanes leave the FP32 stage undecided, so
+    // nearly every wavefront iteration would run the FP64 stage (a few hundred FP64
+    // operations) for a handful of lanes, and on the workloads it decides almost none of
+    // them. The streaming loop stops after the FP32 stage instead, and the ring drain runs
+    // the FP64 stage on full wavefronts of undecided tuples before the exact stages.
+    constexpr bool FP64_IN_DRAIN = MODE == kModeExact && NPTS == 5;
     // the global counters are only touched by the passes after this one
     if (blockIdx.x == 0 && threadIdx.x == 0) *ws.ctr = PredCounters{};
     constexpr unsigned kRing = ring_size<NPL>();
@@ -313,10 +334,16 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
         uint32_t i = 0;
         const bool has = ring.pop(t, i);
         int s = kUndecided;
-        const bool dec = has && decide_inline_exact<NPTS>(t, s);
-        if (dec) out[i] = (int8_t)s;
+        bool dec64 = false;
+        if (FP64_IN_DRAIN && has) {
+            s = decide_fp64<NPTS>(t);
+            dec64 = s != kUndecided;
+        }
+        const bool dec = has && !dec64 && decide_inline_exact<NPTS>(t, s);
+        if (dec || dec64) out[i] = (int8_t)s;
         nx += dec;
-        wave_push(has && !dec, i, seg, off);
+        n64 += dec64;
+        wave_push(has && !dec && !dec64, i, seg, off);
     };
     for (unsigned it = 0; it < iters; ++it) {
         const size_t p = tid + (size_t)it * stride;
@@ -342,8 +369,8 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                 }
             }
             int st0 = 0, st1 = 0;
-            const int s0 = decide<NPTS, MODE>(v0, st0);
-            const int s1 = decide<NPTS, MODE>(v1, st1);
+            const int s0 = decide<NPTS, MODE, FP64_IN_DRAIN>(v0, st0);
+            const int s1 = decide<NPTS, MODE, FP64_IN_DRAIN>(v1, st1);
             u0 = s0 == kUndecided;
             u1 = two && s1 == kUndecided;
             const int8_t o0 = (int8_t)s0, o1 = (int8_t)s1;
@@ -640,7 +667,9 @@ void launch_exact(Src src, int8_t* out, const PredWorkspace& ws, hipStream_t str
     queue_scan_kernel<<<1, 1024, 0, stream>>>(ws);
     CMG_HIP_LAUNCH();
     if constexpr (NPTS == 5) {
-        fixed_kernel<NPTS, Src><<<(unsigned)cu_count() * 8, 256, 0, stream>>>(src, out, ws);
+        // grid-stride; the queue is usually short, and a large grid of these waves costs
+        // tens of microseconds even when it is empty
+        fixed_kernel<NPTS, Src><<<(unsigned)cu_count(), 256, 0, stream>>>(src, out, ws);
         CMG_HIP_LAUNCH();
     }
     // few lanes reach this pass: a small grid keeps the scratch set-up of its waves cheap

@@ -1,8 +1,8 @@
 """The register-only exact stages of the GPU predicate passes. insphere (expansion.hpp: fixed point with
-62- and 126-bit differences, a double-word determinant with a rigorous error bound) built
-for the host and checked against the oracle's exact insphere:
-every tuple a stage decides must get the oracle's sign, and together they must decide the
-exactly cospherical and 1-ulp workload tuples."""
+31-, 62- and 126-bit differences, a double-word determinant with a rigorous error bound)
+built for the host and checked against the oracle's exact insphere: every tuple a stage
+decides must get the oracle's sign, and together they must decide the exactly
+cospherical and 1-ulp workload tuples."""
 import os
 import subprocess
 
@@ -23,9 +23,11 @@ def _run(tuples, tmp_path):
     src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
     t.tofile(src)
     subprocess.run([BIN, "insphere", str(src), str(dst)], check=True)
-    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 4, 2)[:, :3].astype(np.int64)
+    out = np.fromfile(dst, dtype=np.int8).reshape(-1, 4, 2).astype(np.int64)
     ok, sign = out[:, :, 0] > 0, out[:, :, 1]
-    # the GPU order: narrow fixed point (1), then the double-word bound (2)
+    # slots: fixed point with 62-bit differences, the double-word bound, fixed point with
+    # 126-bit differences, fixed point with 31-bit differences; `stage` reports the first
+    # two in that order (1, 2)
     stage = np.where(ok[:, 0], 1, np.where(ok[:, 1], 2, 0))
     first = np.where(ok[:, 0], sign[:, 0], sign[:, 1])
     _run.last = (ok, sign)
@@ -41,7 +43,7 @@ def _check(tuples, tmp_path):
     stage, sign = _run(tuples, tmp_path)
     ok, each = _run.last
     want = _oracle(tuples)
-    for k in range(3):  # every stage on its own agrees wherever it decides
+    for k in range(4):  # every stage on its own agrees wherever it decides
         assert np.array_equal(each[ok[:, k], k], want[ok[:, k]]), k
     return stage, want
 
@@ -54,6 +56,56 @@ def test_workload_degenerate_tuples(tmp_path):
     assert (want[kind == 1] == 0).all()
     assert (stage[kind == 2] > 0).all()         # one coordinate 1 ulp off
     assert (stage[kind == 2] == 1).mean() > 0.9
+
+
+def test_inline_order_on_the_workload(tmp_path):
+    """The order the streaming filter drains its ring in (insphere_sign_inline): the
+    31-bit fixed point decides every exactly cospherical workload tuple (differences of
+    about 24 bits on the 2^-24 grid), the double-word bound about 95 % of the 1-ulp ones
+    (a moved coordinate adds its ulp to the spread, beyond 31 bits), and the 62-bit fixed
+    point the rest."""
+    soa, kind = W.tuples_insphere(20000, seed=78)
+    tuples = soa.T.copy()
+    _check(tuples, tmp_path)
+    ok, sign = _run.last
+    assert ok[kind == 1, 3].all() and (sign[kind == 1, 3] == 0).all()
+    ulp = kind == 2
+    assert (ok[ulp, 3] | ok[ulp, 1] | ok[ulp, 0]).all()
+    assert ok[ulp, 1].mean() > 0.9
+
+
+def test_narrow_fixed_point_limit_and_extremes(tmp_path):
+    """Differences spanning 31 bits are decided by the 31-bit stage and 32 bits are not;
+    differences at the extremes of the range (+-(2^31 - 1) times a power of two) keep
+    every intermediate in range: the sign equals the oracle's."""
+    base = np.array([1.0, 0.25, 0.5, -0.5, 0.125, 0.75, 0.375, -0.25, 0.0, 0.0, 0.0, 0.0,
+                     0.0, 0.0, 0.0])
+    t31 = base.copy()
+    t31[8] = 2.0 ** -30
+    t32 = base.copy()
+    t32[8] = 2.0 ** -31
+    _run(np.stack([t31, t32]), tmp_path)
+    ok, _ = _run.last
+    assert ok[0, 3] and not ok[1, 3]
+    rng = np.random.default_rng(31)
+    top = float(2 ** 31 - 1)
+    rows = []
+    for scale in (-100, -31, 0, 20, 90):
+        # coordinates drawn from {+-top, +-(top - small), small}; e = 0 so p - e = p
+        q = rng.choice([top, -top, top - 1, -(top - 7), 5.0, -3.0, 0.0], size=(200, 15))
+        q[:, 12:] = 0.0
+        rows.append(np.ldexp(q, scale))
+        # exactly cospherical at the extremes: e on the sphere too, four more points
+        c = np.zeros(3)
+        r = top / 2
+        pts = np.array([[r, 0, 0], [-r, 0, 0], [0, r, 0], [0, -r, 0], [0, 0, r]])
+        rows.append(np.ldexp(np.tile(pts.reshape(1, 15), (4, 1)) + np.tile(c, 5), scale))
+    tuples = np.concatenate(rows)
+    _check(tuples, tmp_path)
+    ok, sign = _run.last
+    assert ok[:, 3].mean() > 0.95
+    want = _oracle(tuples)
+    assert (want[ok[:, 3]] == 0).any() and (want[ok[:, 3]] > 0).any() and (want[ok[:, 3]] < 0).any()
 
 
 def test_integer_grids_scaled_by_powers_of_two(tmp_path):
