@@ -137,6 +137,9 @@ struct Ctx {  // kernel arguments shared by every phase
     float* sdf;
     const float* gsdf;
     float* grad;
+    float* xgrad;               // backward: per-XCD partial gradients (kXcds arrays of
+                                // 3 floats per point, xstride floats apart)
+    long long xstride;
     int* nearest;
     int total_tiles;
 };
@@ -1221,19 +1224,30 @@ __global__ __launch_bounds__(256) void sdf_kernel(Ctx C, long long total_cells) 
 // and the cell's contribution d(dist)/d(vertex) = -w_m (p - q) / d, times the upstream
 // gradient and the sign, is summed per triangle: over the wavefront (one group of lanes
 // per distinct triangle), then over the workgroup in an LDS table keyed by triangle, and
-// only then added to the vertices with global atomics. A workgroup takes a contiguous
-// range of items (neighbouring blocks share their nearest triangles: far from a coarse
-// mesh whole regions share one), so the same vertices are not hammered by every
-// wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
+// only then added to the vertices with global atomics, into the copy of the gradient of
+// the XCD the workgroup runs on (xgrad_sum_kernel adds the copies). A workgroup takes
+// chunks of neighbouring blocks (neighbouring blocks share their nearest triangles: far
+// from a coarse mesh whole regions share one), so the same vertices are not hammered by
+// every wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
+constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2 (power of two)
 constexpr int kAdjSlots = 512;    // LDS table entries per workgroup (power of two)
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
 constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
 
-// Adds the 9 partial derivatives of triangle slot `tri` (mesh M) to its vertices.
+// The XCD this wave runs on: bits 3:0 of hardware register XCC_ID (register 20;
+// s_getreg operand (size - 1) << 11 | offset << 6 | id).
+__device__ __forceinline__ int xcd_id() {
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));
+}
+
+// Adds the 9 partial derivatives v9 of triangle slot `tri` (mesh M) to its vertices in
+// gx, the copy of the gradient that belongs to the XCD the wave runs on. Each XCD adds
+// into its own copy, so atomics from different XCDs never meet on one address; the
+// atomics are device-scope, so the sum is right whichever copy a wave picks.
 __device__ __forceinline__ void add_tri_grad(const Ctx& C, const MeshDesc& M, int tri,
-                                             const float* v9) {
+                                             const float* v9, float* gx) {
     const int* tv = C.tris + 3 * (size_t)C.perm[tri];
-    float* gbase = C.grad + 3 * M.vbase;
+    float* gbase = gx + 3 * M.vbase;
 #pragma unroll
     for (int vtx = 0; vtx < 3; ++vtx) {
         float* gp = gbase + 3 * (size_t)tv[vtx];
@@ -1243,11 +1257,25 @@ __device__ __forceinline__ void add_tri_grad(const Ctx& C, const MeshDesc& M, in
     }
 }
 
+// grad = the sum of the per-XCD copies of the gradient, in XCD order (one pass after
+// the adjoint).
+__global__ __launch_bounds__(256) void xgrad_sum_kernel(const float* xg, long long stride,
+                                                       float* g) {
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < stride;
+         i += step) {
+        float s = 0.0f;
+#pragma unroll
+        for (int x = 0; x < kXcds; ++x) s += xg[x * stride + i];
+        g[i] = s;
+    }
+}
+
 // Adds v9 into the workgroup table entry of triangle slot `tri` (or, when the table is
 // crowded, to the vertices directly); a newly taken entry is listed in used[].
 __device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int tri,
                                           const float* v9, int* key, float* val, int* used,
-                                          int* nused) {
+                                          int* nused, float* gx) {
     const unsigned hsh = (unsigned)tri * 0x9E3779B1u;
     for (int pr = 0; pr < kAdjProbe; ++pr) {
         const int e = (int)((hsh >> 22) + (unsigned)pr) & (kAdjSlots - 1);
@@ -1259,7 +1287,7 @@ __device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int t
             return;
         }
     }
-    add_tri_grad(C, M, tri, v9);
+    add_tri_grad(C, M, tri, v9, gx);
 }
 
 // Workgroups (4 wavefronts) take chunks of kAdjChunk neighbouring blocks from a work
@@ -1277,6 +1305,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
     const int lane = threadIdx.x & 63;
     const int nwave = blockDim.x / 64;
     const bool scatter = C.grad != nullptr;
+    float* const gx = scatter ? C.xgrad + (size_t)xcd_id() * C.xstride : nullptr;
     for (int e = threadIdx.x; e < kAdjSlots; e += blockDim.x) s_key[e] = -1;
     for (int e = threadIdx.x; e < kAdjSlots * 9; e += blockDim.x) s_val[e] = 0.0f;
     if (threadIdx.x == 0) s_nused = 0;
@@ -1364,10 +1393,11 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
                 float sum[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
-                if (lane == leader) table_add(C, M, lt, sum, s_key, s_val, s_used, &s_nused);
+                if (lane == leader)
+                    table_add(C, M, lt, sum, s_key, s_val, s_used, &s_nused, gx);
                 act = act && !match;
             }
-            if (act) table_add(C, M, tri, c, s_key, s_val, s_used, &s_nused);
+            if (act) table_add(C, M, tri, c, s_key, s_val, s_used, &s_nused, gx);
         }
         if (!scatter) {
             if (done) break;
@@ -1382,7 +1412,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
         for (int u = threadIdx.x; u < nu; u += blockDim.x) {
             const int e = s_used[u];
             const int t = s_key[e];
-            add_tri_grad(C, C.mesh[mesh_of_tri(C, t)], t, &s_val[e * 9]);
+            add_tri_grad(C, C.mesh[mesh_of_tri(C, t)], t, &s_val[e * 9], gx);
             s_key[e] = -1;
 #pragma unroll
             for (int q = 0; q < 9; ++q) s_val[e * 9 + q] = 0.0f;
@@ -1532,8 +1562,16 @@ struct BatchEngine::Impl {
         }
         reset_queue_kernel<<<1, 64, 0, st>>>(ctx.ctr, 2);
         CMG_HIP_LAUNCH();
+        if (ctx.grad) {
+            zero_grad_kernel<<<grid256, 256, 0, st>>>(ctx.xgrad, kXcds * ctx.xstride);
+            CMG_HIP_LAUNCH();
+        }
         adjoint_kernel<<<grid, 256, 0, st>>>(ctx);
         CMG_HIP_LAUNCH();
+        if (ctx.grad) {
+            xgrad_sum_kernel<<<grid256, 256, 0, st>>>(ctx.xgrad, ctx.xstride, ctx.grad);
+            CMG_HIP_LAUNCH();
+        }
     }
 };
 
@@ -1588,9 +1626,6 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
 }
 
 void BatchEngine::Impl::launch_backward(hipStream_t st) {
-    Ctx& C = ctx;
-    zero_grad_kernel<<<grid256, 256, 0, st>>>(C.grad, 3 * total_pts);
-    CMG_HIP_LAUNCH();
     {
         TimedLaunch timed("batch_sdf_backward", st);
         launch_adjoint(st);
@@ -1759,6 +1794,8 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
     C.cell_tri = I.dalloc<int>(blocked);
+    C.xstride = 3 * I.total_pts;
+    C.xgrad = I.dalloc<float>((size_t)kXcds * C.xstride);
     I.h_cell_start.resize(nm + 1);
     for (int m = 0; m < nm; ++m) I.h_cell_start[m] = I.h_mesh[m].cell_base;
     I.h_cell_start[nm] = cells;

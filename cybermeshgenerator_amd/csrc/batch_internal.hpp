@@ -63,8 +63,8 @@ struct BatchCounters {
                                    // 3 tiles redone with exact predicates
     unsigned long long tile_used;  // tile-list pool cursor
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
-    // work of the voxel passes, counted per wavefront: cylinder tests of tree nodes,
-    // bounding-disk tests of triangles, FP32 point-triangle distances
+    // work of the voxel passes, counted per wavefront: box tests of tree nodes, box
+    // tests of triangles, FP32 point-triangle distances
     unsigned long long node_tests, tri_tests, tri_evals;
     unsigned overflow;             // bit 0: tile lists
     unsigned pad;
