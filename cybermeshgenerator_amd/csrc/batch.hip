@@ -115,6 +115,7 @@ struct Ctx {  // kernel arguments shared by every phase
     const double* xyz;
     const int* tris;            // local point indices of each mesh
     const int* perm;            // tree slot -> global triangle
+    int4* svtx;                 // tree slot -> global point indices of its vertices (w: 0)
     BatchCounters* ctr;
     TriRec* rec;                // per tree slot
     TriRec64* rec64;
@@ -204,6 +205,29 @@ __device__ __forceinline__ float wave_max(float x) {
     x = dpp_max<0x140, 0xF>(x);
     x = dpp_max<0x142, 0xA>(x);
     x = dpp_max<0x143, 0xC>(x);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
+}
+
+// One step of a DPP sum: x plus the x of the lane `ctrl` selects, on the rows row_mask
+// enables (the other rows add 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float x) {
+    const int y = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROW_MASK,
+                                              0xF, false);
+    return x + __builtin_bit_cast(float, y);
+}
+
+// Sum of x over the wavefront (every lane active), as a wave-uniform value, in VALU
+// cross-lane steps (no LDS): wave_max's steps with additions, each adding disjoint partial
+// sums — pairs, quads, halves of rows, rows; then row 0 into row 1 and row 2 into row 3,
+// then rows 0-1 into row 3 — so lane 63 ends with the total.
+__device__ __forceinline__ float wave_sum_dpp(float x) {
+    x = dpp_add<0xB1, 0xF>(x);
+    x = dpp_add<0x4E, 0xF>(x);
+    x = dpp_add<0x141, 0xF>(x);
+    x = dpp_add<0x140, 0xF>(x);
+    x = dpp_add<0x142, 0xA>(x);
+    x = dpp_add<0x143, 0xC>(x);
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), 63));
 }
 
@@ -329,6 +353,20 @@ __global__ __launch_bounds__(256) void prep_kernel(Ctx C) {
             const int gt = has ? M.tile_base + (ty0 + r / nx) * M.ntx + tx0 + r % nx : 0;
             wave_increment(C.tile_count, gt, has);
         }
+    }
+}
+
+// Per tree slot: the global point indices of its triangle's vertices (once at creation:
+// the adjoint's scatter then takes one load instead of the slot's triangle and its
+// indices).
+__global__ __launch_bounds__(256) void slot_vtx_kernel(Ctx C) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < C.total_tris;
+         s += stride) {
+        const MeshDesc& M = C.mesh[mesh_of_tri(C, s)];
+        const int* tv = C.tris + 3 * (size_t)C.perm[s];
+        C.svtx[s] = make_int4((int)(M.vbase + tv[0]), (int)(M.vbase + tv[1]),
+                              (int)(M.vbase + tv[2]), 0);
     }
 }
 
@@ -1232,7 +1270,7 @@ __global__ __launch_bounds__(256) void sdf_kernel(Ctx C, long long total_cells) 
 constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2 (power of two)
 constexpr int kAdjSlots = 512;    // LDS table entries per workgroup (power of two)
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
-constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
+constexpr int kAdjGroup = 4;      // lanes sharing a triangle worth a wavefront sum
 
 // The XCD this wave runs on: bits 3:0 of hardware register XCC_ID (register 20;
 // s_getreg operand (size - 1) << 11 | offset << 6 | id).
@@ -1240,17 +1278,19 @@ __device__ __forceinline__ int xcd_id() {
     return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));
 }
 
-// Adds the 9 partial derivatives v9 of triangle slot `tri` (mesh M) to its vertices in
-// gx, the copy of the gradient that belongs to the XCD the wave runs on. Each XCD adds
-// into its own copy, so atomics from different XCDs never meet on one address; the
-// atomics are device-scope, so the sum is right whichever copy a wave picks.
-__device__ __forceinline__ void add_tri_grad(const Ctx& C, const MeshDesc& M, int tri,
-                                             const float* v9, float* gx) {
-    const int* tv = C.tris + 3 * (size_t)C.perm[tri];
-    float* gbase = gx + 3 * M.vbase;
+// Adds the 9 partial derivatives v9 of triangle slot `tri` to its vertices in gx, the
+// copy of the gradient that belongs to the XCD the wave runs on. Each XCD adds into its
+// own copy, so atomics from different XCDs never meet on one address; the atomics are
+// device-scope, so the sum is right whichever copy a wave picks. The three vertex indices
+// come in one load before any atomic is issued (a load issued after atomics waits for
+// them: one memory counter covers both).
+__device__ __forceinline__ void add_tri_grad(const Ctx& C, int tri, const float* v9,
+                                             float* gx) {
+    const int4 tv = C.svtx[tri];
+    const int vs[3] = {tv.x, tv.y, tv.z};
 #pragma unroll
     for (int vtx = 0; vtx < 3; ++vtx) {
-        float* gp = gbase + 3 * (size_t)tv[vtx];
+        float* gp = gx + 3 * (size_t)vs[vtx];
         atomicAdd(gp + 0, v9[3 * vtx + 0]);
         atomicAdd(gp + 1, v9[3 * vtx + 1]);
         atomicAdd(gp + 2, v9[3 * vtx + 2]);
@@ -1273,9 +1313,8 @@ __global__ __launch_bounds__(256) void xgrad_sum_kernel(const float* xg, long lo
 
 // Adds v9 into the workgroup table entry of triangle slot `tri` (or, when the table is
 // crowded, to the vertices directly); a newly taken entry is listed in used[].
-__device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int tri,
-                                          const float* v9, int* key, float* val, int* used,
-                                          int* nused, float* gx) {
+__device__ __forceinline__ void table_add(const Ctx& C, int tri, const float* v9, int* key,
+                                          float* val, int* used, int* nused, float* gx) {
     const unsigned hsh = (unsigned)tri * 0x9E3779B1u;
     for (int pr = 0; pr < kAdjProbe; ++pr) {
         const int e = (int)((hsh >> 22) + (unsigned)pr) & (kAdjSlots - 1);
@@ -1287,15 +1326,17 @@ __device__ __forceinline__ void table_add(const Ctx& C, const MeshDesc& M, int t
             return;
         }
     }
-    add_tri_grad(C, M, tri, v9, gx);
+    add_tri_grad(C, tri, v9, gx);
 }
 
-// Workgroups (4 wavefronts) take chunks of kAdjChunk neighbouring blocks from a work
-// queue and walk each chunk four blocks at a time (one per wavefront); the table is added
-// to the vertices and cleared whenever it is a quarter full (an iteration takes at most
-// 256 entries, so the table never passes three quarters) and at the end.
-constexpr int kAdjChunk = 16;
-__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
+// Workgroups (4 wavefronts) take chunks of `chunk` neighbouring blocks from a work queue
+// and walk each chunk four blocks at a time (one per wavefront); the table is added to
+// the vertices and cleared whenever it is a quarter full (an iteration takes at most 256
+// entries, so the table never passes three quarters) and at the end. Longer chunks keep
+// more of a workgroup's cells in one region (fewer flushes of the same triangles); the
+// host sizes them so that every workgroup still gets several.
+constexpr int kAdjChunkMin = 16, kAdjChunkMax = 64;
+__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, int chunk) {
     __shared__ int s_key[kAdjSlots];
     __shared__ float s_val[kAdjSlots * 9];
     __shared__ int s_used[kAdjSlots];  // entries taken since the last flush
@@ -1311,19 +1352,27 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
     if (threadIdx.x == 0) s_nused = 0;
     __syncthreads();
     long long base = 0, last = 0;
+    // this wavefront's current mesh: position pos in processing order, items [p_lo, p_hi)
+    // (consecutive items mostly stay in one mesh: the search runs when they leave it)
+    int pos = 0;
+    long long p_lo = 0, p_hi = 0;
     while (true) {
         if (base >= last) {  // next chunk (workgroup-uniform)
-            base = next_item(&C.ctr->queue[2]) * kAdjChunk;
-            last = base + kAdjChunk < total ? base + kAdjChunk : total;
+            base = next_item(&C.ctr->queue[2]) * chunk;
+            last = base + chunk < total ? base + chunk : total;
         }
         const bool done = base >= total;
         const long long item = base + (threadIdx.x >> 6);
         base += nwave;
         if (!done && item < last) {
-            const int pos = find_pos(off, C.nm, item);
+            if (item < p_lo || item >= p_hi) {
+                pos = find_pos(off, C.nm, item);
+                p_lo = off[pos];
+                p_hi = off[pos + 1];
+            }
             const MeshDesc& M = C.mesh[C.order[pos]];
             const GridSpec& g = M.g;
-            const int b = (int)(item - off[pos]);
+            const int b = (int)(item - p_lo);
             const int ix = b % M.L0.bx, iy = (b / M.L0.bx) % M.L0.by,
                       iz = b / (M.L0.bx * M.L0.by);
             const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3),
@@ -1392,12 +1441,12 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
                 if (__popcll(__ballot(match)) < kAdjGroup) break;
                 float sum[9];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) sum[e] = wave_sum(match ? c[e] : 0.0f);
+                for (int e = 0; e < 9; ++e) sum[e] = wave_sum_dpp(match ? c[e] : 0.0f);
                 if (lane == leader)
-                    table_add(C, M, lt, sum, s_key, s_val, s_used, &s_nused, gx);
+                    table_add(C, lt, sum, s_key, s_val, s_used, &s_nused, gx);
                 act = act && !match;
             }
-            if (act) table_add(C, M, tri, c, s_key, s_val, s_used, &s_nused, gx);
+            if (act) table_add(C, tri, c, s_key, s_val, s_used, &s_nused, gx);
         }
         if (!scatter) {
             if (done) break;
@@ -1412,7 +1461,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C) {
         for (int u = threadIdx.x; u < nu; u += blockDim.x) {
             const int e = s_used[u];
             const int t = s_key[e];
-            add_tri_grad(C, C.mesh[mesh_of_tri(C, t)], t, &s_val[e * 9], gx);
+            add_tri_grad(C, t, &s_val[e * 9], gx);
             s_key[e] = -1;
 #pragma unroll
             for (int q = 0; q < 9; ++q) s_val[e * 9 + q] = 0.0f;
@@ -1566,7 +1615,11 @@ struct BatchEngine::Impl {
             zero_grad_kernel<<<grid256, 256, 0, st>>>(ctx.xgrad, kXcds * ctx.xstride);
             CMG_HIP_LAUNCH();
         }
-        adjoint_kernel<<<grid, 256, 0, st>>>(ctx);
+        // about four chunks per workgroup, within [kAdjChunkMin, kAdjChunkMax] blocks
+        const long long items = h_item_off[(size_t)kPhL0 * (nm + 1) + nm];
+        const int chunk = (int)std::min<long long>(
+            kAdjChunkMax, std::max<long long>(kAdjChunkMin, items / (4LL * grid)));
+        adjoint_kernel<<<grid, 256, 0, st>>>(ctx, chunk);
         CMG_HIP_LAUNCH();
         if (ctx.grad) {
             xgrad_sum_kernel<<<grid256, 256, 0, st>>>(ctx.xgrad, ctx.xstride, ctx.grad);
@@ -1647,6 +1700,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     I.total_pts = pt_start[nm];
     I.total_tris = tri_start[nm];
     if (I.total_tris >= (1LL << 31)) throw InvalidInput("batch: more than 2^31 triangles");
+    if (I.total_pts >= (1LL << 31)) throw InvalidInput("batch: more than 2^31 points");
     for (int m = 0; m < nm; ++m) {
         if (pt_start[m + 1] <= pt_start[m]) throw InvalidInput("batch: a mesh has no points");
         if (tri_start[m + 1] <= tri_start[m])
@@ -1830,6 +1884,9 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
         CMG_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * I.total_tris,
                                hipMemcpyHostToDevice, st));
         C.perm = d_perm;
+        C.svtx = I.dalloc<int4>(I.total_tris);
+        slot_vtx_kernel<<<I.grid256, 256, 0, st>>>(C);
+        CMG_HIP_LAUNCH();
         CMG_HIP(hipStreamSynchronize(st));
     }
     // planning pass: the tile-list sizes

@@ -238,20 +238,6 @@ __device__ double point_tri_dist2_ref(const double* p, const double* a, const do
     return dist2_to(p, q);
 }
 
-// ---- wave helpers ------------------------------------------------------------------
-
-__device__ __forceinline__ float wave_min(float v) {
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
-__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
 // Loads of records every lane needs (uniform index t) through the constant address
 // space, so that they become scalar loads into SGPRs (broadcast operands of the VALU)
 // instead of 64 identical vector loads. The records are read-only during every kernel.
