@@ -134,6 +134,7 @@ struct Ctx {  // kernel arguments shared by every phase
                                 // the tree slot of the triangle the cell's distance comes
                                 // from: s >= 0 (FP32 nearest), -2 - s (FP64 near the
                                 // surface), -1 (none)
+    float* blk_dist;            // per cell, blocked like cell_tri: its unsigned distance
     const long long* cell_start;  // [nm + 1] first cell of each mesh id (concatenated)
     float* sdf;
     const float* gsdf;
@@ -1188,11 +1189,18 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
                 }
             });
         }
-        // the cell's source triangle; sdf_kernel turns it into the signed distance
+        // the cell's source triangle and its unsigned distance: the FP32 minimum back in
+        // world units (exact scaling), or the reference's FP64 distance rounded once
+        // (voxelize.cpp:354-355); sdf_kernel signs them in flat order
         int* const tri_out = C.cell_tri + M.blk_base;
+        float* const dist_out = C.blk_dist + M.blk_base;
 #pragma unroll
         for (int v = 0; v < VPL; ++v)
-            if (valid[v]) tri_out[blocked_cell(M, i, j, k[v])] = near[v] ? -2 - arg64[v] : arg[v];
+            if (valid[v]) {
+                const size_t bc = blocked_cell(M, i, j, k[v]);
+                tri_out[bc] = near[v] ? -2 - arg64[v] : arg[v];
+                dist_out[bc] = near[v] ? (float)sqrt(best64[v]) : sqrtf(best[v]) * M.iscf;
+            }
     }
     if (lane == 0) {
         if (nvis) atomicAdd(&C.ctr->node_tests, nvis);
@@ -1201,56 +1209,51 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
     }
 }
 
-// ---- signed distance from the source triangles -------------------------------------------
+// ---- signed distance in flat order ---------------------------------------------------------
 //
-// One cell per thread in flat order (idx = (k ny + j) nx + i per mesh, meshes back to back),
-// so every store instruction writes consecutive floats and the distances leave as whole
-// lines. The distance to the recorded triangle is recomputed with exactly the forward's
-// arithmetic: tri_d2 at the same FP32 point (the FP32 minimum), or the reference's FP64
-// routine near the surface, rounded once (voxelize.cpp:354-355); the sign is the
-// occupancy's.
-__device__ __forceinline__ float cell_sdf(const Ctx& C, const MeshDesc& M, int i, int j,
-                                          int k, uint8_t inside) {
-    const GridSpec& g = M.g;
-    const int code = C.cell_tri[M.blk_base + blocked_cell(M, i, j, k)];
-    if (code <= -2) {
-        const double p64[3] = {g.corner[0] + (i + 0.5) * g.spacing,
-                               g.corner[1] + (j + 0.5) * g.spacing,
-                               g.corner[2] + (k + 0.5) * g.spacing};
-        const TriRec64& T64 = C.rec64[-2 - code];
-        const double d = sqrt(point_tri_dist2_ref(p64, T64.a, T64.b, T64.c));
-        return (float)(inside ? -d : d);
-    }
-    float d2 = FLT_MAX;
-    if (code >= 0) {
-        const float h = (float)g.spacing * M.scf;  // the cell size in search units (exact)
-        d2 = tri_d2(C.rec[code], (i + 0.5f) * h, (j + 0.5f) * h, (k + 0.5f) * h);
-    }
-    const float d32 = sqrtf(d2) * M.iscf;  // back to world units (exact)
-    return inside ? -d32 : d32;
-}
-
-// A workgroup takes chunks of 256 consecutive cells. Chunks go to the XCDs in runs of
-// kSdfRun (workgroup w runs on XCD w % 8: dispatch is round robin), so the neighbouring
-// rows and slices of a run, which share their triangle records, are read through one L2.
-// (gridDim.x is a multiple of 8; any other dispatch only loses that locality.)
-constexpr long long kSdfRun = 64;
-__global__ __launch_bounds__(256) void sdf_kernel(Ctx C, long long total_cells) {
-    const long long xcd = blockIdx.x & 7u, per_xcd = gridDim.x >> 3;
-    const long long nchunk = (total_cells + 255) / 256;
-    int m = 0;  // the mesh of the thread's current cell (cells only increase)
-    for (long long s = blockIdx.x >> 3;; s += per_xcd) {
-        const long long chunk = ((s / kSdfRun) * 8 + xcd) * kSdfRun + s % kSdfRun;
-        if (chunk >= nchunk) break;  // chunks only grow with s
-        const long long c = chunk * 256 + threadIdx.x;
-        if (c >= total_cells) break;
-        while (c >= C.cell_start[m + 1]) ++m;
-        const MeshDesc& M = C.mesh[m];
-        const long long local = c - M.cell_base;
-        const int i = (int)(local % M.g.nx);
-        const int j = (int)((local / M.g.nx) % M.g.ny);
-        const int k = (int)(local / ((long long)M.g.nx * M.g.ny));
-        C.sdf[c] = cell_sdf(C, M, i, j, k, C.occ[c]);
+// The voxel pass leaves each cell's unsigned distance in 4x4x4 blocks (blocked like
+// cell_tri); the SDF is the flat array idx = (k ny + j) nx + i per mesh, meshes back to
+// back, signed by the occupancy (negation is exact, so this equals signing before the
+// rounding). One item is a run of up to kRowRun blocks along x at one (block row, block
+// slice): kRowRun x 64 consecutive floats of the blocked array, staged in LDS (row
+// (j & 3) + 4 (k & 3), column x; one float of padding per row against bank conflicts)
+// and written as 16 grid rows of up to 4 kRowRun consecutive floats. Reads and writes
+// are whole lines.
+__global__ __launch_bounds__(256) void sdf_kernel(Ctx C) {
+    __shared__ float tile[16][4 * kRowRun + 1];
+    const long long* off = phase_off(C, kPhRow);
+    const long long total = off[C.nm];
+    for (long long item = blockIdx.x; item < total; item += gridDim.x) {
+        const int pos = find_pos(off, C.nm, item);
+        const MeshDesc& M = C.mesh[C.order[pos]];
+        const GridSpec& g = M.g;
+        const Level& L = M.L0;
+        const int nrun = (L.bx + kRowRun - 1) / kRowRun;
+        long long r = item - off[pos];
+        const int xr = (int)(r % nrun);
+        r /= nrun;
+        const int iy = (int)(r % L.by), iz = (int)(r / L.by);
+        const int ix0 = xr * kRowRun;
+        const int nb = min(kRowRun, L.bx - ix0);  // blocks in this run
+        const float* src = C.blk_dist + M.blk_base +
+                           64 * ((size_t)ix0 + (size_t)L.bx * ((size_t)iy + (size_t)L.by * iz));
+        __syncthreads();  // the previous item's reads of the tile are done
+        for (int e = threadIdx.x; e < nb * 64; e += blockDim.x) {
+            const int w = e & 63;  // cell (w & 3) + 4 ((w >> 2) & 3) + 16 (w >> 4) of block e / 64
+            tile[w >> 2][4 * (e >> 6) + (w & 3)] = src[e];
+        }
+        __syncthreads();
+        const int i0 = 4 * ix0;
+        const int xw = min(4 * nb, g.nx - i0);  // cells of each row inside the grid
+        for (int e = threadIdx.x; e < 16 * xw; e += blockDim.x) {
+            const int row = e / xw, x = e - row * xw;
+            const int j = 4 * iy + (row & 3), k = 4 * iz + (row >> 2);
+            if (j < g.ny && k < g.nz) {
+                const size_t idx = (size_t)M.cell_base + ((size_t)k * g.ny + j) * g.nx + i0 + x;
+                const float u = tile[row][x];
+                C.sdf[idx] = C.occ[idx] ? -u : u;
+            }
+        }
     }
 }
 
@@ -1673,7 +1676,7 @@ void BatchEngine::Impl::launch_forward(hipStream_t st) {
     {
         TimedLaunch timed("batch_sdf", st);
         launch_vox(st);
-        sdf_kernel<<<grid256 * 4, 256, 0, st>>>(ctx, total_cells);
+        sdf_kernel<<<grid256 * 4, 256, 0, st>>>(ctx);
         CMG_HIP_LAUNCH();
     }
 }
@@ -1809,6 +1812,8 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             if (ph == kPhL0) n = (long long)M.L0.bx * M.L0.by * M.L0.bz;
             if (ph == kPhL0b) n = (long long)M.L0b.bx * M.L0b.by * M.L0b.bz;
             if (ph == kPhL0c) n = (long long)M.L0c.bx * M.L0c.by * M.L0c.bz;
+            if (ph == kPhRow)
+                n = (long long)((M.L0.bx + kRowRun - 1) / kRowRun) * M.L0.by * M.L0.bz;
             o[p + 1] = o[p] + n;
         }
     }
@@ -1848,6 +1853,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
     C.cell_tri = I.dalloc<int>(blocked);
+    C.blk_dist = I.dalloc<float>(blocked);
     C.xstride = 3 * I.total_pts;
     C.xgrad = I.dalloc<float>((size_t)kXcds * C.xstride);
     I.h_cell_start.resize(nm + 1);

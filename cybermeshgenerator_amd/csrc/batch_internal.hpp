@@ -56,7 +56,10 @@ struct MeshDesc {
     float scf, iscf;      // sc and 1 / sc as floats (exact)
 };
 
-enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kNumPhase = 4 };
+// Work items per mesh: column tiles, voxel leaves (4x4x4, 4x4x8, 4x4x16), and runs of up
+// to 64 4x4x4 leaves along x (the SDF pass's rows).
+enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kPhRow = 4, kNumPhase = 5 };
+constexpr int kRowRun = 64;  // 4x4x4 leaves per SDF-pass item
 
 struct BatchCounters {
     unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 adjoint,
