@@ -421,14 +421,18 @@ __global__ __launch_bounds__(256) void tile_fill_kernel(Ctx C) {
 // in ascending index order; pass 6 records the smallest grazing index and a final pass
 // replays that nudge counting only triangles below it.
 
-// write_column for masks updated by other lanes' atomics: the words are read with atomic
-// loads, which bypass the non-coherent L1.
-__device__ inline void write_column_coherent(const unsigned long long* mask, int words,
-                                             const GridSpec& g, int i, int j, uint8_t* occ) {
+// Turns a column's crossing bits (a crossing at height h sets bit K & 63 of word K >> 6,
+// K = cells_below(h)) into its occupancy bits in place: cell k is inside iff the number of
+// crossings above its centre is odd (voxelize.cpp:186-190), the XOR of the crossing bits
+// above k (a suffix XOR from the top word). COHERENT reads the words with atomic loads
+// (the crossings were added by other lanes' atomics), which bypass the non-coherent L1.
+template <bool COHERENT>
+__device__ inline void finish_column(unsigned long long* mask, int words) {
     unsigned long long carry = 0;
     for (int w = words - 1; w >= 0; --w) {
-        const unsigned long long x = __hip_atomic_load(mask + w, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long x =
+            COHERENT ? __hip_atomic_load(mask + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : mask[w];
         unsigned long long s = x;
         s ^= s >> 1;
         s ^= s >> 2;
@@ -436,12 +440,45 @@ __device__ inline void write_column_coherent(const unsigned long long* mask, int
         s ^= s >> 8;
         s ^= s >> 16;
         s ^= s >> 32;  // bit b: XOR of bits b..63
-        const unsigned long long above = (s >> 1) ^ (carry ? ~0ull : 0ull);
+        mask[w] = (s >> 1) ^ (carry ? ~0ull : 0ull);  // bit b: cell 64 w + b
         carry ^= s & 1ull;
-        const int k0 = w * 64;
-        const int k1 = min(g.nz, k0 + 64);
-        for (int k = k0; k < k1; ++k)
-            occ[((size_t)k * g.ny + j) * g.nx + i] = (uint8_t)((above >> (k - k0)) & 1ull);
+    }
+}
+
+// The occupancy bytes in flat order (idx = (k ny + j) nx + i per mesh) from the columns'
+// occupancy bits. One item per (mesh, grid row j, mask word w, run of 64 columns): lane l
+// loads word w of column (64 r + l, j) and writes its up to 64 cells k = 64 w + b, one
+// store instruction of 64 consecutive bytes per cell plane. (A tile pass writing its 8x8
+// columns cell by cell stored 8-byte row pieces, 5x the bytes in partial lines.)
+__global__ __launch_bounds__(256) void occ_rows_kernel(Ctx C) {
+    const long long* off = phase_off(C, kPhOccRow);
+    const long long total = off[C.nm];
+    const int lane = threadIdx.x & 63;
+    const long long nwave = (long long)gridDim.x * (blockDim.x / 64);
+    int pos = 0;
+    long long p_lo = 0, p_hi = 0;  // items of mesh position pos
+    for (long long item = (long long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+         item < total; item += nwave) {
+        if (item < p_lo || item >= p_hi) {
+            pos = find_pos(off, C.nm, item);
+            p_lo = off[pos];
+            p_hi = off[pos + 1];
+        }
+        const MeshDesc& M = C.mesh[C.order[pos]];
+        const GridSpec& g = M.g;
+        const int nrun = (g.nx + 63) / 64;
+        long long r = item - p_lo;  // (j ny-major, w, run)
+        const int run = (int)(r % nrun);
+        r /= nrun;
+        const int w = (int)(r % M.words), j = (int)(r / M.words);
+        const int i = 64 * run + lane;
+        const unsigned long long bits =
+            i < g.nx ? C.masks[M.mask_base + ((size_t)j * g.nx + i) * M.words + w] : 0ull;
+        const int k0 = 64 * w, k1 = min(g.nz, k0 + 64);
+        uint8_t* col = C.occ + M.cell_base + (size_t)j * g.nx + i;
+        const size_t plane = (size_t)g.nx * g.ny;
+        if (i < g.nx)
+            for (int k = k0; k < k1; ++k) col[(size_t)k * plane] = (uint8_t)((bits >> (k - k0)) & 1ull);
     }
 }
 
@@ -567,10 +604,7 @@ __global__ __launch_bounds__(64) void occ_pairs_kernel(Ctx C) {
             continue;
         }
         __threadfence();  // every crossing has landed
-        if (valid && ok) {
-            const unsigned long long* cm = masks + ((size_t)cj * g.nx + ci) * M.words;
-            write_column_coherent(cm, M.words, g, ci, cj, C.occ + M.cell_base);
-        }
+        if (valid && ok) finish_column<true>(masks + ((size_t)cj * g.nx + ci) * M.words, M.words);
     }
 }
 
@@ -700,7 +734,7 @@ __global__ __launch_bounds__(64) void occ_kernel(Ctx C) {
             if (lane == 0) C.redo[atomicAdd(&C.ctr->redo_n, 1ull)] = (int)item;
             continue;
         }
-        if (valid && ok) write_column(mask, M.words, g, ci, cj, C.occ + M.cell_base);
+        if (valid && ok) finish_column<false>(mask, M.words);
     }
 }
 
@@ -1647,12 +1681,12 @@ void BatchEngine::Impl::launch_occupancy(hipStream_t st) {
     static unsigned g_fast = 0, g_exact = 0;
     if (!g_fast) g_fast = resident_grid(occ_pairs_kernel);
     if (!g_exact) g_exact = resident_grid(occ_kernel<true>);
-    {
-        TimedLaunch timed("batch_occupancy", st);
-        occ_pairs_kernel<<<g_fast, 64, 0, st>>>(ctx);
-        CMG_HIP_LAUNCH();
-    }
+    TimedLaunch timed("batch_occupancy", st);
+    occ_pairs_kernel<<<g_fast, 64, 0, st>>>(ctx);
+    CMG_HIP_LAUNCH();
     occ_kernel<true><<<g_exact, 64, 0, st>>>(ctx);
+    CMG_HIP_LAUNCH();
+    occ_rows_kernel<<<grid256 * 4, 256, 0, st>>>(ctx);
     CMG_HIP_LAUNCH();
 }
 
@@ -1814,6 +1848,7 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             if (ph == kPhL0c) n = (long long)M.L0c.bx * M.L0c.by * M.L0c.bz;
             if (ph == kPhRow)
                 n = (long long)((M.L0.bx + kRowRun - 1) / kRowRun) * M.L0.by * M.L0.bz;
+            if (ph == kPhOccRow) n = (long long)M.g.ny * M.words * ((M.g.nx + 63) / 64);
             o[p + 1] = o[p] + n;
         }
     }

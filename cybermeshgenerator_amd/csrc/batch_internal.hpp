@@ -56,9 +56,11 @@ struct MeshDesc {
     float scf, iscf;      // sc and 1 / sc as floats (exact)
 };
 
-// Work items per mesh: column tiles, voxel leaves (4x4x4, 4x4x8, 4x4x16), and runs of up
-// to 64 4x4x4 leaves along x (the SDF pass's rows).
-enum BatchPhase { kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kPhRow = 4, kNumPhase = 5 };
+// Work items per mesh: column tiles, voxel leaves (4x4x4, 4x4x8, 4x4x16), runs of up to
+// 64 4x4x4 leaves along x (the SDF pass's rows), and grid rows (the occupancy bytes).
+enum BatchPhase {
+    kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kPhRow = 4, kPhOccRow = 5, kNumPhase = 6
+};
 constexpr int kRowRun = 64;  // 4x4x4 leaves per SDF-pass item
 
 struct BatchCounters {

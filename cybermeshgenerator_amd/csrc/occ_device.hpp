@@ -98,28 +98,5 @@ __device__ inline int4 xy_range(const double (&P)[3][3], const GridSpec& g) {
     return make_int4(i0, i1, j0, j1);
 }
 
-// Parity of cell k = XOR of the "flip below K" bits above k, written for every cell of
-// the column (i, j) (suffix XOR from the top word).
-__device__ inline void write_column(const unsigned long long* mask, int words,
-                                    const GridSpec& g, int i, int j, uint8_t* occ) {
-    unsigned long long carry = 0;
-    for (int w = words - 1; w >= 0; --w) {
-        unsigned long long x = mask[w];
-        unsigned long long s = x;
-        s ^= s >> 1;
-        s ^= s >> 2;
-        s ^= s >> 4;
-        s ^= s >> 8;
-        s ^= s >> 16;
-        s ^= s >> 32;  // bit b: XOR of bits b..63
-        unsigned long long above = (s >> 1) ^ (carry ? ~0ull : 0ull);
-        carry ^= s & 1ull;
-        const int k0 = w * 64;
-        const int k1 = min(g.nz, k0 + 64);
-        for (int k = k0; k < k1; ++k)
-            occ[((size_t)k * g.ny + j) * g.nx + i] = (uint8_t)((above >> (k - k0)) & 1ull);
-    }
-}
-
 }  // namespace occdev
 }  // namespace cmg::hip
