@@ -302,8 +302,9 @@ class PLC:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h:
-            self._destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy is not None:
+            destroy(h)
             self._handle = None
 
     def add_points(self, points) -> "PLC":

@@ -69,6 +69,7 @@ def read_plc(path: str) -> PLC:
     _check(lib().cmg_read_plc(os.fsencode(path), ctypes.byref(h), err, len(err)), err)
     plc = PLC.__new__(PLC)
     plc._handle = h.value
+    plc._destroy = lib().cmg_plc_destroy
     pts, tris = plc_geometry(h)
     plc._points = pts
     plc._facets = [t.copy() for t in tris]
