@@ -1254,39 +1254,64 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
 // and written as 16 grid rows of up to 4 kRowRun consecutive floats. Reads and writes
 // are whole lines.
 __global__ __launch_bounds__(256) void sdf_kernel(Ctx C) {
-    __shared__ float tile[16][4 * kRowRun + 1];
+    constexpr int kW = 4 * kRowRun;  // cells per tile row
+    __shared__ __align__(16) float tile[16][kW + 4];
     const long long* off = phase_off(C, kPhRow);
     const long long total = off[C.nm];
-    for (long long item = blockIdx.x; item < total; item += gridDim.x) {
-        const int pos = find_pos(off, C.nm, item);
+    // a contiguous range of items per workgroup: neighbouring rows of one mesh, whose
+    // position in processing order is looked up only when the range enters a new mesh
+    const long long per = (total + gridDim.x - 1) / gridDim.x;
+    const long long first = (long long)blockIdx.x * per;
+    const long long last = min(total, first + per);
+    int pos = 0;
+    long long p_lo = 0, p_hi = 0;
+    for (long long item = first; item < last; ++item) {
+        if (item >= p_hi) {
+            pos = find_pos(off, C.nm, item);
+            p_lo = off[pos];
+            p_hi = off[pos + 1];
+        }
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec& g = M.g;
         const Level& L = M.L0;
         const int nrun = (L.bx + kRowRun - 1) / kRowRun;
-        long long r = item - off[pos];
+        long long r = item - p_lo;
         const int xr = (int)(r % nrun);
         r /= nrun;
         const int iy = (int)(r % L.by), iz = (int)(r / L.by);
         const int ix0 = xr * kRowRun;
         const int nb = min(kRowRun, L.bx - ix0);  // blocks in this run
-        const float* src = C.blk_dist + M.blk_base +
-                           64 * ((size_t)ix0 + (size_t)L.bx * ((size_t)iy + (size_t)L.by * iz));
+        const float4* src = reinterpret_cast<const float4*>(
+            C.blk_dist + M.blk_base +
+            64 * ((size_t)ix0 + (size_t)L.bx * ((size_t)iy + (size_t)L.by * iz)));
         __syncthreads();  // the previous item's reads of the tile are done
-        for (int e = threadIdx.x; e < nb * 64; e += blockDim.x) {
-            const int w = e & 63;  // cell (w & 3) + 4 ((w >> 2) & 3) + 16 (w >> 4) of block e / 64
-            tile[w >> 2][4 * (e >> 6) + (w & 3)] = src[e];
+        for (int e = threadIdx.x; e < nb * 16; e += blockDim.x) {
+            // cells 4q..4q+3 of block e / 16, q = e % 16: x = 4 (e / 16) + 0..3 of tile row q
+            const int q = e & 15;
+            *reinterpret_cast<float4*>(&tile[q][4 * (e >> 4)]) = src[e];
         }
         __syncthreads();
         const int i0 = 4 * ix0;
-        const int xw = min(4 * nb, g.nx - i0);  // cells of each row inside the grid
-        for (int e = threadIdx.x; e < 16 * xw; e += blockDim.x) {
-            const int row = e / xw, x = e - row * xw;
-            const int j = 4 * iy + (row & 3), k = 4 * iz + (row >> 2);
-            if (j < g.ny && k < g.nz) {
-                const size_t idx = (size_t)M.cell_base + ((size_t)k * g.ny + j) * g.nx + i0 + x;
-                const float u = tile[row][x];
-                C.sdf[idx] = C.occ[idx] ? -u : u;
+        const int x = threadIdx.x;  // this thread's column of the tile (blockDim.x == kW)
+        if (x < min(4 * nb, g.nx - i0)) {
+            // the 16 occupancy bytes are loaded before any store: a byte load issued after
+            // a store waits for it (one memory counter), and uint8_t may alias the SDF
+            size_t idx[16];
+            bool in[16];
+            uint8_t inside[16];
+#pragma unroll
+            for (int row = 0; row < 16; ++row) {
+                const int j = 4 * iy + (row & 3), k = 4 * iz + (row >> 2);
+                in[row] = j < g.ny && k < g.nz;
+                idx[row] = (size_t)M.cell_base + ((size_t)k * g.ny + j) * g.nx + i0 + x;
+                inside[row] = in[row] ? C.occ[idx[row]] : 0;
             }
+#pragma unroll
+            for (int row = 0; row < 16; ++row)
+                if (in[row]) {
+                    const float u = tile[row][x];
+                    C.sdf[idx[row]] = inside[row] ? -u : u;
+                }
         }
     }
 }
