@@ -7,8 +7,9 @@
 // instead of each occupying the GPU alone. The sequence is captured once into a hipGraph
 // and replayed; a step needs no host round trip.
 //
-// Nearest-triangle search. At creation every mesh's triangles are put in Morton order of
-// their centroids. A level-l tree node covers 8^l consecutive triangles (an implicit
+// Nearest-triangle search. At creation every mesh's triangles are put in a search order of
+// their centroids built by median splits (every run of 8^l slots spatially compact). A
+// level-l tree node covers 8^l consecutive triangles (an implicit
 // 8-ary tree, no pointers) and carries an oriented bounding box whose first axis is the
 // patch's aligned area-weighted normal and whose other two are its principal in-plane
 // directions. For a surface patch seen from afar the box is a thin plate, and the
@@ -43,11 +44,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -275,31 +278,20 @@ __global__ void reset_queue_kernel(BatchCounters* ctr, int q) {
     if (threadIdx.x == 0) ctr->queue[q] = 0;
 }
 
-// Morton code (10 bits per axis) of each triangle's centroid within its mesh's grid.
-__device__ __forceinline__ unsigned spread10(unsigned x) {
-    x &= 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
-__global__ __launch_bounds__(256) void morton_kernel(Ctx C, unsigned* code) {
+// Centroid of each triangle relative to its mesh's grid corner (for the search order).
+__global__ __launch_bounds__(256) void centroid_kernel(Ctx C, float4* cen) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < C.total_tris;
          t += stride) {
         const MeshDesc& M = C.mesh[mesh_of_tri(C, t)];
         const double* xyz = C.xyz + 3 * M.vbase;
-        const double ext = fmax(fmax(M.g.nx, M.g.ny), M.g.nz) * M.g.spacing;
-        unsigned q[3];
+        float q[3];
         for (int a = 0; a < 3; ++a) {
             double c = 0.0;
             for (int v = 0; v < 3; ++v) c += xyz[3 * (size_t)C.tris[3 * t + v] + a];
-            const double u = (c / 3.0 - M.g.corner[a]) / ext;
-            q[a] = (unsigned)fmin(fmax(u * 1024.0, 0.0), 1023.0);
+            q[a] = (float)(c / 3.0 - M.g.corner[a]);
         }
-        code[t] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+        cen[t] = make_float4(q[0], q[1], q[2], 0.0f);
     }
 }
 
@@ -1570,23 +1562,44 @@ __global__ __launch_bounds__(256) void bbox_kernel(const double* xyz, const long
 namespace {
 
 // Stable LSD radix sort of (code, index) pairs by the 30-bit code, 3 passes of 10 bits.
-void sort_by_code(std::vector<unsigned>& code, std::vector<int>& idx) {
-    const size_t n = code.size();
-    std::vector<unsigned> c2(n);
-    std::vector<int> i2(n);
-    for (int pass = 0; pass < 3; ++pass) {
-        const int sh = 10 * pass;
-        size_t cnt[1025] = {0};
-        for (size_t k = 0; k < n; ++k) ++cnt[((code[k] >> sh) & 1023u) + 1];
-        for (int b = 0; b < 1024; ++b) cnt[b + 1] += cnt[b];
-        for (size_t k = 0; k < n; ++k) {
-            const size_t at = cnt[(code[k] >> sh) & 1023u]++;
-            c2[at] = code[k];
-            i2[at] = idx[k];
-        }
-        code.swap(c2);
-        idx.swap(i2);
+// Search order of one mesh's triangles idx[0, n) (global triangle ids, centroids cen):
+// the implicit tree makes every run of 8^l consecutive slots a node, so the runs are made
+// spatially compact: `parts` groups of `child` slots each, formed by binary splits at the
+// group boundary nearest the middle, each along the longest extent of the range's
+// centroids (a median split by count, ties by id); each group is then split the same way
+// into 8 groups of child / 8 slots, down to single triangles.
+static void split_order(int* idx, size_t n, size_t child, int parts, const float4* cen) {
+    if (n <= 1) return;
+    if (parts == 1) {
+        if (child > 1) split_order(idx, n, child / 8, 8, cen);
+        return;
     }
+    const int lp = parts / 2;
+    const size_t ln = std::min(n, (size_t)lp * child);
+    if (ln < n) {
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (size_t k = 0; k < n; ++k) {
+            const float4 c = cen[idx[k]];
+            const float v[3] = {c.x, c.y, c.z};
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], v[a]);
+                hi[a] = std::max(hi[a], v[a]);
+            }
+        }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        auto key = [&](int t) {
+            const float4 c = cen[t];
+            return ax == 0 ? c.x : (ax == 1 ? c.y : c.z);
+        };
+        std::nth_element(idx, idx + ln, idx + n, [&](int x, int y) {
+            const float kx = key(x), ky = key(y);
+            return kx != ky ? kx < ky : x < y;
+        });
+    }
+    split_order(idx, ln, child, lp, cen);
+    split_order(idx + ln, n - ln, child, parts - lp, cen);
 }
 
 }  // namespace
@@ -1621,6 +1634,14 @@ struct BatchEngine::Impl {
         CMG_HIP(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 256)));
         owned.push_back(p);
         return static_cast<T*>(p);
+    }
+
+    // Frees a dalloc buffer before the engine ends (the caller has synchronised its uses).
+    void dfree(void* p) {
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it == owned.end()) return;
+        owned.erase(it);
+        CMG_HIP(hipFree(p));
     }
 
     ~Impl() {
@@ -1928,24 +1949,34 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.total_tiles = (int)tiles;
     I.grid64 = (unsigned)cu_count() * 16;
     I.grid256 = (unsigned)cu_count() * 4;
-    // Morton order of every mesh's triangles (once, from the creation-time positions)
+    // search order of every mesh's triangles (once, from the creation-time positions),
+    // meshes on host threads
     {
-        unsigned* d_code = I.dalloc<unsigned>(I.total_tris);
-        morton_kernel<<<I.grid256, 256, 0, st>>>(C, d_code);
+        float4* d_cen = I.dalloc<float4>(I.total_tris);
+        centroid_kernel<<<I.grid256, 256, 0, st>>>(C, d_cen);
         CMG_HIP_LAUNCH();
-        std::vector<unsigned> code(I.total_tris);
-        CMG_HIP(hipMemcpyAsync(code.data(), d_code, sizeof(unsigned) * I.total_tris,
+        std::vector<float4> cen(I.total_tris);
+        CMG_HIP(hipMemcpyAsync(cen.data(), d_cen, sizeof(float4) * I.total_tris,
                                hipMemcpyDeviceToHost, st));
         CMG_HIP(hipStreamSynchronize(st));
         std::vector<int> perm(I.total_tris);
-        for (int m = 0; m < nm; ++m) {
-            const long long t0 = tri_start[m], t1 = tri_start[m + 1];
-            std::vector<unsigned> c(code.begin() + t0, code.begin() + t1);
-            std::vector<int> idx(t1 - t0);
-            std::iota(idx.begin(), idx.end(), (int)t0);
-            sort_by_code(c, idx);
-            std::copy(idx.begin(), idx.end(), perm.begin() + t0);
-        }
+        std::iota(perm.begin(), perm.end(), 0);
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int m; (m = next.fetch_add(1)) < nm;) {
+                const long long t0 = tri_start[m], t1 = tri_start[m + 1];
+                size_t cap = 1;
+                while (cap < (size_t)(t1 - t0)) cap *= 8;
+                split_order(perm.data() + t0, (size_t)(t1 - t0), cap / 8, 8, cen.data());
+            }
+        };
+        const int nthr = (int)std::min<unsigned>(
+            (unsigned)nm, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nthr; ++k) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        I.dfree(d_cen);
         int* d_perm = I.dalloc<int>(I.total_tris);
         CMG_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * I.total_tris,
                                hipMemcpyHostToDevice, st));

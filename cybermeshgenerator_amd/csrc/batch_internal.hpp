@@ -12,8 +12,8 @@
 
 namespace cmg::hip {
 
-// Levels of the per-mesh triangle tree: level 0 are the triangles (in Morton order of
-// their centroids), a level-l node covers 8^l consecutive triangles; a mesh has at most
+// Levels of the per-mesh triangle tree: level 0 are the triangles (in a median-split order
+// of their centroids), a level-l node covers 8^l consecutive triangles; a mesh has at most
 // 2^30 = 8^10 triangles, so at most 10 levels above the triangles.
 constexpr int kMaxLevels = 10;
 
@@ -89,8 +89,8 @@ class BatchEngine {
   public:
     // pt_start/tri_start: host prefix arrays (nm + 1 entries) into the concatenated
     // device arrays d_xyz (3 doubles per point) and d_tris (3 local point indices per
-    // triangle). Plans every grid, orders every mesh's triangles along a Morton curve
-    // and sizes the tile pool (synchronises `st`).
+    // triangle). Plans every grid, orders every mesh's triangles by median splits of
+    // their centroids and sizes the tile pool (synchronises `st`).
     BatchEngine(int nm, const long long* pt_start, const long long* tri_start,
                 const int* resolution, const int* pad, const double* d_xyz,
                 const int* d_tris, hipStream_t st);
