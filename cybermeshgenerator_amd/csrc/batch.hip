@@ -993,6 +993,11 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
         int ver = 0;
         const int root = M.node_base[M.nlev];
         const long long tbase = M.tbase;
+        // the stack entry below a leaf being visited (its link), and the records of that
+        // entry's leaf loaded ahead into registers (slot pf_s0, -1: none)
+        unsigned next_link = 0u;
+        long long pf_s0 = -1;
+        float4 pf0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pf1 = pf0;
         auto walk = [&](const float* b2, auto&& on_leaf) {
             float mb = 0.0f;  // largest bound of the wavefront, valid at version mb_ver
             int mb_ver = -1;
@@ -1018,6 +1023,7 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
                 }
                 const int cnt = (int)(link & 15u), first = (int)((link >> 4) & 0x7ffffffu);
                 if (link & kLinkLeaf) {
+                    next_link = sp > 0 ? (unsigned)__builtin_amdgcn_readlane(stl, sp - 1) : 0u;
                     on_leaf(first, cnt);
                     continue;
                 }
@@ -1093,15 +1099,34 @@ __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox
             }
         };
 
-        // triangles of one tree leaf, through LDS (D > 0) or scalar loads
+        // triangles of one tree leaf, through LDS (D > 0) or scalar loads. With LDS, the
+        // records of the leaf stacked below this one (usually a sibling, visited next) are
+        // loaded into registers while this leaf's triangles are tested; a leaf that is
+        // then visited takes them from there (records do not change during the pass).
         auto leaf_rec = [&](int loc, int cnt) -> long long {
             const long long s0 = tbase + 8LL * loc;
             if (D > 0) {
+                static_assert(8 * kTriRecF4 <= 128, "a leaf's records fit two loads per lane");
+                const int nq = cnt * kTriRecF4;
                 wave_sync();
-                const float4* src = reinterpret_cast<const float4*>(C.rec + s0);
                 float4* dst = reinterpret_cast<float4*>(s_rec);
-                for (int q = lane; q < cnt * kTriRecF4; q += 64) dst[q] = src[q];
+                if (s0 == pf_s0) {
+                    if (lane < nq) dst[lane] = pf0;
+                    if (lane + 64 < nq) dst[lane + 64] = pf1;
+                } else {
+                    const float4* src = reinterpret_cast<const float4*>(C.rec + s0);
+                    if (lane < nq) dst[lane] = src[lane];
+                    if (lane + 64 < nq) dst[lane + 64] = src[lane + 64];
+                }
                 wave_sync();
+                pf_s0 = -1;
+                if (next_link & kLinkLeaf) {
+                    pf_s0 = tbase + 8LL * (long long)((next_link >> 4) & 0x7ffffffu);
+                    const int nn = (int)(next_link & 15u) * kTriRecF4;
+                    const float4* src = reinterpret_cast<const float4*>(C.rec + pf_s0);
+                    if (lane < nn) pf0 = src[lane];
+                    if (lane + 64 < nn) pf1 = src[lane + 64];
+                }
             }
             return s0;
         };
