@@ -1415,7 +1415,8 @@ __device__ __forceinline__ void table_add(const Ctx& C, int tri, const float* v9
 // more of a workgroup's cells in one region (fewer flushes of the same triangles); the
 // host sizes them so that every workgroup still gets several.
 constexpr int kAdjChunkMin = 16, kAdjChunkMax = 64;
-__global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, int chunk) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+void adjoint_kernel(Ctx C, int chunk) {  // register budget: 5 waves per SIMD (96 VGPRs)
     __shared__ int s_key[kAdjSlots];
     __shared__ float s_val[kAdjSlots * 9];
     __shared__ int s_used[kAdjSlots];  // entries taken since the last flush
@@ -1435,6 +1436,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, int chunk) {
     // (consecutive items mostly stay in one mesh: the search runs when they leave it)
     int pos = 0;
     long long p_lo = 0, p_hi = 0;
+    long long pf_item = -1;  // the block whose cell triangles pf_code holds
+    int pf_code = 0;
     while (true) {
         if (base >= last) {  // next chunk (workgroup-uniform)
             base = next_item(&C.ctr->queue[2]) * chunk;
@@ -1457,7 +1460,16 @@ __global__ __launch_bounds__(256) void adjoint_kernel(Ctx C, int chunk) {
             const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3),
                       k = iz * kLeaf + (lane >> 4);
             const bool valid = i < g.nx && j < g.ny && k < g.nz;
-            const int code = valid ? C.cell_tri[M.blk_base + (size_t)b * 64 + lane] : -1;
+            const int code_b = item == pf_item ? pf_code
+                                               : C.cell_tri[M.blk_base + (size_t)b * 64 + lane];
+            const int code = valid ? code_b : -1;
+            // the cell triangles of this wavefront's next block of the chunk (same mesh)
+            // load while this block is processed
+            pf_item = -1;
+            if (item + nwave < last && item + nwave < p_hi) {
+                pf_item = item + nwave;
+                pf_code = C.cell_tri[M.blk_base + (size_t)(b + nwave) * 64 + lane];
+            }
             const bool near = code <= -2;
             const int tri = near ? -2 - code : code;
             const size_t idx = valid ? ((size_t)k * g.ny + j) * g.nx + i : 0;
