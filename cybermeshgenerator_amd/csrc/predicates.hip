@@ -307,12 +307,12 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
                                                          PredWorkspace ws, unsigned iters) {
     constexpr int NPL = NPTS * 3;
     constexpr bool STAGE = true;
-    // insphere, exact mode: about 10 % of the lanes leave the FP32 stage undecided, so
-    // nearly every wavefront iteration would run the FP64 stage (a few hundred FP64
-    // operations) for a handful of lanes, and on the workloads it decides almost none of
-    // them. The streaming loop stops after the FP32 stage instead, and the ring drain runs
-    // the FP64 stage on full wavefronts of undecided tuples before the exact stages.
-    constexpr bool FP64_IN_DRAIN = MODE == kModeExact && NPTS == 5;
+    // exact mode: about 10 % of the lanes leave the FP32 stage undecided, so nearly every
+    // wavefront iteration would run the FP64 stage for a handful of lanes, and on the
+    // workloads it decides almost none of them. The streaming loop stops after the FP32
+    // stage instead, and the ring drain runs the FP64 stage on full wavefronts of
+    // undecided tuples before the exact stages.
+    constexpr bool FP64_IN_DRAIN = MODE == kModeExact;
     // the global counters are only touched by the passes after this one
     if (blockIdx.x == 0 && threadIdx.x == 0) *ws.ctr = PredCounters{};
     constexpr unsigned kRing = ring_size<NPL>();
