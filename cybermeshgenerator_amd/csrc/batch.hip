@@ -25,14 +25,15 @@
 // error margin and take the reference's FP64 distance over what survives.
 //
 // Phases of a forward step (all meshes at once):
-//   reset        zero the queues, pool cursor and tile counters (a fill kernel: no
-//                memset nodes in the graph)
+//   reset        zero the queues, pool cursor, tile counters and occupancy masks (a fill
+//                kernel: no memset nodes in the graph)
 //   prep         per tree slot: FP32/FP64 triangle records; per triangle: xy bucket
 //                range and tile counts
-//   tile alloc   per tile: list offset from the tile pool
+//   tile alloc   per tile: list offset from the tile pool, occupancy work items
 //   tile fill    per triangle: append to the lists of the tiles its range touches
-//   occupancy    per (mesh, 8x8 column tile): exact column parity (below); tiles where
-//                a naive determinant is exactly 0 are redone with exact predicates
+//   occupancy    per segment of an 8x8 column tile's list: exact column parity (below);
+//                tiles where a naive determinant is exactly 0 are redone with exact
+//                predicates
 //   tree         level 1 from the triangles, then each level from the one below
 //   voxels       per (mesh, voxel leaf): FP32 nearest, FP64 near the surface, signed;
 //                records per cell (4 bytes) the triangle the distance comes from
@@ -67,6 +68,7 @@ using namespace occdev;
 
 namespace {
 
+constexpr int kSegTris = 8 * kChunk;  // tile-list entries per occupancy work item
 
 // ---- work queue -----------------------------------------------------------------------
 
@@ -130,7 +132,12 @@ struct Ctx {  // kernel arguments shared by every phase
     int* tile_cursor;
     int* tile_pool;
     long long tile_cap;
+    int2* seg;                  // occupancy work items: (tile item, segment of its list)
+    long long seg_cap;
+    int* tile_left;             // per tile: segments not yet finished
+    int* tile_flag;             // per tile: 1 if a segment met an exactly-0 determinant
     unsigned long long* masks;
+    long long total_words;      // mask words of all meshes
     int* redo;                  // tiles (global items) for the exact occupancy pass
     uint8_t* occ;
     int* cell_tri;              // per cell, blocked (64 cells of a 4x4x4 block in a row):
@@ -266,6 +273,8 @@ __global__ __launch_bounds__(256) void reset_kernel(Ctx C) {
         C.tile_count[i] = 0;
         C.tile_cursor[i] = 0;
     }
+    // the occupancy segments of a tile XOR their crossings into cleared columns
+    for (long long i = tid; i < C.total_words; i += stride) C.masks[i] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void zero_grad_kernel(float* g, long long n) {
@@ -363,14 +372,30 @@ __global__ __launch_bounds__(256) void slot_vtx_kernel(Ctx C) {
     }
 }
 
+// Per tile (in processing order): its list's offset in the pool and its occupancy work
+// items, one per kSegTris list entries (none for an empty tile: its columns stay clear).
 __global__ __launch_bounds__(256) void tile_alloc_kernel(Ctx C) {
+    const long long* off = phase_off(C, kPhTile);
+    const long long total = off[C.nm];
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C.total_tiles;
-         i += stride) {
-        const int c = C.tile_count[i];
-        const long long off = (long long)atomicAdd(&C.ctr->tile_used, (unsigned long long)c);
-        C.tile_off[i] = (int)off;
-        if (off + c > C.tile_cap) atomicOr(&C.ctr->overflow, 1u);
+    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+         item += stride) {
+        const int pos = find_pos(off, C.nm, item);
+        const int gt = C.mesh[C.order[pos]].tile_base + (int)(item - off[pos]);
+        const int c = C.tile_count[gt];
+        const long long at = (long long)atomicAdd(&C.ctr->tile_used, (unsigned long long)c);
+        C.tile_off[gt] = (int)at;
+        if (at + c > C.tile_cap) {
+            atomicOr(&C.ctr->overflow, 1u);  // reported; the list is not read
+            continue;
+        }
+        if (c == 0 || !C.seg) continue;
+        const int nseg = (c + kSegTris - 1) / kSegTris;
+        C.tile_left[gt] = nseg;
+        C.tile_flag[gt] = 0;
+        const long long s0 = (long long)atomicAdd(&C.ctr->seg_used, (unsigned long long)nseg);
+        for (int k = 0; k < nseg; ++k)
+            if (s0 + k < C.seg_cap) C.seg[s0 + k] = make_int2((int)item, k);
     }
 }
 
@@ -488,45 +513,45 @@ __device__ __forceinline__ int o3_naive_sign(const double* pa, const double* pb,
     return det > 0.0 ? 1 : (det < 0.0 ? -1 : 0);
 }
 
-// Fast occupancy pass: one 8x8 column tile per workgroup item, but the lanes work on
-// (column, triangle) pairs instead of columns, so that the FP64 predicates run on full
-// wavefronts (a triangle usually covers only a few of a tile's 64 columns). Each LDS
-// chunk of 64 triangles is expanded into the pairs of its triangles' column ranges
-// clipped to the tile (a wave prefix sum gives every pair a lane); a crossing flips its
-// column's "below K" bit with an atomic XOR. Every predicate is the naive FP64 sign: a
-// tile in which any of them is exactly 0 writes nothing and is queued for the exact pass
-// (occ_kernel), which settles zeros with the exact sign, nudges and replays as the
-// reference does.
+// Fast occupancy pass. A work item is a segment of up to kSegTris entries of one 8x8
+// column tile's triangle list, so that a tile under a dense patch (tens of thousands of
+// list entries) is shared by many workgroups. The lanes work on (column, triangle) pairs
+// instead of columns, so that the FP64 predicates run on full wavefronts (a triangle
+// usually covers only a few of a tile's 64 columns): each LDS chunk of 64 triangles is
+// expanded into the pairs of its triangles' column ranges clipped to the tile (a wave
+// prefix sum gives every pair a lane), and a crossing flips its column's "below K" bit
+// with an atomic XOR into the cleared masks. Every predicate is the naive FP64 sign. The
+// last segment of a tile to finish (a per-tile countdown) turns the crossing bits into
+// occupancy bits, or, if any segment met a determinant that is exactly 0, queues the tile
+// for the exact pass (occ_kernel), which settles zeros with the exact sign, nudges and
+// replays as the reference does.
 __global__ __launch_bounds__(64) void occ_pairs_kernel(Ctx C) {
     __shared__ double sv[9][kChunk];
     __shared__ int s_i0[kChunk], s_j0[kChunk], s_ni[kChunk];
     __shared__ int s_off[kChunk + 1];
     const long long* off = phase_off(C, kPhTile);
-    const long long total = off[C.nm];
+    const long long total = (long long)min(C.ctr->seg_used, (unsigned long long)C.seg_cap);
     const int lane = threadIdx.x;
     while (true) {
-        const long long item = next_item(&C.ctr->queue[0]);
-        if (item >= total) break;
+        const long long sitem = next_item(&C.ctr->queue[0]);
+        if (sitem >= total) break;
+        const int2 sg = C.seg[sitem];
+        const long long item = sg.x;
         const int pos = find_pos(off, C.nm, item);
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec g = M.g;
         const int tile = (int)(item - off[pos]);
         const int tx0 = (tile % M.ntx) * kTile, ty0 = (tile / M.ntx) * kTile;
-        const int ci = tx0 + (lane % kTile), cj = ty0 + (lane / kTile);
-        const bool valid = ci < g.nx && cj < g.ny;
         const int gt = M.tile_base + tile;
-        const long long beg = C.tile_off[gt];
-        const long long end = beg + C.tile_count[gt];
-        const bool ok = end <= C.tile_cap;  // overflowed lists are reported, not read
+        const long long tbeg = C.tile_off[gt];
+        const long long beg = tbeg + (long long)sg.y * kSegTris;
+        const long long end = min(tbeg + C.tile_count[gt], beg + kSegTris);
         const double* xyz = C.xyz + 3 * M.vbase;
         unsigned long long* const masks = C.masks + M.mask_base;
-        if (valid)
-            for (int w = 0; w < M.words; ++w) masks[((size_t)cj * g.nx + ci) * M.words + w] = 0ull;
-        __threadfence();  // the clears land before any lane's crossings
         const double zmin = g.corner[2] - g.spacing;
         const double zmax = g.corner[2] + (g.nz + 1) * g.spacing;
         bool unsure = false;
-        for (long long c0 = beg; ok && c0 < end; c0 += kChunk) {
+        for (long long c0 = beg; c0 < end; c0 += kChunk) {
             const int cnt = (int)min((long long)kChunk, end - c0);
             int npair = 0;
             __syncthreads();
@@ -591,12 +616,20 @@ __global__ __launch_bounds__(64) void occ_pairs_kernel(Ctx C) {
                 atomicXor(masks + ((size_t)j * g.nx + i) * M.words + (K >> 6), 1ull << (K & 63));
             }
         }
-        if (__syncthreads_or(unsure)) {
+        unsure = __syncthreads_or(unsure);
+        if (unsure && lane == 0) atomicOr(&C.tile_flag[gt], 1);
+        __threadfence();  // this segment's crossings and flag land before its countdown
+        int last = 0;
+        if (lane == 0) last = atomicSub(&C.tile_left[gt], 1) == 1;
+        if (!__shfl(last, 0)) continue;
+        __threadfence();  // every segment's crossings are visible (they live in L2)
+        if (__hip_atomic_load(&C.tile_flag[gt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             if (lane == 0) C.redo[atomicAdd(&C.ctr->redo_n, 1ull)] = (int)item;
             continue;
         }
-        __threadfence();  // every crossing has landed
-        if (valid && ok) finish_column<true>(masks + ((size_t)cj * g.nx + ci) * M.words, M.words);
+        const int ci = tx0 + (lane % kTile), cj = ty0 + (lane / kTile);
+        if (ci < g.nx && cj < g.ny)
+            finish_column<true>(masks + ((size_t)cj * g.nx + ci) * M.words, M.words);
     }
 }
 
@@ -1967,7 +2000,10 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     C.tile_count = I.dalloc<int>(tiles);
     C.tile_off = I.dalloc<int>(tiles);
     C.tile_cursor = I.dalloc<int>(tiles);
+    C.tile_left = I.dalloc<int>(tiles);
+    C.tile_flag = I.dalloc<int>(tiles);
     C.masks = I.dalloc<unsigned long long>(words);
+    C.total_words = words;
     C.redo = I.dalloc<int>(tiles);
     C.occ = I.dalloc<uint8_t>(cells);
     C.cell_tri = I.dalloc<int>(blocked);
@@ -2026,6 +2062,8 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     // planning pass: the tile-list sizes
     C.tile_cap = 0;
     C.tile_pool = nullptr;
+    C.seg = nullptr;
+    C.seg_cap = 0;
     C.sdf = nullptr;
     reset_kernel<<<I.grid256, 256, 0, st>>>(C);
     prep_kernel<<<I.grid256, 256, 0, st>>>(C);
@@ -2036,6 +2074,9 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
     CMG_HIP(hipStreamSynchronize(st));
     C.tile_cap = (long long)(h.tile_used + h.tile_used / 4 + 4096);
     C.tile_pool = I.dalloc<int>(C.tile_cap);
+    // a tile of c <= pool entries has ceil(c / kSegTris) segments
+    C.seg_cap = (long long)tiles + C.tile_cap / kSegTris + 1;
+    C.seg = I.dalloc<int2>(C.seg_cap);
 }
 
 BatchEngine::~BatchEngine() = default;

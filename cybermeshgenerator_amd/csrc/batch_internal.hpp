@@ -67,6 +67,7 @@ struct BatchCounters {
     unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 adjoint,
                                    // 3 tiles redone with exact predicates
     unsigned long long tile_used;  // tile-list pool cursor
+    unsigned long long seg_used;   // occupancy work items (tile list segments)
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
     // work of the voxel passes, counted per wavefront: box tests of tree nodes, box
     // tests of triangles, FP32 point-triangle distances

@@ -17,8 +17,13 @@ def _cases():
     hv, hf = W.harmonic_sphere(3, 0.05, 0.3, rng)
     tv, tf, _ = W.torus(0.3, 0.1, 64, 32)
     cp, ct = subdivided_cube(4)
+    # 12,288 triangles on a 10^3 grid: column tiles with thousands of list entries (the
+    # occupancy pass splits them among workgroups) and vertices on column lines (grazes,
+    # so whole split tiles are redone with exact predicates)
+    cq, tq = subdivided_cube(32)
     return [(c1.points, c1.tris, 64, 0), (c1s.points, c1s.tris, 40, 0),
-            (W.anchored(hv + 0.5), hf, 48, 0), (tv + 0.5, tf, 80, 2), (cp, ct, 16, 1)]
+            (W.anchored(hv + 0.5), hf, 48, 0), (tv + 0.5, tf, 80, 2), (cp, ct, 16, 1),
+            (cq, tq, 8, 1)]
 
 
 def test_batch_forward_equals_single_mesh_path(gpu):
