@@ -1372,15 +1372,19 @@ __global__ __launch_bounds__(256) void sdf_kernel(Ctx C) {
 // distance comes from; the closest point on it (barycentric weights: Ericson's regions in
 // FP32, or the reference's FP64 routine within one cell of the surface) is recomputed here
 // and the cell's contribution d(dist)/d(vertex) = -w_m (p - q) / d, times the upstream
-// gradient and the sign, is summed per triangle: over the wavefront (one group of lanes
-// per distinct triangle), then over the workgroup in an LDS table keyed by triangle, and
-// only then added to the vertices with global atomics, into the copy of the gradient of
+// gradient and the sign, is summed over the wavefront per triangle (one group of lanes per
+// distinct triangle), then over the workgroup per vertex in an LDS table keyed by point
+// (a vertex collects the sums of all the triangles around it: three atomics per point
+// where a per-triangle table flushed nine per triangle), and only then added to the
+// gradient with global atomics, into the copy of the gradient of
 // the XCD the workgroup runs on (xgrad_sum_kernel adds the copies). A workgroup takes
 // chunks of neighbouring blocks (neighbouring blocks share their nearest triangles: far
 // from a coarse mesh whole regions share one), so the same vertices are not hammered by
 // every wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
 constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2 (power of two)
-constexpr int kAdjSlots = 512;    // LDS table entries per workgroup (power of two)
+constexpr int kAdjSlots = 1024;   // LDS table entries (points) per workgroup
+constexpr int kAdjShift = 22;     // hash bits: the top log2(kAdjSlots)
+static_assert((1 << (32 - kAdjShift)) == kAdjSlots, "table size is a power of two");
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
 constexpr int kAdjGroup = 4;      // lanes sharing a triangle worth a wavefront sum
 
@@ -1390,23 +1394,15 @@ __device__ __forceinline__ int xcd_id() {
     return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));
 }
 
-// Adds the 9 partial derivatives v9 of triangle slot `tri` to its vertices in gx, the
-// copy of the gradient that belongs to the XCD the wave runs on. Each XCD adds into its
-// own copy, so atomics from different XCDs never meet on one address; the atomics are
-// device-scope, so the sum is right whichever copy a wave picks. The three vertex indices
-// come in one load before any atomic is issued (a load issued after atomics waits for
-// them: one memory counter covers both).
-__device__ __forceinline__ void add_tri_grad(const Ctx& C, int tri, const float* v9,
-                                             float* gx) {
-    const int4 tv = C.svtx[tri];
-    const int vs[3] = {tv.x, tv.y, tv.z};
-#pragma unroll
-    for (int vtx = 0; vtx < 3; ++vtx) {
-        float* gp = gx + 3 * (size_t)vs[vtx];
-        atomicAdd(gp + 0, v9[3 * vtx + 0]);
-        atomicAdd(gp + 1, v9[3 * vtx + 1]);
-        atomicAdd(gp + 2, v9[3 * vtx + 2]);
-    }
+// Adds the 3 partial derivatives v3 of point `vtx` (global index) to gx, the copy of the
+// gradient that belongs to the XCD the wave runs on. Each XCD adds into its own copy, so
+// atomics from different XCDs never meet on one address; the atomics are device-scope, so
+// the sum is right whichever copy a wave picks.
+__device__ __forceinline__ void add_vtx_grad(float* gx, int vtx, const float* v3) {
+    float* gp = gx + 3 * (size_t)vtx;
+    atomicAdd(gp + 0, v3[0]);
+    atomicAdd(gp + 1, v3[1]);
+    atomicAdd(gp + 2, v3[2]);
 }
 
 // grad = the sum of the per-XCD copies of the gradient, in XCD order (one pass after
@@ -1423,35 +1419,36 @@ __global__ __launch_bounds__(256) void xgrad_sum_kernel(const float* xg, long lo
     }
 }
 
-// Adds v9 into the workgroup table entry of triangle slot `tri` (or, when the table is
-// crowded, to the vertices directly); a newly taken entry is listed in used[].
-__device__ __forceinline__ void table_add(const Ctx& C, int tri, const float* v9, int* key,
-                                          float* val, int* used, int* nused, float* gx) {
-    const unsigned hsh = (unsigned)tri * 0x9E3779B1u;
+// Adds v3 into the workgroup table entry of point `vtx` (or, when the table is crowded,
+// to the gradient directly); a newly taken entry is listed in used[].
+__device__ __forceinline__ void table_add(int vtx, const float* v3, int* key, float* val,
+                                          int* used, int* nused, float* gx) {
+    const unsigned hsh = (unsigned)vtx * 0x9E3779B1u;
     for (int pr = 0; pr < kAdjProbe; ++pr) {
-        const int e = (int)((hsh >> 22) + (unsigned)pr) & (kAdjSlots - 1);
-        const int cur = atomicCAS(&key[e], -1, tri);
+        const int e = (int)((hsh >> kAdjShift) + (unsigned)pr) & (kAdjSlots - 1);
+        const int cur = atomicCAS(&key[e], -1, vtx);
         if (cur == -1) used[atomicAdd(nused, 1)] = e;
-        if (cur == -1 || cur == tri) {
+        if (cur == -1 || cur == vtx) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) atomicAdd(&val[e * 9 + q], v9[q]);
+            for (int q = 0; q < 3; ++q) atomicAdd(&val[e * 3 + q], v3[q]);
             return;
         }
     }
-    add_tri_grad(C, tri, v9, gx);
+    add_vtx_grad(gx, vtx, v3);
 }
 
 // Workgroups (4 wavefronts) take chunks of `chunk` neighbouring blocks from a work queue
 // and walk each chunk four blocks at a time (one per wavefront); the table is added to
-// the vertices and cleared whenever it is a quarter full (an iteration takes at most 256
-// entries, so the table never passes three quarters) and at the end. Longer chunks keep
-// more of a workgroup's cells in one region (fewer flushes of the same triangles); the
-// host sizes them so that every workgroup still gets several.
+// the gradient and cleared whenever it is a quarter full and at the end (an iteration
+// adds at most 3 x 256 entries: a table that fills up sends the rest straight to the
+// gradient, so the sum is right in any case). Longer chunks keep more of a workgroup's
+// cells in one region (fewer flushes of the same points); the host sizes them so that
+// every workgroup still gets several.
 constexpr int kAdjChunkMin = 16, kAdjChunkMax = 64;
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 void adjoint_kernel(Ctx C, int chunk) {  // register budget: 5 waves per SIMD (96 VGPRs)
     __shared__ int s_key[kAdjSlots];
-    __shared__ float s_val[kAdjSlots * 9];
+    __shared__ float s_val[kAdjSlots * 3];
     __shared__ int s_used[kAdjSlots];  // entries taken since the last flush
     __shared__ int s_nused;
     const long long* off = phase_off(C, kPhL0);
@@ -1461,7 +1458,7 @@ void adjoint_kernel(Ctx C, int chunk) {  // register budget: 5 waves per SIMD (9
     const bool scatter = C.grad != nullptr;
     float* const gx = scatter ? C.xgrad + (size_t)xcd_id() * C.xstride : nullptr;
     for (int e = threadIdx.x; e < kAdjSlots; e += blockDim.x) s_key[e] = -1;
-    for (int e = threadIdx.x; e < kAdjSlots * 9; e += blockDim.x) s_val[e] = 0.0f;
+    for (int e = threadIdx.x; e < kAdjSlots * 3; e += blockDim.x) s_val[e] = 0.0f;
     if (threadIdx.x == 0) s_nused = 0;
     __syncthreads();
     long long base = 0, last = 0;
@@ -1555,22 +1552,38 @@ void adjoint_kernel(Ctx C, int chunk) {  // register budget: 5 waves per SIMD (9
                 }
             }
             // large groups of lanes on one triangle (far from the surface) are summed
-            // across the wavefront first; the rest go into the table lane by lane
-            while (true) {
-                const unsigned long long mask = __ballot(act);
-                if (mask == 0) break;
-                const int leader = __ffsll((long long)mask) - 1;
+            // across the wavefront first; the rest go into the table lane by lane (every
+            // triangle of the wavefront is looked at once: a small group does not hide
+            // a large one behind it)
+            unsigned long long pending = __ballot(act);
+            while (pending != 0ull) {
+                const int leader = __ffsll((long long)pending) - 1;
                 const int lt = __builtin_amdgcn_readlane(tri, leader);
                 const bool match = act && tri == lt;
-                if (__popcll(__ballot(match)) < kAdjGroup) break;
+                const unsigned long long mm = __ballot(match);
+                pending &= ~mm;
+                if (__popcll(mm) < kAdjGroup) continue;
                 float sum[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) sum[e] = wave_sum_dpp(match ? c[e] : 0.0f);
-                if (lane == leader)
-                    table_add(C, lt, sum, s_key, s_val, s_used, &s_nused, gx);
+                // lanes 0, 1, 2 add the sums of the triangle's three vertices
+                const int4 tv = C.svtx[lt];
+                if (lane < 3) {
+                    const bool l0 = lane == 0, l1 = lane == 1;
+                    const float v3[3] = {l0 ? sum[0] : (l1 ? sum[3] : sum[6]),
+                                         l0 ? sum[1] : (l1 ? sum[4] : sum[7]),
+                                         l0 ? sum[2] : (l1 ? sum[5] : sum[8])};
+                    table_add(l0 ? tv.x : (l1 ? tv.y : tv.z), v3, s_key, s_val, s_used,
+                              &s_nused, gx);
+                }
                 act = act && !match;
             }
-            if (act) table_add(C, tri, c, s_key, s_val, s_used, &s_nused, gx);
+            if (act) {
+                const int4 tv = C.svtx[tri];
+                table_add(tv.x, c, s_key, s_val, s_used, &s_nused, gx);
+                table_add(tv.y, c + 3, s_key, s_val, s_used, &s_nused, gx);
+                table_add(tv.z, c + 6, s_key, s_val, s_used, &s_nused, gx);
+            }
         }
         if (!scatter) {
             if (done) break;
@@ -1585,10 +1598,10 @@ void adjoint_kernel(Ctx C, int chunk) {  // register budget: 5 waves per SIMD (9
         for (int u = threadIdx.x; u < nu; u += blockDim.x) {
             const int e = s_used[u];
             const int t = s_key[e];
-            add_tri_grad(C, t, &s_val[e * 9], gx);
+            add_vtx_grad(gx, t, &s_val[e * 3]);
             s_key[e] = -1;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) s_val[e * 9 + q] = 0.0f;
+            for (int q = 0; q < 3; ++q) s_val[e * 3 + q] = 0.0f;
         }
         __syncthreads();
         if (threadIdx.x == 0) s_nused = 0;
