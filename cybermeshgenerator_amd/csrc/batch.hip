@@ -1386,7 +1386,7 @@ constexpr int kAdjSlots = 1024;   // LDS table entries (points) per workgroup
 constexpr int kAdjShift = 22;     // hash bits: the top log2(kAdjSlots)
 static_assert((1 << (32 - kAdjShift)) == kAdjSlots, "table size is a power of two");
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
-constexpr int kAdjGroup = 4;      // lanes sharing a triangle worth a wavefront sum
+constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
 
 // The XCD this wave runs on: bits 3:0 of hardware register XCC_ID (register 20;
 // s_getreg operand (size - 1) << 11 | offset << 6 | id).
