@@ -68,7 +68,7 @@ using namespace occdev;
 
 namespace {
 
-constexpr int kSegTris = 8 * kChunk;  // tile-list entries per occupancy work item
+constexpr int kSegTris = 16 * kChunk;  // tile-list entries per occupancy work item
 
 // ---- work queue -----------------------------------------------------------------------
 
