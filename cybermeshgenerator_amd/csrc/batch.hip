@@ -622,7 +622,9 @@ __global__ __launch_bounds__(64) void occ_pairs_kernel(Ctx C) {
         int last = 0;
         if (lane == 0) last = atomicSub(&C.tile_left[gt], 1) == 1;
         if (!__shfl(last, 0)) continue;
-        __threadfence();  // every segment's crossings are visible (they live in L2)
+        // every segment's crossings are visible: they are device-scope atomics (segments
+        // of one tile may run on different XCDs), read back below with device-scope loads
+        __threadfence();
         if (__hip_atomic_load(&C.tile_flag[gt], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             if (lane == 0) C.redo[atomicAdd(&C.ctr->redo_n, 1ull)] = (int)item;
             continue;
