@@ -69,6 +69,13 @@ using namespace occdev;
 namespace {
 
 constexpr int kSegTris = 16 * kChunk;  // tile-list entries per occupancy work item
+constexpr int kXcds = 8;  // XCDs of an MI355X, each with its own L2 (power of two)
+
+// The XCD this wave runs on: bits 3:0 of hardware register XCC_ID (register 20;
+// s_getreg operand (size - 1) << 11 | offset << 6 | id).
+__device__ __forceinline__ int xcd_id() {
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));
+}
 
 // ---- work queue -----------------------------------------------------------------------
 
@@ -984,20 +991,49 @@ template <int VPL, int D>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(VPL == 1 ? 5 : (VPL == 2 ? 4 : 3)))) void vox_kernel(Ctx C) {
     __shared__ TriRec s_rec[D > 0 ? D : 1];
-    const long long* off = phase_off(C, VPL == 1 ? kPhL0 : (VPL == 2 ? kPhL0b : kPhL0c));
+    const long long* off = phase_off(C, VPL == 1 ? kPhV1 : (VPL == 2 ? kPhV2 : kPhV4));
     const long long total = off[C.nm];
     const int lane = threadIdx.x;
-    unsigned long long* q = &C.ctr->queue[1];
     unsigned long long nvis = 0, ntest = 0, neval = 0;
+    // Items run over super-tiles of 4 x 4 x SZ leaves (16^3 cells; super-tiles in x-fastest
+    // order per mesh, meshes in processing order). XCD x takes super-tiles x, x + 8,
+    // x + 16, ... from its own counter, so that the waves sharing one L2 walk one compact
+    // region at a time; a wave whose XCD's super-tiles are all taken helps the next XCD.
+    constexpr int SZ = kSuperZ / VPL;
+    constexpr int kPer = kSuper * kSuper * SZ;  // items per super-tile
+    const long long nst = (total + kPer - 1) / kPer;
+    int yoff = 0;  // the XCD counter this wave takes from: xcd_id() + yoff
+    const int x0 = xcd_id();
     while (true) {
-        const long long item = next_item(q);
-        if (item >= total) break;
+        long long item = -1;
+        while (yoff < kXcds) {
+            const int y = (x0 + yoff) & (kXcds - 1);
+            const long long mine = (nst - y + kXcds - 1) / kXcds;  // super-tiles of XCD y
+            unsigned long long k = 0;
+            if (lane == 0) k = atomicAdd(&C.ctr->xq[y], 1ull);
+            // wave-uniform (readfirstlane keeps it in scalar registers)
+            k = (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)k) |
+                ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(k >> 32)) << 32);
+            if ((long long)(k / kPer) < mine) {
+                item = ((long long)(k / kPer) * kXcds + y) * kPer + (long long)(k % kPer);
+                break;
+            }
+            ++yoff;
+        }
+        if (item < 0) break;
+        if (item >= total) continue;  // the last super-tile is only partly in range
         const int pos = find_pos(off, C.nm, item);
         const MeshDesc& M = C.mesh[C.order[pos]];
         const GridSpec& g = M.g;
         const Level L0 = VPL == 1 ? M.L0 : (VPL == 2 ? M.L0b : M.L0c);
-        const int b = (int)(item - off[pos]);
-        const int ix = b % L0.bx, iy = (b / L0.bx) % L0.by, iz = b / (L0.bx * L0.by);
+        const long long rel = item - off[pos];
+        const int sbx = (L0.bx + kSuper - 1) / kSuper, sby = (L0.by + kSuper - 1) / kSuper;
+        const long long st = rel / kPer;
+        const int w = (int)(rel % kPer);
+        const int ix = kSuper * (int)(st % sbx) + w % kSuper;
+        const int iy = kSuper * (int)((st / sbx) % sby) + (w / kSuper) % kSuper;
+        const int iz = SZ * (int)(st / ((long long)sbx * sby)) + w / (kSuper * kSuper);
+        if (ix >= L0.bx || iy >= L0.by || iz >= L0.bz) continue;  // past the grid's edge
         const int i = ix * kLeaf + (lane & 3), j = iy * kLeaf + ((lane >> 2) & 3);
         int k[VPL];
         bool valid[VPL];
@@ -1383,18 +1419,12 @@ __global__ __launch_bounds__(256) void sdf_kernel(Ctx C) {
 // chunks of neighbouring blocks (neighbouring blocks share their nearest triangles: far
 // from a coarse mesh whole regions share one), so the same vertices are not hammered by
 // every wavefront. When C.nearest is set, the mesh-local triangle of every cell is written too.
-constexpr int kXcds = 8;          // XCDs of an MI355X, each with its own L2 (power of two)
 constexpr int kAdjSlots = 1024;   // LDS table entries (points) per workgroup
 constexpr int kAdjShift = 22;     // hash bits: the top log2(kAdjSlots)
 static_assert((1 << (32 - kAdjShift)) == kAdjSlots, "table size is a power of two");
 constexpr int kAdjProbe = 16;     // probes before a contribution goes straight to HBM
 constexpr int kAdjGroup = 8;      // lanes sharing a triangle worth a wavefront sum
 
-// The XCD this wave runs on: bits 3:0 of hardware register XCC_ID (register 20;
-// s_getreg operand (size - 1) << 11 | offset << 6 | id).
-__device__ __forceinline__ int xcd_id() {
-    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kXcds - 1));
-}
 
 // Adds the 3 partial derivatives v3 of point `vtx` (global index) to gx, the copy of the
 // gradient that belongs to the XCD the wave runs on. Each XCD adds into its own copy, so
@@ -1977,6 +2007,13 @@ BatchEngine::BatchEngine(int nm, const long long* pt_start, const long long* tri
             if (ph == kPhL0) n = (long long)M.L0.bx * M.L0.by * M.L0.bz;
             if (ph == kPhL0b) n = (long long)M.L0b.bx * M.L0b.by * M.L0b.bz;
             if (ph == kPhL0c) n = (long long)M.L0c.bx * M.L0c.by * M.L0c.bz;
+            if (ph == kPhV1 || ph == kPhV2 || ph == kPhV4) {
+                const int vpl = ph == kPhV1 ? 1 : (ph == kPhV2 ? 2 : 4);
+                const Level& L = vpl == 1 ? M.L0 : (vpl == 2 ? M.L0b : M.L0c);
+                const int sz = kSuperZ / vpl;
+                n = (long long)((L.bx + kSuper - 1) / kSuper) * ((L.by + kSuper - 1) / kSuper) *
+                    ((L.bz + sz - 1) / sz) * (kSuper * kSuper * sz);
+            }
             if (ph == kPhRow)
                 n = (long long)((M.L0.bx + kRowRun - 1) / kRowRun) * M.L0.by * M.L0.bz;
             if (ph == kPhOccRow) n = (long long)M.g.ny * M.words * ((M.g.nx + 63) / 64);

@@ -59,9 +59,14 @@ struct MeshDesc {
 // Work items per mesh: column tiles, voxel leaves (4x4x4, 4x4x8, 4x4x16), runs of up to
 // 64 4x4x4 leaves along x (the SDF pass's rows), and grid rows (the occupancy bytes).
 enum BatchPhase {
-    kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kPhRow = 4, kPhOccRow = 5, kNumPhase = 6
+    kPhTile = 0, kPhL0 = 1, kPhL0b = 2, kPhL0c = 3, kPhRow = 4, kPhOccRow = 5,
+    kPhV1 = 6, kPhV2 = 7, kPhV4 = 8,  // voxel-pass items (1, 2, 4 voxels per lane), by super-tile
+    kNumPhase = 9
 };
 constexpr int kRowRun = 64;  // 4x4x4 leaves per SDF-pass item
+// Voxel-pass super-tiles: kSuper x kSuper leaves in x and y, 16 cells in z (kSuperZ / vpl
+// leaves of 4 vpl cells); their items cover the grid's edge too and are skipped there.
+constexpr int kSuper = 4, kSuperZ = 4;
 
 struct BatchCounters {
     unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 adjoint,
@@ -69,6 +74,7 @@ struct BatchCounters {
     unsigned long long tile_used;  // tile-list pool cursor
     unsigned long long seg_used;   // occupancy work items (tile list segments)
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
+    unsigned long long xq[8];      // voxel pass: items taken from each XCD's super-tiles
     // work of the voxel passes, counted per wavefront: box tests of tree nodes, box
     // tests of triangles, FP32 point-triangle distances
     unsigned long long node_tests, tri_tests, tri_evals;
