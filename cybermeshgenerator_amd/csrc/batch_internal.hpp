@@ -69,12 +69,13 @@ constexpr int kRowRun = 64;  // 4x4x4 leaves per SDF-pass item
 constexpr int kSuper = 4, kSuperZ = 4;
 
 struct BatchCounters {
-    unsigned long long queue[4];   // work queues: 0 tiles, 1 forward voxels, 2 adjoint,
-                                   // 3 tiles redone with exact predicates
+    unsigned long long queue[4];   // work queues: 0 tile-list segments, 3 tiles redone
+                                   // with exact predicates (1, 2 unused)
     unsigned long long tile_used;  // tile-list pool cursor
     unsigned long long seg_used;   // occupancy work items (tile list segments)
     unsigned long long redo_n;     // tiles queued for the exact occupancy pass
-    unsigned long long xq[8];      // voxel pass: items taken from each XCD's super-tiles
+    unsigned long long xq[8];      // voxel pass: items taken from each XCD's range
+    unsigned long long aq[8];      // adjoint: chunks taken from each XCD's range
     // work of the voxel passes, counted per wavefront: box tests of tree nodes, box
     // tests of triangles, FP32 point-triangle distances
     unsigned long long node_tests, tri_tests, tri_evals;

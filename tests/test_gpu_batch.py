@@ -158,9 +158,17 @@ def test_variants_agree_and_autotune_picks_one(gpu):
             assert torch.equal(sdf, ref_sdf), (vpl, depth)
             rel = float((grad - ref_grad).norm() / ref_grad.norm())
             assert rel < 1e-5, (vpl, depth, rel)
+    b.set_variant(1, 8)
+    b.step(gs, graph=False)
+    torch.cuda.synchronize()
+    one = b.counters()["node_tests"]
     tune = b.autotune(gs)
     assert (tune["vpl"], tune["depth"]) in {(v, d) for v in (1, 2, 4) for d in (0, 8)}
     assert all(ms > 0 for ms in tune["ms"].values()), tune
+    # every timed run walks the voxels again (its work queue is reset before each run):
+    # the counters, which only a forward's reset clears, hold the autotuner's forward plus
+    # 6 variants x 3 runs of the voxel pass, well over 6 forwards' worth
+    assert b.counters()["node_tests"] > 6 * one, (b.counters()["node_tests"], one)
     sdf, _ = b.step(gs, graph=True)
     torch.cuda.synchronize()
     assert torch.equal(sdf, ref_sdf)
