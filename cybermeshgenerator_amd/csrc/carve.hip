@@ -343,7 +343,7 @@ __global__ __launch_bounds__(256) void rays_pass_kernel(const double* qpts, long
     const long long m = n - *nbad;  // valid queries: each has one pair per grid
     const long long base = K * m;
     const RayDir& G = P.k[K];
-    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m;
+    for (long long j = xcd_block() * blockDim.x + threadIdx.x; j < m;
          j += (long long)gridDim.x * blockDim.x) {
         const unsigned i = sval[base + j];
         const unsigned c = skey[base + j];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void parity_pass_kernel(const double* qpts, lo
     const double zmin = g.corner[2] - g.spacing;
     const double zmax = g.corner[2] + (g.nz + 1) * g.spacing;
     const long long m = n - *nbad;
-    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m;
+    for (long long j = xcd_block() * blockDim.x + threadIdx.x; j < m;
          j += (long long)gridDim.x * blockDim.x) {
         const unsigned i = sval[j];
         const double x = qpts[3 * (long long)i], y = qpts[3 * (long long)i + 1],
@@ -737,8 +737,9 @@ void classify(const CarveEngine::Impl& I, const double* d_q, const int* d_tets, 
     const double* qpts = TETS ? qbuf : d_q;
     // 20 waves per CU: the ray passes' registers allow 5 per SIMD to hide their dependent
     // loads; the parity pass (2 per SIMD) keeps a grid of 8 per CU, measured faster
-    const unsigned nb = grid_for((size_t)n, 256, 20);
-    const unsigned nb_parity = grid_for((size_t)n, 256, 8);
+    // (multiples of 8: the passes renumber their workgroups by XCD, xcd_block)
+    const unsigned nb = (grid_for((size_t)n, 256, 20) + 7u) & ~7u;
+    const unsigned nb_parity = (grid_for((size_t)n, 256, 8) + 7u) & ~7u;
     {
         TimedLaunch tl("carve_query", st);
         CMG_HIP(hipMemsetAsync(nbad, 0, sizeof(unsigned), st));

@@ -50,6 +50,17 @@ inline unsigned grid_for(size_t work, unsigned block, unsigned waves_per_cu = 8)
     return g ? (unsigned)g : 1u;
 }
 
+// Workgroup index for grid-stride loops whose neighbouring items share data (queries
+// sorted by cell): workgroups are dispatched round-robin over the 8 XCDs (workgroup b to
+// XCD b % 8), each XCD with its own L2, so b is renumbered to (b % 8) * (G / 8) + b / 8
+// when the grid G is a multiple of 8. In every round of the stride the workgroups of one
+// XCD then cover one contiguous range of items instead of every eighth block of 256. A
+// bijection of [0, G), so every item is still visited exactly once.
+__device__ inline long long xcd_block() {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    return (G % 8u) ? (long long)b : (long long)(b % 8u) * (G / 8u) + b / 8u;
+}
+
 // Growable device scratch owned by the library, one per device. Calls that use it
 // serialise on its mutex and synchronise their stream before releasing it.
 class Scratch {
