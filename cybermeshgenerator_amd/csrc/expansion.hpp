@@ -713,6 +713,68 @@ __host__ __device__ inline bool orient3d_sign_fixed(const double* pa, const doub
     return true;
 }
 
+// Exact orient3d sign when the nine differences p - pd are exact in FP64 and integer
+// multiples of 2^(E-31), where 2^(E-1) <= max |difference| < 2^E <= 2^31: then
+// D = difference * 2^(31-E) is an integer below 2^31 in magnitude (the scaling is exact:
+// by 2^(31-E) >= 1, no overflow), the minors bdx*cdy - cdx*bdy (two products below 2^62)
+// fit 64 bits, and det = adz*m1 + bdz*m2 + cdz*m3 (three products below 2^94) fits 128
+// bits, each product built from 32 x 32-bit ones (12 in all). Decides the exact zeros
+// and the small determinants of tuples on a common grid, whatever their sign, in a few
+// hundred instructions. False when a difference is inexact, the differences span more
+// than 31 bits, or max |difference| lies outside [2^-260, 2^31) (the lower limit keeps
+// every nonzero difference above 2^-291, inside the range of the other stages).
+__host__ __device__ inline bool orient3d_sign_fixed31(const double* pa, const double* pb,
+                                                      const double* pc, const double* pd,
+                                                      int& sign) {
+    double d[9];
+    double amax = 0.0;
+    bool exact = true;
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        const double* p = v == 0 ? pa : (v == 1 ? pb : pc);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            double x, y;
+            two_sum(p[a], -pd[a], x, y);
+            exact = exact && y == 0.0;
+            d[3 * v + a] = x;
+            amax = __builtin_fmax(amax, __builtin_fabs(x));
+        }
+    }
+    if (!exact) return false;
+    if (amax == 0.0) {  // four equal points
+        sign = 0;
+        return true;
+    }
+    if (!(amax >= 0x1p-260 && amax < 0x1p31)) return false;
+    int E = 0;
+    (void)__builtin_frexp(amax, &E);  // amax in [2^(E-1), 2^E), E <= 31
+    const double scale = __builtin_ldexp(1.0, 31 - E);
+    int D[9];
+    bool ints = true;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const double t = d[i] * scale;  // exact, |t| < 2^31
+        ints = ints && t == __builtin_rint(t);
+        D[i] = (int)t;
+    }
+    if (!ints) return false;
+    const int adx = D[0], ady = D[1], adz = D[2], bdx = D[3], bdy = D[4], bdz = D[5],
+              cdx = D[6], cdy = D[7], cdz = D[8];
+    const long long m1 = (long long)bdx * cdy - (long long)cdx * bdy;
+    const long long m2 = (long long)cdx * ady - (long long)adx * cdy;
+    const long long m3 = (long long)adx * bdy - (long long)bdx * ady;
+    auto zm = [](int z, long long m) -> __int128 {  // z * m, |z| < 2^31, |m| < 2^63
+        const unsigned az = (unsigned)(z < 0 ? -z : z);
+        const unsigned long long am = m < 0 ? 0ull - (unsigned long long)m : (unsigned long long)m;
+        const __int128 p = (__int128)mul_u32_u64(az, am);
+        return ((z < 0) != (m < 0)) ? -p : p;
+    };
+    const __int128 det = zm(adz, m1) + zm(bdz, m2) + zm(cdz, m3);
+    sign = det == 0 ? 0 : (det < 0 ? -1 : 1);
+    return true;
+}
+
 // ---- wide fixed point: the same evaluation for differences spanning up to 126 bits ----
 
 // r = a * b (unsigned, little-endian 64-bit limbs)

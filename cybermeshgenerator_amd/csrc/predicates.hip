@@ -169,17 +169,20 @@ __device__ __forceinline__ int decide(const double* v, int& stage) {
 }
 
 // Register-only exact stage run inside the streaming kernels for the lanes the filters
-// leave undecided. orient3d: an exact sum of the triple products when the coordinate
-// differences are exact, else double-word arithmetic with a rigorous bound; the lanes it
-// cannot decide (in practice exact zeros with inexact differences) are queued for the
-// exact pass, whose fixed-point stages resolve them. insphere: fixed point with 62-bit
-// differences, else the double-word bound; the rest is queued for the 126-bit stage. (A
-// cheaper first stage for narrow tuples does not pay here: a wavefront runs every stage
-// any of its lanes needs, and the drained lanes mix narrow and wide tuples.)
+// leave undecided. orient3d: fixed point when the coordinate differences are exact and
+// span at most 31 bits (exact zeros and small determinants on a common grid; 12 integer
+// products, where the exact sum of the 24 FMA-split triple-product terms took several
+// error-free passes), else double-word arithmetic with a rigorous bound (nonzero
+// determinants); the lanes neither decides (in practice exact zeros with inexact or wider
+// differences) are queued for the exact pass, whose 62- and 126-bit fixed-point stages
+// resolve them. insphere: fixed point with 62-bit differences, else the double-word
+// bound; the rest is queued for the 126-bit stage. (For insphere a cheaper 31-bit first
+// stage does not pay: a wavefront runs every stage any of its lanes needs, the drained
+// lanes mix narrow and wide tuples, and the extra live registers spill.)
 template <int NPTS>
 __device__ __forceinline__ bool decide_inline_exact(const double* v, int& s) {
     if constexpr (NPTS == 4) {
-        if (orient3d_sign_exact_diffs(v, v + 3, v + 6, v + 9, s)) return true;
+        if (orient3d_sign_fixed31(v, v + 3, v + 6, v + 9, s)) return true;
         return orient3d_sign_dw(v, v + 3, v + 6, v + 9, s);
     } else {
         return insphere_sign_fixed(v, v + 3, v + 6, v + 9, v + 12, s) ||
@@ -345,6 +348,18 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
         n64 += dec64;
         wave_push(has && !dec && !dec64, i, seg, off);
     };
+    // orient3d: the next iteration's planes are loaded before this iteration's tuples are
+    // decided and the ring drained, so the wave's loads stay in flight while it computes
+    // (48 more VGPRs; insphere's kernel has no room for its 60)
+    constexpr bool PREFETCH = VEC && NPTS == 4;
+    auto full_pair = [&](size_t q) { return q < npairs && 2 * q + 1 < n; };
+    d2v nxt[PREFETCH ? NPL : 1];
+    if (PREFETCH && iters > 0 && full_pair(tid)) {
+#pragma unroll
+        for (int q = 0; q < NPL; ++q)
+            nxt[q] = __builtin_nontemporal_load(
+                reinterpret_cast<const d2v*>(soa + (size_t)q * ld + 2 * tid));
+    }
     for (unsigned it = 0; it < iters; ++it) {
         const size_t p = tid + (size_t)it * stride;
         const bool valid = p < npairs;
@@ -352,8 +367,26 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
         const bool two = valid && i0 + 1 < n;
         bool u0 = false, u1 = false;
         double v0[NPL], v1[NPL];
+        d2v cur[PREFETCH ? NPL : 1];
+        if (PREFETCH) {
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) cur[q] = nxt[q];
+            const size_t pn = p + stride;
+            if (it + 1 < iters && full_pair(pn)) {
+#pragma unroll
+                for (int q = 0; q < NPL; ++q)
+                    nxt[q] = __builtin_nontemporal_load(
+                        reinterpret_cast<const d2v*>(soa + (size_t)q * ld + 2 * pn));
+            }
+        }
         if (valid) {
-            if (VEC && two) {
+            if (PREFETCH && two) {
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) {
+                    v0[q] = cur[q].x;
+                    v1[q] = cur[q].y;
+                }
+            } else if (VEC && two) {
 #pragma unroll
                 for (int q = 0; q < NPL; ++q) {
                     d2v x = __builtin_nontemporal_load(

@@ -3,9 +3,10 @@
 // Usage: register_stages insphere|orient3d <in: n tuples of float64> <out: n x 8 int8>
 // insphere tuples are pa pb pc pd pe (15 doubles); per tuple, for each stage in turn
 // (fixed point with 62-bit differences, double-word bound, fixed point with 126-bit
-// differences, fixed point with 31-bit differences): decided (0/1), sign. orient3d tuples are pa pb pc pd (12 doubles); the
-// stages are the exact-difference sum, the double-word bound and fixed point with 62- and
-// 126-bit differences. Unused slots read 0.
+// differences, fixed point with 31-bit differences): decided (0/1), sign. orient3d tuples
+// are pa pb pc pd (12 doubles); the stages are fixed point with 31-bit differences, the
+// double-word bound (the streaming kernel's ring drain, in that order) and fixed point
+// with 62- and 126-bit differences (the exact pass). Unused slots read 0.
 // Modes orient3d-exp and insphere-exp replay the exact pass instead: fixed point with 62-
 // and 126-bit differences, then the expansion arithmetic with the first pass's small
 // capacities and with the wide re-run's capacities (decided 0 = capacity exceeded).
@@ -58,7 +59,7 @@ int main(int argc, char** argv) {
             ok[2] = cmg::hip::insphere_sign_fixed_wide(t, t + 3, t + 6, t + 9, t + 12, s[2]);
             ok[3] = cmg::hip::insphere_sign_fixed31(t, t + 3, t + 6, t + 9, t + 12, s[3]);
         } else {
-            ok[0] = cmg::hip::orient3d_sign_exact_diffs(t, t + 3, t + 6, t + 9, s[0]);
+            ok[0] = cmg::hip::orient3d_sign_fixed31(t, t + 3, t + 6, t + 9, s[0]);
             ok[1] = cmg::hip::orient3d_sign_dw(t, t + 3, t + 6, t + 9, s[1]);
             ok[2] = cmg::hip::orient3d_sign_fixed(t, t + 3, t + 6, t + 9, s[2]);
             ok[3] = cmg::hip::orient3d_sign_fixed_wide(t, t + 3, t + 6, t + 9, s[3]);

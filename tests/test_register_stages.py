@@ -211,16 +211,21 @@ def _run_o3(tuples, tmp_path):
 
 
 def test_orient3d_stages_against_oracle(tmp_path):
-    # orient3d: the exact-difference sum and the double-word bound (inside the streaming
-    # kernel) and fixed point (exact pass); each agrees with the oracle wherever it
-    # decides, and together they decide every workload tuple, including the exact zeros
-    # with inexact differences (a point moved by 1 ulp onto another)
+    # orient3d: fixed point with 31-bit differences and the double-word bound (inside the
+    # streaming kernel) and fixed point with 62- and 126-bit differences (exact pass); each
+    # agrees with the oracle wherever it decides, and together they decide every workload
+    # tuple, including the exact zeros with inexact differences (a point moved by 1 ulp
+    # onto another). The 31-bit stage decides every exactly coplanar workload tuple (their
+    # differences lie on the 2^-28 grid below 2^3), the double-word bound every 1-ulp one
+    # with a nonzero determinant.
     soa, kind = W.tuples_orient3d(1 << 18, seed=2)
     t = soa.T.copy()
     ok, sign = _run_o3(t, tmp_path)
     want = oracle.orient3d_batch(soa, oracle.EXACT).astype(np.int64)
     for k in range(4):
         assert np.array_equal(sign[ok[:, k], k], want[ok[:, k]]), k
+    assert ok[kind == 1, 0].all() and (want[kind == 1] == 0).all()
+    assert ok[(kind == 2) & (want != 0), 1].all()
     left = ~ok[:, 0] & ~ok[:, 1]
     assert left.sum() > 0 and (want[left] == 0).all()
     assert ok[left, 2].mean() > 0.9              # the rest span more than 62 bits
@@ -252,6 +257,55 @@ def test_orient3d_stages_against_oracle(tmp_path):
     want = oracle.orient3d_batch(np.ascontiguousarray(t.T), oracle.EXACT).astype(np.int64)
     assert np.array_equal(sign[ok[:, 3], 3], want[ok[:, 3]])
     assert ok[:, 3].mean() > 0.95 and (want[ok[:, 3]] == 0).any()
+
+
+def test_orient3d_fixed31_limits(tmp_path):
+    """The 31-bit orient3d stage (the ring drain's first, slot 0): with the oracle's exact
+    sign wherever it decides; decides every tuple whose differences are exact, lie on one
+    grid and stay below 2^31 grid steps, at any power-of-two scale in range (exact zeros
+    from repeated and coplanar points included); declines inexact differences, spreads
+    beyond 31 bits and largest differences outside [2^-260, 2^31)."""
+    rng = np.random.default_rng(12)
+    rows, expect = [], []
+
+    def add(q, decided):
+        rows.append(q)
+        expect.append(np.full(len(q), decided))
+
+    for bits in (3, 20, 28):
+        for scale in (-240, -40, 0, 1):
+            q = rng.integers(-(2 ** bits), 2 ** bits, size=(60, 12)).astype(np.float64)
+            q[:10, 3:6] = q[:10, 0:3]                                  # repeated point
+            q[10:20, 9:12] = q[10:20, 3:6] + q[10:20, 6:9] - q[10:20, 0:3]  # coplanar
+            add(np.ldexp(q, scale), True)
+    # at the width limit: largest difference 2^31 - 1 grid steps, smallest 1
+    lim = np.zeros((4, 12))
+    lim[:, 0] = 2.0 ** 31 - 1
+    lim[:, 4] = 1.0
+    lim[:, 8] = rng.integers(1, 2 ** 20, size=4)
+    lim[1:, 9] = [1.0, 0.0, 3.0]
+    add(lim, True)
+    wide = lim.copy()
+    wide[:, 0] = 2.0 ** 31 + wide[:, 9]                                # 32 bits
+    add(wide, False)
+    # 41-bit spread with a small largest difference: not on a 31-bit grid
+    q = rng.integers(2 ** 39, 2 ** 40, size=(20, 12)).astype(np.float64)
+    q[:, 1] += 1.0
+    add(np.ldexp(q, -20), False)
+    # inexact difference: 1 - 2^-60
+    q = rng.integers(-4, 5, size=(20, 12)).astype(np.float64)
+    q[:, 0] = 1.0
+    q[:, 9] = 2.0 ** -60
+    add(q, False)
+    # below range: every difference under 2^-260
+    add(np.ldexp(rng.integers(-8, 9, size=(20, 12)).astype(np.float64), -275), False)
+    t = np.concatenate(rows)
+    expect = np.concatenate(expect)
+    ok, sign = _run_o3(t, tmp_path)
+    want = oracle.orient3d_batch(np.ascontiguousarray(t.T), oracle.EXACT).astype(np.int64)
+    assert np.array_equal(sign[ok[:, 0], 0], want[ok[:, 0]])
+    assert np.array_equal(ok[:, 0], expect)
+    assert (want[expect] == 0).sum() >= 200 and (want[expect] != 0).sum() >= 200
 
 
 def _run_exp(mode, tuples, tmp_path):
