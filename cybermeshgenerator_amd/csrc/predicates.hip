@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void filter_soa_kernel(const double* __restric
     // (48 more VGPRs; insphere's kernel has no room for its 60)
     constexpr bool PREFETCH = VEC && NPTS == 4;
     auto full_pair = [&](size_t q) { return q < npairs && 2 * q + 1 < n; };
-    d2v nxt[PREFETCH ? NPL : 1];
+    d2v nxt[PREFETCH ? NPL : 1] = {};  // lanes without a full pair never read it
     if (PREFETCH && iters > 0 && full_pair(tid)) {
 #pragma unroll
         for (int q = 0; q < NPL; ++q)
